@@ -1,0 +1,146 @@
+/*
+ * xb_capi.h -- C ABI of the MI355X multi-source BFS engine (libxbmsbfs.so).
+ *
+ * Every entry point returns an int32_t status (XB_OK == 0).  The message of
+ * the last failure on the calling thread is available from xb_last_error().
+ * Callers own every input and output buffer; the library owns the device copy
+ * of the graph and its scratch memory.  A graph handle may be used by one
+ * host thread at a time; distinct handles may run concurrently (each has its
+ * own HIP stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   xb_graph_create_csr / xb_graph_load_text
+ *       zblossom::HGraph(row_path, col_path)
+ *         apps/xb/src/include/graph/hgraph_gpu.h:37-48
+ *       ReadFileIntoHVector  apps/xb/src/include/common/utils_gpu.h:9-25
+ *       zblossom::DGraph(const HGraph&)  apps/xb/src/include/graph/dgraph.h:50-63
+ *   xb_msbfs_run
+ *       the search step of XBlossomGpuEngine10: source seeding
+ *         apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:83-185,
+ *       the level loop :72-81 and the expand step :270-359, with the path
+ *       table write of apps/xb/src/include/path_table/path_table.h:256-271
+ *       turned into per-source level/parent tables.
+ *   xb_graph_destroy   ~HGraph / ~DGraph (RAII in the reference)
+ *   xb_last_error      replaces CUDA_CHECK's terminate
+ *                      (apps/xb/src/include/utils/utils_gpu.h:54-63)
+ */
+#ifndef XB_CAPI_H
+#define XB_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define XB_OK 0
+#define XB_ERR_INVALID_ARG (-1)   /* null pointer, negative size, bad option */
+#define XB_ERR_BAD_CSR (-2)       /* non-monotone offsets, ids out of range   */
+#define XB_ERR_IO (-3)            /* unreadable or malformed text file        */
+#define XB_ERR_NOMEM (-4)         /* host or device allocation failed         */
+#define XB_ERR_DEVICE (-5)        /* HIP runtime error                        */
+#define XB_ERR_SOURCE_RANGE (-6)  /* a source id is outside [0, n)            */
+#define XB_ERR_BUDGET (-7)        /* request exceeds the HBM admission budget */
+#define XB_ERR_UNSUPPORTED (-8)   /* option combination not supported         */
+#define XB_ERR_NO_DEVICE (-9)     /* no HIP device visible                    */
+
+/* ---- graph flags ------------------------------------------------------ */
+#define XB_GRAPH_SYMMETRIC 0u /* undirected: the CSR holds both directions */
+#define XB_GRAPH_DIRECTED 1u  /* directed: the transpose is built on load  */
+
+/* ---- run modes -------------------------------------------------------- */
+#define XB_MODE_AUTO 0    /* batched bit-parallel engine (falls back to plain) */
+#define XB_MODE_PLAIN 1   /* one top-down push BFS launch sequence per source  */
+#define XB_MODE_BATCHED 2 /* bit-parallel, 64 sources per 64-bit word          */
+
+/* ---- run flags -------------------------------------------------------- */
+#define XB_OUT_DEVICE 1u  /* levels/parents are device pointers (else host)  */
+#define XB_PROFILE 2u     /* time every kernel class with HIP events         */
+#define XB_NO_LEVELS 4u   /* levels == NULL allowed: only parents written     */
+
+typedef struct xb_graph xb_graph;
+
+/* Tunables of the batched engine.  Zero means "use the default". */
+typedef struct xb_msbfs_opts {
+  int32_t mode;            /* XB_MODE_*                                        */
+  uint32_t flags;          /* XB_OUT_DEVICE | XB_PROFILE | ...                 */
+  int32_t words_per_tile;  /* 64-bit source words traversed together: 1,2,4,8 */
+  int32_t alpha;           /* pull when |F| + outdeg(F) > m / alpha            */
+  int32_t pull_heavy_cut;  /* in-degree above which pull uses a wave/vertex    */
+  int32_t emit_heavy_cut;  /* in-degree above which emit uses a wave/vertex    */
+  int32_t reserved[8];
+} xb_msbfs_opts;
+
+/* Per-run statistics (filled when the pointer is non-null). */
+typedef struct xb_run_stats {
+  double ms_total;         /* HIP-event time of the whole run on its stream   */
+  double ms_init;          /* tile set-up (state clears, source seeding)      */
+  double ms_push;          /* top-down push + commit kernels                  */
+  double ms_pull;          /* bottom-up pull kernels                          */
+  double ms_emit;          /* level/parent table materialisation              */
+  int64_t launches_push;
+  int64_t launches_pull;
+  int64_t launches_emit;
+  int32_t max_level;       /* deepest level reached over all sources          */
+  int32_t tiles;           /* number of source tiles run                      */
+  int32_t push_levels;
+  int32_t pull_levels;
+  int32_t fallback_sources;/* sources re-run on the plain path (depth > 254)   */
+  int32_t reserved[7];
+} xb_run_stats;
+
+/* Fills *o with the defaults. */
+int32_t xb_opts_default(xb_msbfs_opts* o);
+
+/* Copies a CSR (row_offsets int32[n+1], col_indices int32[m]) to the device.
+ * Validates monotone offsets, row_offsets[0] == 0, row_offsets[n] == m and
+ * ids in [0, n).  Adjacency lists are sorted ascending on ingest if needed
+ * (the lowest-id parent rule relies on it); duplicate edges are kept. */
+int32_t xb_graph_create_csr(const int32_t* row_offsets, int64_t n_plus_1,
+                            const int32_t* col_indices, int64_t m,
+                            uint32_t flags, xb_graph** out);
+
+/* Reads the two whitespace-separated integer files the reference's
+ * ReadFileIntoHVector reads, then behaves as xb_graph_create_csr. */
+int32_t xb_graph_load_text(const char* row_offsets_path,
+                           const char* col_indices_path, uint32_t flags,
+                           xb_graph** out);
+
+int32_t xb_graph_info(const xb_graph* g, int64_t* n, int64_t* m,
+                      uint32_t* flags);
+
+int32_t xb_graph_destroy(xb_graph* g);
+
+/* Runs S single-source BFS traversals.  levels and parents are S x n int32,
+ * row s = sources[s]:
+ *   levels[s][v]  = hop distance, -1 if unreached;
+ *   parents[s][v] = lowest-id u with (u->v) in E and levels[s][u] ==
+ *                   levels[s][v]-1; parents[s][sources[s]] = sources[s];
+ *                   -1 if unreached.
+ * Results are identical for every mode and option setting. */
+int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
+                     int32_t* levels, int32_t* parents,
+                     const xb_msbfs_opts* opts, xb_run_stats* stats);
+
+/* Sum over sources of the out-degrees of the vertices each source reaches
+ * (the traversed-edge count of a full top-down BFS), computed on the device
+ * from a levels table already in device memory. */
+int32_t xb_count_traversed(xb_graph* g, const int32_t* levels_dev, int32_t S,
+                           int64_t* edges_out);
+
+/* Device memory helpers for callers without their own allocator. */
+int32_t xb_device_alloc(size_t bytes, void** ptr);
+int32_t xb_device_free(void* ptr);
+int32_t xb_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes);
+
+/* Copies the last error message of the calling thread into buf. */
+int32_t xb_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XB_CAPI_H */

@@ -1,0 +1,107 @@
+"""GPU parity: the HIP engine through the C ABI against the CPU oracle.
+
+Bar: bit-exact levels and parents (integer work).  Cases cover both modes,
+every words-per-tile value, forced push-only / pull-only direction choices,
+heavy-vertex kernels (tiny cuts), directed graphs, partial tiles, duplicate
+and isolated sources, and depth beyond the 8-bit level code (plain fallback).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import xbmsbfs as xb
+
+pytestmark = pytest.mark.gpu
+
+
+def _path_graph(n):
+    ro = np.zeros(n + 1, dtype=np.int32)
+    cols = []
+    for v in range(n):
+        nb = [u for u in (v - 1, v + 1) if 0 <= u < n]
+        cols.extend(nb)
+        ro[v + 1] = ro[v] + len(nb)
+    return xb.Csr(ro, np.array(cols, dtype=np.int32))
+
+
+@pytest.fixture(scope="module")
+def rmat12():
+    g = xb.gen_rmat(12, 8, 0.57, 0.19, 0.19, seed=11)
+    return g, xb.HGraph(g.row_offsets, g.col_indices)
+
+
+def _check(g, hg, src, **kw):
+    lv_ref, pa_ref, _ = oracle.msbfs(g.row_offsets, g.col_indices, src, threads=8)
+    eng = xb.XBlossomGpuEngine10(hg)
+    lv, pa = eng.MultiSourceBFS(src, **kw)
+    assert np.array_equal(lv, lv_ref), f"levels differ in {(lv != lv_ref).sum()} entries ({kw})"
+    assert np.array_equal(pa, pa_ref), f"parents differ in {(pa != pa_ref).sum()} entries ({kw})"
+    return eng.last_stats
+
+
+@pytest.mark.parametrize("T", [1, 2, 4, 8])
+@pytest.mark.parametrize("alpha", [1, 20, 1_000_000_000])
+def test_batched_rmat(rmat12, T, alpha):
+    g, hg = rmat12
+    src = xb.sample_sources(g, 150, seed=3)
+    st = _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, alpha=alpha)
+    assert st.tiles == -(-150 // (64 * T))
+
+
+@pytest.mark.parametrize("T", [1, 8])
+def test_batched_heavy_kernels(rmat12, T):
+    g, hg = rmat12
+    src = xb.sample_sources(g, 200, seed=4)
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_heavy_cut=3, emit_heavy_cut=2)
+
+
+def test_plain_rmat(rmat12):
+    g, hg = rmat12
+    src = xb.sample_sources(g, 40, seed=5)
+    _check(g, hg, src, mode=xb.XB_MODE_PLAIN)
+
+
+def test_duplicate_and_isolated_sources():
+    g = xb.gen_rmat(10, 4, 0.57, 0.19, 0.19, seed=7)
+    deg = g.degrees()
+    iso = np.flatnonzero(deg == 0)
+    assert iso.size > 0
+    src = np.array([iso[0], 5, 5, 17, iso[-1], 5] * 13, dtype=np.int32)
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    for mode in (xb.XB_MODE_BATCHED, xb.XB_MODE_PLAIN):
+        _check(g, hg, src, mode=mode)
+
+
+def test_directed_rmat():
+    g = xb.gen_rmat(11, 8, 0.6, 0.15, 0.15, seed=9, symmetric=False)
+    hg = xb.HGraph(g.row_offsets, g.col_indices, directed=True)
+    src = xb.sample_sources(g, 130, seed=1)
+    for mode in (xb.XB_MODE_BATCHED, xb.XB_MODE_PLAIN):
+        _check(g, hg, src, mode=mode)
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=2, pull_heavy_cut=2, emit_heavy_cut=2,
+           alpha=1_000_000)
+
+
+def test_chung_lu_small():
+    g = xb.gen_chung_lu(20_000, 2.1, 10.0, 800, seed=3)
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    src = xb.sample_sources(g, 300, seed=3, degree_weighted=True)
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=4)
+
+
+def test_deep_path_falls_back():
+    g = _path_graph(600)
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    src = np.array([0, 599, 300, 10], dtype=np.int32)
+    st = _check(g, hg, src, mode=xb.XB_MODE_BATCHED)
+    assert st.fallback_sources == 4
+    src2 = np.array([250, 300, 350], dtype=np.int32)  # depth <= 350 -> still deep
+    _check(g, hg, src2, mode=xb.XB_MODE_PLAIN)
+
+
+def test_medium_path_no_fallback():
+    g = _path_graph(200)
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    src = np.arange(0, 200, 3, dtype=np.int32)
+    st = _check(g, hg, src, mode=xb.XB_MODE_BATCHED)
+    assert st.fallback_sources == 0 and st.max_level == 199

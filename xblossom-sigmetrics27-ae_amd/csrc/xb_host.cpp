@@ -1,0 +1,848 @@
+// xb_host.cpp -- host side of the MI355X multi-source BFS engine: CSR ingest
+// and validation, device graph, the plain per-source and batched bit-parallel
+// level loops, and the extern "C" boundary declared in include/xb/xb_capi.h.
+//
+// Reference counterparts (paths relative to the reference repository):
+//   CSR ingest           apps/xb/src/include/graph/hgraph_gpu.h:37-48,
+//                        apps/xb/src/include/common/utils_gpu.h:9-25
+//   device graph         apps/xb/src/include/graph/dgraph.h:50-63
+//   source seeding       apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:83-185
+//   level loop           apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:72-81
+//   expand step          apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:270-359
+//   push/pull switch     apps/ligra/ligra/ligra.h:236-277 (|F|+outdeg(F) > m/20)
+//
+// Unlike the reference's level loop (about ten blocking scalar reads per
+// level), each level here costs one 8-byte device-to-host read of the packed
+// (edges, count) queue counter.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/xb/xb_capi.h"
+#include "xb_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct XbError {
+  int32_t code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int32_t code, const std::string& msg) { throw XbError{code, msg}; }
+
+#define HIPCHK(expr)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      fail(e_ == hipErrorOutOfMemory ? XB_ERR_NOMEM : XB_ERR_DEVICE,                             \
+           std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
+  } while (0)
+
+template <class F>
+int32_t guarded(F&& f) {
+  try {
+    f();
+    return XB_OK;
+  } catch (const XbError& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return XB_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return XB_ERR_DEVICE;
+  }
+}
+
+template <class T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  return (T*)p;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int resolve_threads() {
+  unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc, 16u));
+}
+
+template <class F>
+void parallel_rows(int64_t n, F&& fn) {
+  int T = resolve_threads();
+  if (n < 65536) T = 1;
+  std::vector<std::thread> th;
+  int64_t chunk = (n + T - 1) / T;
+  for (int t = 0; t < T; ++t) {
+    int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=, &fn] { fn(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Pinned host scratch for the per-level counter read.
+struct Pinned {
+  unsigned long long* p = nullptr;
+  Pinned() { HIPCHK(hipHostMalloc((void**)&p, 64, hipHostMallocDefault)); }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+struct Workspace {
+  int T = 0;
+  int32_t n = 0;
+  xb::BPState bp{};
+  bool bp_dirty = false;
+  xb::Queue q[2]{};
+  unsigned long long* ctrs = nullptr;  // q[0].ctr, q[1].ctr
+  uint32_t* tcount = nullptr;
+  int32_t* d_src = nullptr;
+  int64_t src_cap = 0;
+  int32_t* side = nullptr;
+  int64_t side_cap = 0;
+  int32_t* stage = nullptr;  // host-output staging: levels rows then parent rows
+  int64_t stage_cap = 0;
+  int32_t* q0_host = nullptr;  // pinned upload buffer for the first frontier
+  int64_t q0_cap = 0;
+
+  void free_bp() {
+    dfree(bp.seen);
+    dfree(bp.front[0]);
+    dfree(bp.front[1]);
+    dfree(bp.nextw);
+    dfree(bp.planes);
+    dfree(bp.tflag);
+    dfree(bp.tlist);
+    bp = xb::BPState{};
+    T = 0;
+  }
+  void free_all() {
+    free_bp();
+    for (auto& x : q) {
+      dfree(x.v);
+      dfree(x.eo);
+    }
+    dfree(ctrs);
+    dfree(tcount);
+    dfree(d_src);
+    dfree(side);
+    dfree(stage);
+    if (q0_host) (void)hipHostFree(q0_host);
+    *this = Workspace{};
+  }
+};
+
+}  // namespace
+
+struct xb_graph {
+  int32_t n = 0;
+  int64_t m = 0;
+  uint32_t flags = 0;
+  std::vector<int32_t> h_off;     // out-edge offsets (host copy)
+  std::vector<int32_t> h_in_off;  // in-edge offsets (empty when symmetric)
+  int32_t* d_off = nullptr;
+  int32_t* d_col = nullptr;
+  int32_t* d_in_off = nullptr;  // == d_off when symmetric
+  int32_t* d_in_col = nullptr;  // == d_col when symmetric
+  hipStream_t stream = nullptr;
+  Pinned pinned;
+  // heavy-vertex lists (by in-degree), rebuilt when a cut changes
+  int32_t pull_cut = -1, emit_cut = -1;
+  int32_t* d_heavy_pull = nullptr;
+  int32_t n_heavy_pull = 0;
+  int32_t* d_heavy_emit = nullptr;
+  int32_t* d_hidx_emit = nullptr;
+  int32_t n_heavy_emit = 0;
+  Workspace ws;
+
+  const std::vector<int32_t>& in_off() const { return h_in_off.empty() ? h_off : h_in_off; }
+  int32_t outdeg(int32_t v) const { return h_off[v + 1] - h_off[v]; }
+
+  ~xb_graph() {
+    ws.free_all();
+    if (d_in_col != d_col) dfree(d_in_col);
+    if (d_in_off != d_off) dfree(d_in_off);
+    dfree(d_off);
+    dfree(d_col);
+    dfree(d_heavy_pull);
+    dfree(d_heavy_emit);
+    dfree(d_hidx_emit);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void ensure_device() {
+  int cnt = 0;
+  hipError_t e = hipGetDeviceCount(&cnt);
+  if (e != hipSuccess || cnt <= 0) fail(XB_ERR_NO_DEVICE, "no HIP device visible");
+}
+
+// Validates and (if needed) sorts; returns the possibly sorted column array.
+void validate_csr(const int32_t* off, int64_t n, const int32_t* col, int64_t m, std::vector<int32_t>& col_out) {
+  if (n < 0) fail(XB_ERR_INVALID_ARG, "n_plus_1 must be >= 1");
+  if (m < 0 || m > INT32_MAX) fail(XB_ERR_INVALID_ARG, "m out of range");
+  if (n > INT32_MAX - 1) fail(XB_ERR_INVALID_ARG, "n out of range");
+  if (off[0] != 0) fail(XB_ERR_BAD_CSR, "row_offsets[0] != 0");
+  for (int64_t v = 0; v < n; ++v)
+    if (off[v + 1] < off[v]) fail(XB_ERR_BAD_CSR, "row_offsets not monotone at row " + std::to_string(v));
+  if (off[n] != m) fail(XB_ERR_BAD_CSR, "row_offsets[n] != number of column indices");
+  std::vector<char> bad((size_t)resolve_threads() + 1, 0), unsorted((size_t)resolve_threads() + 1, 0);
+  std::vector<int64_t> first_bad((size_t)resolve_threads() + 1, -1);
+  col_out.assign(col, col + m);
+  // Range check and sortedness check in parallel.
+  {
+    int T = resolve_threads();
+    if (n < 65536) T = 1;
+    std::vector<std::thread> th;
+    int64_t chunk = (n + T - 1) / T;
+    for (int t = 0; t < T; ++t) {
+      int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+      if (b >= e) break;
+      th.emplace_back([&, t, b, e] {
+        for (int64_t v = b; v < e; ++v) {
+          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+            int32_t u = col[j];
+            if (u < 0 || u >= n) {
+              bad[t] = 1;
+              if (first_bad[t] < 0) first_bad[t] = j;
+            }
+            if (j > off[v] && col[j - 1] > u) unsorted[t] = 1;
+          }
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+  }
+  for (size_t t = 0; t < bad.size(); ++t)
+    if (bad[t])
+      fail(XB_ERR_BAD_CSR, "column index out of range at position " + std::to_string(first_bad[t]));
+  bool any_unsorted = false;
+  for (char c : unsorted) any_unsorted |= (c != 0);
+  if (any_unsorted) {
+    parallel_rows(n, [&](int64_t b, int64_t e) {
+      for (int64_t v = b; v < e; ++v) std::sort(col_out.begin() + off[v], col_out.begin() + off[v + 1]);
+    });
+  }
+}
+
+xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, int64_t m, uint32_t flags) {
+  if (!off || n_plus_1 < 1 || (m > 0 && !col)) fail(XB_ERR_INVALID_ARG, "null or empty CSR arrays");
+  if (flags & ~XB_GRAPH_DIRECTED) fail(XB_ERR_INVALID_ARG, "unknown graph flag");
+  const int64_t n = n_plus_1 - 1;
+  std::vector<int32_t> colv;
+  validate_csr(off, n, col, m, colv);
+  ensure_device();
+  std::unique_ptr<xb_graph> g(new xb_graph());
+  g->n = (int32_t)n;
+  g->m = m;
+  g->flags = flags;
+  g->h_off.assign(off, off + n + 1);
+  HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  g->d_off = dalloc<int32_t>((size_t)n + 1);
+  g->d_col = dalloc<int32_t>((size_t)m);
+  HIPCHK(hipMemcpy(g->d_off, off, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
+  if (m) HIPCHK(hipMemcpy(g->d_col, colv.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice));
+  if (flags & XB_GRAPH_DIRECTED) {
+    // Transpose by counting sort on the destination; sources are visited in
+    // ascending order, so every in-list comes out ascending.
+    std::vector<int32_t>& ioff = g->h_in_off;
+    ioff.assign((size_t)n + 1, 0);
+    for (int64_t j = 0; j < m; ++j) ioff[(size_t)colv[j] + 1]++;
+    for (int64_t v = 0; v < n; ++v) ioff[v + 1] += ioff[v];
+    std::vector<int32_t> pos(ioff.begin(), ioff.end() - 1);
+    std::vector<int32_t> icol((size_t)std::max<int64_t>(m, 1));
+    for (int64_t u = 0; u < n; ++u)
+      for (int32_t j = off[u]; j < off[u + 1]; ++j) icol[pos[colv[j]]++] = (int32_t)u;
+    g->d_in_off = dalloc<int32_t>((size_t)n + 1);
+    g->d_in_col = dalloc<int32_t>((size_t)m);
+    HIPCHK(hipMemcpy(g->d_in_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    if (m) HIPCHK(hipMemcpy(g->d_in_col, icol.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice));
+  } else {
+    g->d_in_off = g->d_off;
+    g->d_in_col = g->d_col;
+  }
+  return g.release();
+}
+
+// Reads whitespace-separated int32 values (ReadFileIntoHVector's format).
+std::vector<int32_t> read_ints(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) fail(XB_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  std::fseek(f, 0, SEEK_END);
+  long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  std::string buf((size_t)std::max(sz, 0L), '\0');
+  size_t rd = sz > 0 ? std::fread(&buf[0], 1, (size_t)sz, f) : 0;
+  std::fclose(f);
+  if ((long)rd != sz) fail(XB_ERR_IO, std::string("short read on ") + path);
+  std::vector<int32_t> out;
+  out.reserve((size_t)sz / 4 + 1);
+  const char* p = buf.data();
+  const char* end = p + buf.size();
+  while (p < end) {
+    while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) ++p;
+    if (p >= end) break;
+    bool neg = false;
+    if (*p == '-' || *p == '+') {
+      neg = (*p == '-');
+      ++p;
+    }
+    if (p >= end || *p < '0' || *p > '9') fail(XB_ERR_IO, std::string("non-integer token in ") + path);
+    int64_t v = 0;
+    while (p < end && *p >= '0' && *p <= '9') {
+      v = v * 10 + (*p - '0');
+      if (v > (int64_t)INT32_MAX + 1) fail(XB_ERR_IO, std::string("integer overflow in ") + path);
+      ++p;
+    }
+    if (p < end && !(*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t'))
+      fail(XB_ERR_IO, std::string("non-integer token in ") + path);
+    v = neg ? -v : v;
+    if (v > INT32_MAX) fail(XB_ERR_IO, std::string("integer overflow in ") + path);
+    out.push_back((int32_t)v);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Heavy lists (by in-degree)
+// ---------------------------------------------------------------------------
+void ensure_heavy(xb_graph* g, int32_t pull_cut, int32_t emit_cut) {
+  const auto& io = g->in_off();
+  if (pull_cut != g->pull_cut) {
+    std::vector<int32_t> h;
+    for (int32_t v = 0; v < g->n; ++v)
+      if (io[v + 1] - io[v] > pull_cut) h.push_back(v);
+    dfree(g->d_heavy_pull);
+    g->d_heavy_pull = dalloc<int32_t>(h.size());
+    if (!h.empty())
+      HIPCHK(hipMemcpy(g->d_heavy_pull, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    g->n_heavy_pull = (int32_t)h.size();
+    g->pull_cut = pull_cut;
+  }
+  if (emit_cut != g->emit_cut) {
+    std::vector<int32_t> h, idx((size_t)g->n, -1);
+    for (int32_t v = 0; v < g->n; ++v)
+      if (io[v + 1] - io[v] > emit_cut) {
+        idx[v] = (int32_t)h.size();
+        h.push_back(v);
+      }
+    dfree(g->d_heavy_emit);
+    dfree(g->d_hidx_emit);
+    g->d_heavy_emit = dalloc<int32_t>(h.size());
+    g->d_hidx_emit = dalloc<int32_t>((size_t)g->n);
+    if (!h.empty())
+      HIPCHK(hipMemcpy(g->d_heavy_emit, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    if (g->n) HIPCHK(hipMemcpy(g->d_hidx_emit, idx.data(), sizeof(int32_t) * (size_t)g->n, hipMemcpyHostToDevice));
+    g->n_heavy_emit = (int32_t)h.size();
+    g->emit_cut = emit_cut;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Workspace
+// ---------------------------------------------------------------------------
+void ensure_queues(xb_graph* g) {
+  Workspace& w = g->ws;
+  if (w.ctrs) return;
+  const size_t n = (size_t)g->n + 1;
+  w.n = g->n;
+  w.ctrs = dalloc<unsigned long long>(2);
+  for (int i = 0; i < 2; ++i) {
+    w.q[i].v = dalloc<int32_t>(n);
+    w.q[i].eo = dalloc<int32_t>(n);
+    w.q[i].ctr = w.ctrs + i;
+  }
+  w.tcount = dalloc<uint32_t>(1);
+}
+
+void ensure_bp(xb_graph* g, int T) {
+  Workspace& w = g->ws;
+  if (w.T == T) return;
+  w.free_bp();
+  const size_t nw = (size_t)g->n * (size_t)T + 1;
+  w.bp.seen = dalloc<uint64_t>(nw);
+  w.bp.front[0] = dalloc<uint64_t>(nw);
+  w.bp.front[1] = dalloc<uint64_t>(nw);
+  w.bp.nextw = dalloc<uint64_t>(nw);
+  w.bp.planes = dalloc<uint64_t>(nw * 8);
+  w.bp.tflag = dalloc<uint32_t>((size_t)g->n + 1);
+  w.bp.tlist = dalloc<int32_t>((size_t)g->n + 1);
+  w.bp.tcount = w.tcount;
+  w.T = T;
+  w.bp_dirty = true;
+}
+
+void ensure_src(xb_graph* g, int64_t cnt) {
+  Workspace& w = g->ws;
+  if (w.src_cap >= cnt) return;
+  dfree(w.d_src);
+  w.d_src = dalloc<int32_t>((size_t)cnt);
+  w.src_cap = cnt;
+}
+
+void ensure_side(xb_graph* g) {
+  Workspace& w = g->ws;
+  const int64_t need = (int64_t)std::max(g->n_heavy_emit, 1) * 64;
+  if (w.side_cap >= need) return;
+  dfree(w.side);
+  w.side = dalloc<int32_t>((size_t)need);
+  w.side_cap = need;
+}
+
+int32_t* ensure_stage(xb_graph* g, int64_t ints) {
+  Workspace& w = g->ws;
+  if (w.stage_cap < ints) {
+    dfree(w.stage);
+    w.stage = dalloc<int32_t>((size_t)ints);
+    w.stage_cap = ints;
+  }
+  return w.stage;
+}
+
+int32_t* ensure_q0_host(xb_graph* g, int64_t ints) {
+  Workspace& w = g->ws;
+  if (w.q0_cap < ints) {
+    if (w.q0_host) (void)hipHostFree(w.q0_host);
+    w.q0_host = nullptr;
+    HIPCHK(hipHostMalloc((void**)&w.q0_host, sizeof(int32_t) * (size_t)ints, hipHostMallocDefault));
+    w.q0_cap = ints;
+  }
+  return w.q0_host;
+}
+
+// ---------------------------------------------------------------------------
+// Event timing
+// ---------------------------------------------------------------------------
+enum Cat { kInit = 0, kPush = 1, kPull = 2, kEmit = 3, kNumCats = 4 };
+
+struct Profiler {
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t open_ev = nullptr;
+  int open_cat = -1;
+
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  void begin(int cat) {
+    if (!on) return;
+    open_ev = get();
+    open_cat = cat;
+    HIPCHK(hipEventRecord(open_ev, st));
+  }
+  void end() {
+    if (!on || open_cat < 0) return;
+    hipEvent_t e = get();
+    HIPCHK(hipEventRecord(e, st));
+    marks.push_back({open_cat, {open_ev, e}});
+    open_cat = -1;
+  }
+  void collect(double* ms) {
+    for (auto& mk : marks) {
+      float t = 0;
+      HIPCHK(hipEventSynchronize(mk.second.second));
+      HIPCHK(hipEventElapsedTime(&t, mk.second.first, mk.second.second));
+      ms[mk.first] += t;
+      pool.push_back(mk.second.first);
+      pool.push_back(mk.second.second);
+    }
+    marks.clear();
+  }
+  ~Profiler() {
+    for (auto& mk : marks) {
+      (void)hipEventDestroy(mk.second.first);
+      (void)hipEventDestroy(mk.second.second);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// Reads the packed (edges << 32 | count) counter of a queue.
+void read_ctr(xb_graph* g, const unsigned long long* dctr, int32_t* qn, int32_t* E) {
+  HIPCHK(hipMemcpyAsync(g->pinned.p, dctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  const unsigned long long v = *g->pinned.p;
+  *qn = (int32_t)(uint32_t)(v & 0xFFFFFFFFull);
+  *E = (int32_t)(uint32_t)(v >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-source engine
+// ---------------------------------------------------------------------------
+void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows, int32_t* par_rows,
+               bool dev_out, Profiler& prof, xb_run_stats* st) {
+  ensure_queues(g);
+  Workspace& w = g->ws;
+  const int32_t n = g->n;
+  int32_t* stage = nullptr;
+  if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)n);
+  int32_t* q0 = ensure_q0_host(g, 2);
+  for (int32_t s = 0; s < S; ++s) {
+    const int32_t src = sources[s];
+    int32_t* lv = dev_out ? (lvl_rows ? lvl_rows + (int64_t)s * n : stage) : stage;
+    int32_t* pa = dev_out ? par_rows + (int64_t)s * n : stage + n;
+    if (dev_out && !lvl_rows) lv = ensure_stage(g, n);
+    prof.begin(kInit);
+    xb::launch_plain_init(g->stream, lv, pa, n, src);
+    q0[0] = src;
+    q0[1] = 0;
+    HIPCHK(hipMemcpyAsync(w.q[0].v, &q0[0], sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(w.q[0].eo, &q0[1], sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+    prof.end();
+    int32_t qn = 1, E = g->outdeg(src);
+    int cur = 0, d = 0;
+    while (qn > 0) {
+      HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), g->stream));
+      prof.begin(kPush);
+      xb::launch_plain_push(g->stream, g->d_off, g->d_col, w.q[cur], qn, E, lv, pa, d, w.q[cur ^ 1]);
+      prof.end();
+      if (st) st->launches_push++;
+      read_ctr(g, w.q[cur ^ 1].ctr, &qn, &E);
+      cur ^= 1;
+      if (qn > 0) ++d;
+    }
+    if (st) {
+      st->max_level = std::max(st->max_level, d);
+      st->push_levels += d + 1;
+    }
+    if (!dev_out) {
+      if (lvl_rows)
+        HIPCHK(hipMemcpyAsync(lvl_rows + (int64_t)s * n, stage, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                              g->stream));
+      HIPCHK(hipMemcpyAsync(par_rows + (int64_t)s * n, stage + n, sizeof(int32_t) * (size_t)n,
+                            hipMemcpyDeviceToHost, g->stream));
+      HIPCHK(hipStreamSynchronize(g->stream));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched bit-parallel engine
+// ---------------------------------------------------------------------------
+struct BPOpts {
+  int T;
+  int alpha;
+  int pull_cut;
+  int emit_cut;
+};
+
+// Returns false when the tile got deeper than the 8-bit level code allows.
+bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, int32_t* lvl_rows,
+                 int32_t* par_rows, Profiler& prof, xb_run_stats* st) {
+  Workspace& w = g->ws;
+  const int T = o.T;
+  const int32_t n = g->n;
+  const size_t nw = (size_t)n * (size_t)T;
+  xb::BPState& s = w.bp;
+  prof.begin(kInit);
+  if (w.bp_dirty) {
+    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * nw, g->stream));
+    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * nw, g->stream));
+    HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * nw, g->stream));
+    HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), g->stream));
+    w.bp_dirty = false;
+  }
+  HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, g->stream));
+  HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, g->stream));
+  // First frontier: the distinct source vertices with their degree prefix.
+  std::vector<int32_t> uniq(src, src + cnt);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  const int32_t qn0 = (int32_t)uniq.size();
+  HIPCHK(hipStreamSynchronize(g->stream));  // earlier uploads from the pinned buffer are done
+  int32_t* q0 = ensure_q0_host(g, 2 * (int64_t)qn0 + cnt);
+  int64_t E0 = 0;
+  for (int32_t i = 0; i < qn0; ++i) {
+    q0[i] = uniq[i];
+    q0[qn0 + i] = (int32_t)E0;
+    E0 += g->outdeg(uniq[i]);
+  }
+  std::memcpy(q0 + 2 * qn0, src, sizeof(int32_t) * (size_t)cnt);
+  HIPCHK(hipMemcpyAsync(w.q[0].v, q0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(w.q[0].eo, q0 + qn0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(w.d_src, q0 + 2 * qn0, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice, g->stream));
+  xb::launch_bp_seed(g->stream, T, s, w.d_src, cnt);
+  prof.end();
+
+  int32_t qn = qn0, E = (int32_t)E0;
+  int cur = 0, d = 0;
+  int pushes = 0, pulls = 0;
+  const int64_t thresh = g->m / std::max(o.alpha, 1);
+  bool overflow = false;
+  while (qn > 0) {
+    const int32_t code = d + 2;  // level d+1
+    if (code > 255) {
+      overflow = true;
+      break;
+    }
+    xb::Queue& qc = w.q[cur];
+    xb::Queue& qx = w.q[cur ^ 1];
+    HIPCHK(hipMemsetAsync(qx.ctr, 0, sizeof(unsigned long long), g->stream));
+    if ((int64_t)qn + E > thresh) {
+      prof.begin(kPull);
+      xb::launch_bp_pull(g->stream, T, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off, n, o.pull_cut,
+                         g->d_heavy_pull, g->n_heavy_pull, qx);
+      xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
+      prof.end();
+      ++pulls;
+      if (st) st->launches_pull += 2 + (g->n_heavy_pull > 0);
+    } else {
+      prof.begin(kPush);
+      HIPCHK(hipMemsetAsync(w.tcount, 0, sizeof(uint32_t), g->stream));
+      xb::launch_bp_push(g->stream, T, s, cur, g->d_off, g->d_col, qc, qn, E);
+      xb::launch_bp_commit(g->stream, T, s, cur, code, g->d_off, qx, (int32_t)std::min<int64_t>(E, n));
+      xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
+      prof.end();
+      ++pushes;
+      if (st) st->launches_push += 3;
+    }
+    read_ctr(g, qx.ctr, &qn, &E);
+    cur ^= 1;
+    if (qn > 0) ++d;
+  }
+  if (overflow) {
+    w.bp_dirty = true;
+    return false;
+  }
+  // Emit: K planes cover codes up to d+1.
+  int K = 0;
+  while ((1 << K) <= d + 1) ++K;
+  ensure_side(g);
+  prof.begin(kEmit);
+  const int words = (cnt + 63) / 64;
+  for (int t = 0; t < words; ++t) {
+    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, g->d_in_off, g->d_in_col, n, o.emit_cut, g->d_heavy_emit,
+                       g->d_hidx_emit, g->n_heavy_emit, w.side, lvl_rows, par_rows);
+  }
+  prof.end();
+  if (st) {
+    st->launches_emit += (int64_t)words * (1 + (g->n_heavy_emit > 0));
+    st->max_level = std::max(st->max_level, d);
+    st->push_levels += pushes;
+    st->pull_levels += pulls;
+    st->tiles += 1;
+  }
+  return true;
+}
+
+void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl, int32_t* par,
+                 bool dev_out, Profiler& prof, xb_run_stats* st) {
+  ensure_queues(g);
+  ensure_bp(g, o.T);
+  ensure_heavy(g, o.pull_cut, o.emit_cut);
+  const int32_t n = g->n;
+  const int32_t tile = 64 * o.T;
+  ensure_src(g, tile);
+  int32_t* stage = nullptr;
+  if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)tile * n);
+  for (int32_t base = 0; base < S; base += tile) {
+    const int32_t cnt = std::min(tile, S - base);
+    int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : stage;
+    int32_t* pr = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
+    if (!dev_out && !lvl) lr = nullptr;
+    bool ok = run_bp_tile(g, o, sources + base, cnt, lr, pr, prof, st);
+    if (!ok) {
+      // Deeper than 254 levels: these sources take the plain path.
+      if (st) st->fallback_sources += cnt;
+      run_plain(g, sources + base, cnt, dev_out ? lr : (lvl ? lvl + (int64_t)base * n : nullptr),
+                dev_out ? pr : par + (int64_t)base * n, dev_out, prof, st);
+      continue;
+    }
+    if (!dev_out) {
+      if (lvl)
+        HIPCHK(hipMemcpyAsync(lvl + (int64_t)base * n, stage, sizeof(int32_t) * (size_t)cnt * n,
+                              hipMemcpyDeviceToHost, g->stream));
+      HIPCHK(hipMemcpyAsync(par + (int64_t)base * n, stage + (int64_t)tile * n, sizeof(int32_t) * (size_t)cnt * n,
+                            hipMemcpyDeviceToHost, g->stream));
+      HIPCHK(hipStreamSynchronize(g->stream));
+    }
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int32_t xb_opts_default(xb_msbfs_opts* o) {
+  if (!o) {
+    g_last_error = "null opts";
+    return XB_ERR_INVALID_ARG;
+  }
+  std::memset(o, 0, sizeof(*o));
+  o->mode = XB_MODE_AUTO;
+  o->words_per_tile = 8;
+  o->alpha = 20;
+  o->pull_heavy_cut = 512;
+  o->emit_heavy_cut = 256;
+  return XB_OK;
+}
+
+int32_t xb_graph_create_csr(const int32_t* row_offsets, int64_t n_plus_1, const int32_t* col_indices, int64_t m,
+                            uint32_t flags, xb_graph** out) {
+  return guarded([&] {
+    if (!out) fail(XB_ERR_INVALID_ARG, "null out handle");
+    *out = build_graph(row_offsets, n_plus_1, col_indices, m, flags);
+  });
+}
+
+int32_t xb_graph_load_text(const char* row_path, const char* col_path, uint32_t flags, xb_graph** out) {
+  return guarded([&] {
+    if (!out || !row_path || !col_path) fail(XB_ERR_INVALID_ARG, "null argument");
+    std::vector<int32_t> off = read_ints(row_path);
+    std::vector<int32_t> col = read_ints(col_path);
+    if (off.empty()) fail(XB_ERR_IO, std::string("no row offsets in ") + row_path);
+    *out = build_graph(off.data(), (int64_t)off.size(), col.data(), (int64_t)col.size(), flags);
+  });
+}
+
+int32_t xb_graph_info(const xb_graph* g, int64_t* n, int64_t* m, uint32_t* flags) {
+  return guarded([&] {
+    if (!g) fail(XB_ERR_INVALID_ARG, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (flags) *flags = g->flags;
+  });
+}
+
+int32_t xb_graph_destroy(xb_graph* g) {
+  return guarded([&] {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    delete g;
+  });
+}
+
+int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* levels, int32_t* parents,
+                     const xb_msbfs_opts* opts, xb_run_stats* stats) {
+  return guarded([&] {
+    if (!g) fail(XB_ERR_INVALID_ARG, "null graph");
+    if (S < 0) fail(XB_ERR_INVALID_ARG, "S < 0");
+    if (S == 0) return;
+    if (!sources || !parents) fail(XB_ERR_INVALID_ARG, "null sources or parents");
+    xb_msbfs_opts o;
+    xb_opts_default(&o);
+    if (opts) {
+      o.mode = opts->mode;
+      o.flags = opts->flags;
+      if (opts->words_per_tile) o.words_per_tile = opts->words_per_tile;
+      if (opts->alpha) o.alpha = opts->alpha;
+      if (opts->pull_heavy_cut) o.pull_heavy_cut = opts->pull_heavy_cut;
+      if (opts->emit_heavy_cut) o.emit_heavy_cut = opts->emit_heavy_cut;
+    }
+    if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
+    if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
+    const int T = o.words_per_tile;
+    if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
+    if (o.alpha < 1 || o.pull_heavy_cut < 1 || o.emit_heavy_cut < 1) fail(XB_ERR_INVALID_ARG, "bad tunable");
+    for (int32_t i = 0; i < S; ++i)
+      if (sources[i] < 0 || sources[i] >= g->n)
+        fail(XB_ERR_SOURCE_RANGE, "source " + std::to_string(i) + " = " + std::to_string(sources[i]) +
+                                      " outside [0, " + std::to_string(g->n) + ")");
+    const bool dev_out = (o.flags & XB_OUT_DEVICE) != 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    Profiler prof;
+    prof.on = (o.flags & XB_PROFILE) != 0;
+    prof.st = g->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, g->stream));
+    if (o.mode == XB_MODE_PLAIN) {
+      run_plain(g, sources, S, levels, parents, dev_out, prof, stats);
+    } else {
+      BPOpts bo{T, o.alpha, o.pull_heavy_cut, o.emit_heavy_cut};
+      run_batched(g, bo, sources, S, levels, parents, dev_out, prof, stats);
+    }
+    HIPCHK(hipEventRecord(e1, g->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+      double cats[kNumCats] = {0, 0, 0, 0};
+      prof.collect(cats);
+      stats->ms_total = ms;
+      stats->ms_init = cats[kInit];
+      stats->ms_push = cats[kPush];
+      stats->ms_pull = cats[kPull];
+      stats->ms_emit = cats[kEmit];
+    }
+  });
+}
+
+int32_t xb_count_traversed(xb_graph* g, const int32_t* levels_dev, int32_t S, int64_t* edges_out) {
+  return guarded([&] {
+    if (!g || !levels_dev || !edges_out || S < 0) fail(XB_ERR_INVALID_ARG, "bad argument");
+    unsigned long long* d = dalloc<unsigned long long>(1);
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), g->stream));
+    if (S > 0 && g->n > 0) xb::launch_count_traversed(g->stream, g->d_off, levels_dev, S, g->n, d);
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    dfree(d);
+    *edges_out = (int64_t)h;
+  });
+}
+
+int32_t xb_device_alloc(size_t bytes, void** ptr) {
+  return guarded([&] {
+    if (!ptr) fail(XB_ERR_INVALID_ARG, "null ptr");
+    ensure_device();
+    *ptr = nullptr;
+    HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  });
+}
+
+int32_t xb_device_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) HIPCHK(hipFree(ptr));
+  });
+}
+
+int32_t xb_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+int32_t xb_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return XB_ERR_INVALID_ARG;
+  std::snprintf(buf, len, "%s", g_last_error.c_str());
+  return XB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,345 @@
+"""ctypes binding of the MI355X multi-source BFS engine (lib/libxbmsbfs.so).
+
+Mirrors the reference's C++ entry points for the search path:
+
+* ``HGraph(row_offsets_path, col_indices_path)`` --
+  ``zblossom::HGraph`` (apps/xb/src/include/graph/hgraph_gpu.h:37-48), here
+  also constructible from in-memory int32 CSR arrays;
+* ``XBlossomGpuEngine10(graph, path_table_buffer_ratio)`` --
+  ``zblossom::XBlossomGpuEngine10(const HGraph&, size_t)``
+  (apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:25), gaining
+  ``MultiSourceBFS(sources, levels, parents)``.
+
+Every call goes through the C ABI in include/xb/xb_capi.h.  There is no CPU
+fallback: if the shared library is missing or no GPU is visible, calls raise.
+The generator functions wrap lib/libxbgen.so (host-only).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+ENGINE_LIB = os.path.join(LIB_DIR, "libxbmsbfs.so")
+GEN_LIB = os.path.join(LIB_DIR, "libxbgen.so")
+
+XB_OK = 0
+XB_ERR_INVALID_ARG = -1
+XB_ERR_BAD_CSR = -2
+XB_ERR_IO = -3
+XB_ERR_NOMEM = -4
+XB_ERR_DEVICE = -5
+XB_ERR_SOURCE_RANGE = -6
+XB_ERR_BUDGET = -7
+XB_ERR_UNSUPPORTED = -8
+XB_ERR_NO_DEVICE = -9
+
+XB_GRAPH_SYMMETRIC = 0
+XB_GRAPH_DIRECTED = 1
+
+XB_MODE_AUTO = 0
+XB_MODE_PLAIN = 1
+XB_MODE_BATCHED = 2
+
+XB_OUT_DEVICE = 1
+XB_PROFILE = 2
+XB_NO_LEVELS = 4
+
+# Every symbol include/xb/xb_capi.h declares.
+EXPORTED_SYMBOLS = (
+    "xb_opts_default",
+    "xb_graph_create_csr",
+    "xb_graph_load_text",
+    "xb_graph_info",
+    "xb_graph_destroy",
+    "xb_msbfs_run",
+    "xb_count_traversed",
+    "xb_device_alloc",
+    "xb_device_free",
+    "xb_memcpy_d2h",
+    "xb_memcpy_h2d",
+    "xb_last_error",
+)
+
+
+class XbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"xb error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("words_per_tile", ctypes.c_int32),
+        ("alpha", ctypes.c_int32),
+        ("pull_heavy_cut", ctypes.c_int32),
+        ("emit_heavy_cut", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("ms_init", ctypes.c_double),
+        ("ms_push", ctypes.c_double),
+        ("ms_pull", ctypes.c_double),
+        ("ms_emit", ctypes.c_double),
+        ("launches_push", ctypes.c_int64),
+        ("launches_pull", ctypes.c_int64),
+        ("launches_emit", ctypes.c_int64),
+        ("max_level", ctypes.c_int32),
+        ("tiles", ctypes.c_int32),
+        ("push_levels", ctypes.c_int32),
+        ("pull_levels", ctypes.c_int32),
+        ("fallback_sources", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+_gen = None
+
+_P32 = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a):
+    """int32 pointer to a numpy array, or a raw device address (int)."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.cast(ctypes.c_void_p(a), _P32)
+    return a.ctypes.data_as(_P32)
+
+
+def load_engine():
+    """Loads lib/libxbmsbfs.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ENGINE_LIB):
+        raise XbError(XB_ERR_UNSUPPORTED, f"{ENGINE_LIB} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(ENGINE_LIB)
+    I32, U32, I64, VP = ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_void_p
+    lib.xb_opts_default.argtypes = [ctypes.POINTER(Opts)]
+    lib.xb_graph_create_csr.argtypes = [_P32, I64, _P32, I64, U32, ctypes.POINTER(VP)]
+    lib.xb_graph_load_text.argtypes = [ctypes.c_char_p, ctypes.c_char_p, U32, ctypes.POINTER(VP)]
+    lib.xb_graph_info.argtypes = [VP, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(U32)]
+    lib.xb_graph_destroy.argtypes = [VP]
+    lib.xb_msbfs_run.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts), ctypes.POINTER(RunStats)]
+    lib.xb_count_traversed.argtypes = [VP, _P32, I32, ctypes.POINTER(I64)]
+    lib.xb_device_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(VP)]
+    lib.xb_device_free.argtypes = [VP]
+    lib.xb_memcpy_d2h.argtypes = [VP, VP, ctypes.c_size_t]
+    lib.xb_memcpy_h2d.argtypes = [VP, VP, ctypes.c_size_t]
+    lib.xb_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    for name in EXPORTED_SYMBOLS:
+        getattr(lib, name).restype = ctypes.c_int32
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != XB_OK:
+        buf = ctypes.create_string_buffer(1024)
+        _lib.xb_last_error(buf, 1024)
+        raise XbError(rc, buf.value.decode(errors="replace"))
+
+
+def default_opts(**kw) -> Opts:
+    lib = load_engine()
+    o = Opts()
+    _check(lib.xb_opts_default(ctypes.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class HGraph:
+    """Device-resident CSR graph (int32 rowOffsets / colIndices)."""
+
+    def __init__(self, row_offsets=None, col_indices=None, directed: bool = False):
+        lib = load_engine()
+        self._h = ctypes.c_void_p()
+        flags = XB_GRAPH_DIRECTED if directed else XB_GRAPH_SYMMETRIC
+        if isinstance(row_offsets, (str, os.PathLike)):
+            _check(lib.xb_graph_load_text(os.fsencode(row_offsets), os.fsencode(col_indices), flags,
+                                          ctypes.byref(self._h)))
+        else:
+            ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
+            ci = np.ascontiguousarray(col_indices if col_indices is not None else [], dtype=np.int32)
+            _check(lib.xb_graph_create_csr(_ptr(ro), ro.size, _ptr(ci), ci.size, flags, ctypes.byref(self._h)))
+        n, m, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+        _check(lib.xb_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(f)))
+        self.nnodes, self.nedges, self.directed = n.value, m.value, bool(f.value & 1)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _check(_lib.xb_graph_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class XBlossomGpuEngine10:
+    """Engine bound to one graph; ``MultiSourceBFS`` is the batched search."""
+
+    def __init__(self, graph: HGraph, path_table_buffer_ratio: int = 80):
+        self.graph = graph
+        self.path_table_buffer_ratio = path_table_buffer_ratio
+        self.last_stats = RunStats()
+
+    def MultiSourceBFS(self, sources, levels=None, parents=None, *, mode: int = XB_MODE_AUTO,
+                       device_out: bool = False, profile: bool = False, want_levels: bool = True,
+                       **tunables):
+        """Runs len(sources) BFS traversals.
+
+        Host mode (default): returns (levels, parents) as S x n int32 numpy
+        arrays.  Device mode: ``levels``/``parents`` are device addresses
+        (ints, e.g. ``tensor.data_ptr()``) of S x n int32 buffers.
+        """
+        lib = load_engine()
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        S, n = int(src.size), self.graph.nnodes
+        flags = (XB_OUT_DEVICE if device_out else 0) | (XB_PROFILE if profile else 0)
+        if not want_levels:
+            flags |= XB_NO_LEVELS
+        if not device_out:
+            if parents is None:
+                parents = np.empty((S, n), dtype=np.int32)
+            if levels is None and want_levels:
+                levels = np.empty((S, n), dtype=np.int32)
+        o = default_opts(mode=mode, flags=flags, **tunables)
+        st = RunStats()
+        _check(lib.xb_msbfs_run(self.graph.handle, _ptr(src), S, _ptr(levels), _ptr(parents), ctypes.byref(o),
+                                ctypes.byref(st)))
+        self.last_stats = st
+        return levels, parents
+
+    def CountTraversed(self, levels_dev: int, S: int) -> int:
+        lib = load_engine()
+        out = ctypes.c_int64()
+        _check(lib.xb_count_traversed(self.graph.handle, _ptr(levels_dev), S, ctypes.byref(out)))
+        return out.value
+
+
+# ---------------------------------------------------------------------------
+# Synthetic inputs (host-only generator library)
+# ---------------------------------------------------------------------------
+class _Csr(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int64), ("rowoff", _P32), ("col", _P32)]
+
+
+def load_gen():
+    global _gen
+    if _gen is not None:
+        return _gen
+    if not os.path.exists(GEN_LIB):
+        raise XbError(XB_ERR_UNSUPPORTED, f"{GEN_LIB} not built")
+    g = ctypes.CDLL(GEN_LIB)
+    D, I32, I64, U64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+    g.xbgen_rmat.argtypes = [I32, I64, D, D, D, U64, I32, I32, ctypes.POINTER(_Csr)]
+    g.xbgen_chung_lu.argtypes = [I32, D, D, D, D, U64, I32, I32, ctypes.POINTER(_Csr)]
+    g.xbgen_free.argtypes = [ctypes.POINTER(_Csr)]
+    g.xbgen_sample_sources.argtypes = [I32, _P32, I32, U64, I32, _P32]
+    g.xbgen_sample_sources.restype = I32
+    g.xbgen_checksum.argtypes = [I32, _P32, I64, _P32]
+    g.xbgen_checksum.restype = U64
+    _gen = g
+    return g
+
+
+@dataclass
+class Csr:
+    row_offsets: np.ndarray
+    col_indices: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.row_offsets.size - 1
+
+    @property
+    def m(self) -> int:
+        return self.col_indices.size
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row_offsets)
+
+    def checksum(self) -> int:
+        g = load_gen()
+        return int(g.xbgen_checksum(self.n, _ptr(self.row_offsets), self.m, _ptr(self.col_indices)))
+
+
+def _take(c: _Csr) -> Csr:
+    g = load_gen()
+    ro = np.ctypeslib.as_array(c.rowoff, (c.n + 1,)).copy()
+    co = np.ctypeslib.as_array(c.col, (max(c.m, 1),))[: c.m].copy()
+    g.xbgen_free(ctypes.byref(c))
+    return Csr(ro, co)
+
+
+def gen_rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, symmetric: bool = True,
+             threads: int = 0) -> Csr:
+    """R-MAT with 2**scale vertices and edge_factor * 2**scale draws."""
+    g = load_gen()
+    out = _Csr()
+    rc = g.xbgen_rmat(scale, edge_factor * (1 << scale), a, b, c, seed, 1 if symmetric else 0, threads,
+                      ctypes.byref(out))
+    if rc != 0:
+        raise XbError(XB_ERR_INVALID_ARG, f"xbgen_rmat failed ({rc})")
+    return _take(out)
+
+
+def gen_chung_lu(n: int, gamma: float, mean_deg: float, max_deg: float, seed: int, overdraw: float = 1.0,
+                 permute: bool = True, threads: int = 0) -> Csr:
+    g = load_gen()
+    out = _Csr()
+    rc = g.xbgen_chung_lu(n, gamma, mean_deg, max_deg, overdraw, seed, 1 if permute else 0, threads,
+                          ctypes.byref(out))
+    if rc != 0:
+        raise XbError(XB_ERR_INVALID_ARG, f"xbgen_chung_lu failed ({rc})")
+    return _take(out)
+
+
+def sample_sources(csr: Csr, S: int, seed: int, degree_weighted: bool = False) -> np.ndarray:
+    g = load_gen()
+    out = np.empty(S, dtype=np.int32)
+    got = g.xbgen_sample_sources(csr.n, _ptr(csr.row_offsets), S, seed, 1 if degree_weighted else 0, _ptr(out))
+    return out[:got]
+
+
+# Synthetic workloads, one per named configuration.  Sizes are the full ones.
+def workload_rmat_s16():
+    """R-MAT scale 16, edge factor 16, a=.57 b=.19 c=.19, seed 1; 64 uniform sources (seed 2)."""
+    g = gen_rmat(16, 16, 0.57, 0.19, 0.19, seed=1)
+    return g, sample_sources(g, 64, seed=2)
+
+
+def workload_amazon_like():
+    """Chung-Lu n=403k, gamma 2.3, mean degree 8.4, max ~2,750, seed 3; 256 uniform sources."""
+    g = gen_chung_lu(403_000, 2.3, 8.4, 2_800, seed=3, overdraw=1.005)
+    return g, sample_sources(g, 256, seed=3)
+
+
+def workload_livejournal_like():
+    """Chung-Lu n=4.85M, gamma 2.1, mean degree ~14.2, max ~20k, seed 4; 1,024 degree-weighted sources."""
+    g = gen_chung_lu(4_850_000, 2.1, 14.2, 22_500, seed=4, overdraw=1.015)
+    return g, sample_sources(g, 1024, seed=4, degree_weighted=True)
