@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""Batched vs per-source multi-source BFS on one MI355X.
+
+A step is one pass of the batched bit-parallel engine over the whole source
+batch of the workload (default: the LiveJournal-shaped Chung-Lu graph,
+n = 4.85M, m ~ 69M, 1,024 degree-weighted sources), with the graph and the
+S x n level/parent output tables resident in HBM.  The plain per-source engine
+(one top-down push launch sequence per source) runs the same batch once, and
+its tables must match the batched ones bit for bit.  The CPU baseline is the
+oracle (a per-source BFS port with the same output rule) on a bounded sample of
+the same sources, on the host's cores.
+
+Prints one JSON line.  GTEPS = sum over sources of the out-degrees of the
+vertices each source reaches, divided by wall time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "xblossom-sigmetrics27-ae_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "GTEPS batched vs per-source on 1 MI355X (bit-exact); %HBM roofline; p99 query ms"
+HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+# rocprofv3 kernel names of the engine's kernel classes (for the PMC traffic lookup).
+KERNEL_SYMBOL = {
+    "emit": "k_bp_emit_light",
+    "pull": "k_bp_pull_bucket",
+    "pull_hub": "k_bp_pull_hub",
+    "push": "k_bp_push",
+    "emit_heavy": "k_bp_emit_heavy",
+    "plain_push": "k_plain_push",
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    import xbmsbfs as xb
+
+    if name == "livejournal":
+        g, src = xb.workload_livejournal_like()
+        desc = "LiveJournal-shaped Chung-Lu (gamma 2.1, seed 4), 1,024 degree-weighted sources"
+    elif name == "amazon":
+        g, src = xb.workload_amazon_like()
+        desc = "Amazon-shaped Chung-Lu (gamma 2.3, seed 3), 256 uniform sources"
+    elif name == "rmat16":
+        g, src = xb.workload_rmat_s16()
+        desc = "R-MAT scale 16 edge factor 16 (seed 1), 64 uniform sources (seed 2)"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return g, src, desc
+
+
+def traffic_from_profiles(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary, if present.
+
+    The summary file profiles/pmc_<kernel>_<workload>.txt holds a line
+    'traffic_bytes_per_launch=<float>' (FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the copy-kernel calibration factor)."""
+    sym = KERNEL_SYMBOL.get(kernel, kernel)
+    path = os.path.join(PROFILES, f"pmc_{sym}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch="):
+                return float(line.split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(g, src, seconds: float, threads: int):
+    """Times the oracle (per-source BFS port) on chunks of sources until `seconds` elapse."""
+    import oracle
+
+    done, edges, t = 0, 0, 0.0
+    chunk = max(threads, 1)
+    while done < src.size and t < seconds:
+        part = src[done:done + chunk]
+        t0 = time.perf_counter()
+        _, _, e = oracle.msbfs(g.row_offsets, g.col_indices, part, threads=threads)
+        t += time.perf_counter() - t0
+        edges += e
+        done += part.size
+    return {
+        "value": round(edges / t / 1e9, 4) if t > 0 else None,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} of {src.size} sources of the same workload, per-source BFS with lowest-id parents, "
+                  f"{threads} threads, {t:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16"])
+    ap.add_argument("--words-per-tile", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (PMC byte-counter calibration)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the multi-source BFS path shards by source tile with no "
+              "per-level exchange, and this build runs on a single MI355X; use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import xbmsbfs as xb
+
+    t0 = time.perf_counter()
+    g, src, desc = make_workload(args.workload)
+    S, n = int(src.size), g.n
+    log(f"workload {args.workload}: n={n} m={g.m} S={S} maxdeg={int(g.degrees().max())} "
+        f"({time.perf_counter() - t0:.1f}s)")
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    eng = xb.XBlossomGpuEngine10(hg)
+    tun = {}
+    if args.words_per_tile:
+        tun["words_per_tile"] = args.words_per_tile
+    lv = torch.empty((S, n), dtype=torch.int32, device="cuda")
+    pa = torch.empty((S, n), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    def run_batched(profile: bool):
+        eng.MultiSourceBFS(src, lv.data_ptr(), pa.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True,
+                           profile=profile, **tun)
+        return eng.last_stats
+
+    for _ in range(args.warmup):
+        run_batched(False)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    # Timed region: K batched steps, per-kernel HIP events on the engine's stream.
+    agg_ms = {}
+    agg_bytes = {}
+    agg_launch = {}
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    last = None
+    for _ in range(args.steps):
+        st = run_batched(True)
+        for name, k in st.kernels().items():
+            agg_ms[name] = agg_ms.get(name, 0.0) + k["ms"]
+            agg_bytes[name] = agg_bytes.get(name, 0.0) + k["bytes"]
+            agg_launch[name] = agg_launch.get(name, 0) + k["launches"]
+        last = st.as_dict()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t_start
+    edges = eng.CountTraversed(lv.data_ptr(), S)
+    gteps = edges * args.steps / t_batched / 1e9
+    ms_per_step = t_batched / args.steps * 1e3
+    log(f"batched: {ms_per_step:.2f} ms/step, {gteps:.1f} GTEPS, edges/step={edges}")
+
+    if args.copy_calibration_mib:
+        src_t = torch.ones(args.copy_calibration_mib * (1 << 18), dtype=torch.int32, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        torch.cuda.synchronize()
+        dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written")
+        del src_t, dst_t
+
+    # Plain per-source engine over the same batch.
+    plain = None
+    if not args.no_plain:
+        lv2 = torch.empty((S, n), dtype=torch.int32, device="cuda")
+        pa2 = torch.empty((S, n), dtype=torch.int32, device="cuda")
+        eng.MultiSourceBFS(src[:4], lv2.data_ptr(), pa2.data_ptr(), mode=xb.XB_MODE_PLAIN, device_out=True)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            eng.MultiSourceBFS(src, lv2.data_ptr(), pa2.data_ptr(), mode=xb.XB_MODE_PLAIN, device_out=True,
+                               profile=True)
+        torch.cuda.synchronize()
+        t_plain = (time.perf_counter() - tp0) / args.plain_steps
+        exact = bool(torch.equal(lv, lv2)) and bool(torch.equal(pa, pa2))
+        plain = {"gteps": round(edges / t_plain / 1e9, 3), "ms_per_batch": round(t_plain * 1e3, 2),
+                 "bit_exact_vs_batched": exact, "kernels": eng.last_stats.kernels()}
+        log(f"plain: {t_plain * 1e3:.1f} ms/batch, {plain['gteps']} GTEPS, bit-exact={exact}")
+        del lv2, pa2
+
+    # Roofline of the dominant kernel class (largest summed HIP-event time).
+    dom = max(agg_ms, key=lambda k: agg_ms[k])
+    avg_ms = agg_ms[dom] / agg_launch[dom]
+    bytes_per_launch = agg_bytes[dom] / agg_launch[dom]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(args.workload, dom)
+    roofline = {
+        "bound": "hbm",
+        "kernel": KERNEL_SYMBOL.get(dom, dom),
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "avg_launch_ms": round(avg_ms, 4),
+        "launches": agg_launch[dom],
+    }
+    # Whole-step view: section 8(d) bytes of the step over the step time.
+    step_bytes = sum(agg_bytes.values()) / args.steps
+    kernels = {k: {"ms_per_step": round(agg_ms[k] / args.steps, 3),
+                   "launches_per_step": agg_launch[k] / args.steps,
+                   "GBps": round(agg_bytes[k] / (agg_ms[k] * 1e-3) / 1e9, 1) if agg_ms[k] > 0 else None}
+               for k in sorted(agg_ms, key=lambda k: -agg_ms[k])}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(g, src, args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "n": n,
+            "m": int(g.m),
+            "sources": S,
+            "graph_checksum": f"{g.checksum():016x}",
+            "edges_traversed_per_step": int(edges),
+            "max_level": last["max_level"] if last else None,
+            "words_per_tile": args.words_per_tile or 8,
+        },
+        "plain_per_source": plain,
+        "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,
+        "roofline": roofline,
+        "step_algorithmic_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -62,33 +62,53 @@ extern "C" {
 
 typedef struct xb_graph xb_graph;
 
-/* Tunables of the batched engine.  Zero means "use the default". */
+/* Tunables of the batched engine.  Zero means "use the default".  Pull
+ * degree buckets (by in-degree): deg <= pull_cut_small gets T lanes per
+ * vertex (one per source word), deg <= pull_cut_medium 4T lanes, deg <=
+ * pull_cut_hub a whole wave, larger ones are split into hub_chunk-edge pieces
+ * with one wave each. */
 typedef struct xb_msbfs_opts {
   int32_t mode;            /* XB_MODE_*                                        */
   uint32_t flags;          /* XB_OUT_DEVICE | XB_PROFILE | ...                 */
   int32_t words_per_tile;  /* 64-bit source words traversed together: 1,2,4,8 */
   int32_t alpha;           /* pull when |F| + outdeg(F) > m / alpha            */
-  int32_t pull_heavy_cut;  /* in-degree above which pull uses a wave/vertex    */
+  int32_t pull_cut_small;
+  int32_t pull_cut_medium;
+  int32_t pull_cut_hub;
+  int32_t hub_chunk;
   int32_t emit_heavy_cut;  /* in-degree above which emit uses a wave/vertex    */
-  int32_t reserved[8];
+  int32_t reserved[7];
 } xb_msbfs_opts;
 
-/* Per-run statistics (filled when the pointer is non-null). */
+/* Kernel classes timed by XB_PROFILE (index into the xb_run_stats arrays). */
+#define XB_K_MEMSET 0     /* tile state clears (hipMemsetAsync)                */
+#define XB_K_SEED 1       /* source seeding                                     */
+#define XB_K_PUSH 2       /* bit-parallel top-down push                         */
+#define XB_K_COMMIT 3     /* commit of pushed / hub-pulled next words            */
+#define XB_K_PULL 4       /* bit-parallel bottom-up pull, degree buckets          */
+#define XB_K_PULL_HUB 5   /* bottom-up pull, hub adjacency chunks               */
+#define XB_K_CLEAR 6      /* frontier clear                                     */
+#define XB_K_EMIT_HEAVY 7 /* parent resolution for heavy vertices               */
+#define XB_K_EMIT 8       /* level/parent table write-out                       */
+#define XB_K_PLAIN_INIT 9 /* plain engine: row initialisation                   */
+#define XB_K_PLAIN_PUSH 10 /* plain engine: top-down push level                 */
+#define XB_K_CLASSES 16
+
+/* Per-run statistics (filled when the pointer is non-null).  kernel_ms and
+ * kernel_launches are filled only with XB_PROFILE; kernel_bytes always holds
+ * the algorithmic bytes of each class (the bytes a launch must move at
+ * minimum: CSR and state streams plus output, without redundant re-reads). */
 typedef struct xb_run_stats {
-  double ms_total;         /* HIP-event time of the whole run on its stream   */
-  double ms_init;          /* tile set-up (state clears, source seeding)      */
-  double ms_push;          /* top-down push + commit kernels                  */
-  double ms_pull;          /* bottom-up pull kernels                          */
-  double ms_emit;          /* level/parent table materialisation              */
-  int64_t launches_push;
-  int64_t launches_pull;
-  int64_t launches_emit;
-  int32_t max_level;       /* deepest level reached over all sources          */
-  int32_t tiles;           /* number of source tiles run                      */
+  double ms_total;                     /* HIP-event time of the whole run    */
+  double kernel_ms[XB_K_CLASSES];      /* summed HIP-event time per class    */
+  double kernel_bytes[XB_K_CLASSES];   /* summed algorithmic bytes per class */
+  int64_t kernel_launches[XB_K_CLASSES];
+  int32_t max_level;        /* deepest level reached over all sources        */
+  int32_t tiles;            /* number of source tiles run                    */
   int32_t push_levels;
   int32_t pull_levels;
-  int32_t fallback_sources;/* sources re-run on the plain path (depth > 254)   */
-  int32_t reserved[7];
+  int32_t fallback_sources; /* sources re-run on the plain path (depth > 254) */
+  int32_t reserved[11];
 } xb_run_stats;
 
 /* Fills *o with the defaults. */

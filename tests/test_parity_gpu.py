@@ -52,7 +52,7 @@ def test_batched_rmat(rmat12, T, alpha):
 def test_batched_heavy_kernels(rmat12, T):
     g, hg = rmat12
     src = xb.sample_sources(g, 200, seed=4)
-    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_heavy_cut=3, emit_heavy_cut=2)
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=1, pull_cut_medium=3, pull_cut_hub=6, hub_chunk=5, emit_heavy_cut=2)
 
 
 def test_plain_rmat(rmat12):
@@ -78,7 +78,7 @@ def test_directed_rmat():
     src = xb.sample_sources(g, 130, seed=1)
     for mode in (xb.XB_MODE_BATCHED, xb.XB_MODE_PLAIN):
         _check(g, hg, src, mode=mode)
-    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=2, pull_heavy_cut=2, emit_heavy_cut=2,
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=2, pull_cut_small=2, pull_cut_medium=2, pull_cut_hub=4, hub_chunk=3, emit_heavy_cut=2,
            alpha=1_000_000)
 
 
@@ -105,3 +105,15 @@ def test_medium_path_no_fallback():
     src = np.arange(0, 200, 3, dtype=np.int32)
     st = _check(g, hg, src, mode=xb.XB_MODE_BATCHED)
     assert st.fallback_sources == 0 and st.max_level == 199
+
+
+@pytest.mark.parametrize("T", [1, 2, 4, 8])
+def test_bucket_mix(T):
+    """Every pull bucket class plus hub pieces on one graph (cut-offs chosen so all are populated)."""
+    g = xb.gen_chung_lu(30_000, 2.1, 12.0, 3_000, seed=21)
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    src = xb.sample_sources(g, 64 * T + 5, seed=22)
+    deg = g.degrees()
+    assert (deg > 200).sum() > 0 and ((deg > 4) & (deg <= 20)).sum() > 0
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=4, pull_cut_medium=20,
+           pull_cut_hub=200, hub_chunk=64, emit_heavy_cut=30, alpha=100)

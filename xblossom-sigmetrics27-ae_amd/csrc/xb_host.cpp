@@ -163,10 +163,15 @@ struct xb_graph {
   int32_t* d_in_col = nullptr;  // == d_col when symmetric
   hipStream_t stream = nullptr;
   Pinned pinned;
-  // heavy-vertex lists (by in-degree), rebuilt when a cut changes
-  int32_t pull_cut = -1, emit_cut = -1;
-  int32_t* d_heavy_pull = nullptr;
-  int32_t n_heavy_pull = 0;
+  // In-degree layout for the pull and emit, rebuilt when a cut changes.
+  int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1, emit_cut = -1;
+  int32_t* d_perm = nullptr;          // in-degree > 0, bucket-major, ascending id per bucket
+  int32_t bucket_end[3] = {0, 0, 0};  // perm ranges of the small / medium / large buckets
+  int64_t bucket_edges[3] = {0, 0, 0};
+  xb::HubChunk* d_hub_chunks = nullptr;
+  int32_t n_hub_chunks = 0;
+  int32_t n_hubs = 0;
+  int64_t hub_edges = 0;
   int32_t* d_heavy_emit = nullptr;
   int32_t* d_hidx_emit = nullptr;
   int32_t n_heavy_emit = 0;
@@ -181,7 +186,8 @@ struct xb_graph {
     if (d_in_off != d_off) dfree(d_in_off);
     dfree(d_off);
     dfree(d_col);
-    dfree(d_heavy_pull);
+    dfree(d_perm);
+    dfree(d_hub_chunks);
     dfree(d_heavy_emit);
     dfree(d_hidx_emit);
     if (stream) (void)hipStreamDestroy(stream);
@@ -323,37 +329,89 @@ std::vector<int32_t> read_ints(const char* path) {
 }
 
 // ---------------------------------------------------------------------------
-// Heavy lists (by in-degree)
+// Batched-engine options
 // ---------------------------------------------------------------------------
-void ensure_heavy(xb_graph* g, int32_t pull_cut, int32_t emit_cut) {
+struct BPOpts {
+  int T;
+  int alpha;
+  int cut_small;
+  int cut_medium;
+  int cut_hub;
+  int hub_chunk;
+  int emit_cut;
+};
+
+// ---------------------------------------------------------------------------
+// In-degree layout: bucket order for the pull, hub pieces, heavy emit list
+// ---------------------------------------------------------------------------
+void ensure_layout(xb_graph* g, const BPOpts& o) {
   const auto& io = g->in_off();
-  if (pull_cut != g->pull_cut) {
-    std::vector<int32_t> h;
-    for (int32_t v = 0; v < g->n; ++v)
-      if (io[v + 1] - io[v] > pull_cut) h.push_back(v);
-    dfree(g->d_heavy_pull);
-    g->d_heavy_pull = dalloc<int32_t>(h.size());
-    if (!h.empty())
-      HIPCHK(hipMemcpy(g->d_heavy_pull, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
-    g->n_heavy_pull = (int32_t)h.size();
-    g->pull_cut = pull_cut;
+  const int32_t n = g->n;
+  if (o.cut_small != g->cut_small || o.cut_medium != g->cut_medium || o.cut_hub != g->cut_hub ||
+      o.hub_chunk != g->hub_chunk) {
+    auto cls_of = [&](int32_t d) { return d <= o.cut_small ? 0 : (d <= o.cut_medium ? 1 : (d <= o.cut_hub ? 2 : 3)); };
+    std::vector<int32_t> perm;
+    perm.reserve((size_t)n);
+    for (int c = 0; c < 3; ++c) {
+      int64_t edges = 0;
+      for (int32_t v = 0; v < n; ++v) {
+        const int32_t d = io[v + 1] - io[v];
+        if (d > 0 && cls_of(d) == c) {
+          perm.push_back(v);
+          edges += d;
+        }
+      }
+      g->bucket_end[c] = (int32_t)perm.size();
+      g->bucket_edges[c] = edges;
+    }
+    std::vector<xb::HubChunk> chunks;
+    int32_t hubs = 0;
+    int64_t hub_edges = 0;
+    for (int32_t v = 0; v < n; ++v) {
+      const int32_t d = io[v + 1] - io[v];
+      if (d > 0 && cls_of(d) == 3) {
+        ++hubs;
+        hub_edges += d;
+        for (int32_t jb = io[v]; jb < io[v + 1]; jb += o.hub_chunk)
+          chunks.push_back(xb::HubChunk{v, jb, std::min(jb + o.hub_chunk, io[v + 1]), 0});
+      }
+    }
+    dfree(g->d_perm);
+    dfree(g->d_hub_chunks);
+    g->d_perm = nullptr;
+    g->d_hub_chunks = nullptr;
+    g->d_perm = dalloc<int32_t>(perm.size());
+    if (!perm.empty())
+      HIPCHK(hipMemcpy(g->d_perm, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+    g->d_hub_chunks = dalloc<xb::HubChunk>(chunks.size());
+    if (!chunks.empty())
+      HIPCHK(hipMemcpy(g->d_hub_chunks, chunks.data(), sizeof(xb::HubChunk) * chunks.size(), hipMemcpyHostToDevice));
+    g->n_hub_chunks = (int32_t)chunks.size();
+    g->n_hubs = hubs;
+    g->hub_edges = hub_edges;
+    g->cut_small = o.cut_small;
+    g->cut_medium = o.cut_medium;
+    g->cut_hub = o.cut_hub;
+    g->hub_chunk = o.hub_chunk;
   }
-  if (emit_cut != g->emit_cut) {
-    std::vector<int32_t> h, idx((size_t)g->n, -1);
-    for (int32_t v = 0; v < g->n; ++v)
-      if (io[v + 1] - io[v] > emit_cut) {
+  if (o.emit_cut != g->emit_cut) {
+    std::vector<int32_t> h, idx((size_t)n, -1);
+    for (int32_t v = 0; v < n; ++v)
+      if (io[v + 1] - io[v] > o.emit_cut) {
         idx[v] = (int32_t)h.size();
         h.push_back(v);
       }
     dfree(g->d_heavy_emit);
     dfree(g->d_hidx_emit);
+    g->d_heavy_emit = nullptr;
+    g->d_hidx_emit = nullptr;
     g->d_heavy_emit = dalloc<int32_t>(h.size());
-    g->d_hidx_emit = dalloc<int32_t>((size_t)g->n);
+    g->d_hidx_emit = dalloc<int32_t>((size_t)n);
     if (!h.empty())
       HIPCHK(hipMemcpy(g->d_heavy_emit, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
-    if (g->n) HIPCHK(hipMemcpy(g->d_hidx_emit, idx.data(), sizeof(int32_t) * (size_t)g->n, hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(g->d_hidx_emit, idx.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
     g->n_heavy_emit = (int32_t)h.size();
-    g->emit_cut = emit_cut;
+    g->emit_cut = o.emit_cut;
   }
 }
 
@@ -395,6 +453,7 @@ void ensure_src(xb_graph* g, int64_t cnt) {
   Workspace& w = g->ws;
   if (w.src_cap >= cnt) return;
   dfree(w.d_src);
+  w.d_src = nullptr;
   w.d_src = dalloc<int32_t>((size_t)cnt);
   w.src_cap = cnt;
 }
@@ -404,6 +463,7 @@ void ensure_side(xb_graph* g) {
   const int64_t need = (int64_t)std::max(g->n_heavy_emit, 1) * 64;
   if (w.side_cap >= need) return;
   dfree(w.side);
+  w.side = nullptr;
   w.side = dalloc<int32_t>((size_t)need);
   w.side_cap = need;
 }
@@ -412,6 +472,7 @@ int32_t* ensure_stage(xb_graph* g, int64_t ints) {
   Workspace& w = g->ws;
   if (w.stage_cap < ints) {
     dfree(w.stage);
+    w.stage = nullptr;
     w.stage = dalloc<int32_t>((size_t)ints);
     w.stage_cap = ints;
   }
@@ -430,17 +491,21 @@ int32_t* ensure_q0_host(xb_graph* g, int64_t ints) {
 }
 
 // ---------------------------------------------------------------------------
-// Event timing
+// Per-class event timing and algorithmic-byte accounting
 // ---------------------------------------------------------------------------
-enum Cat { kInit = 0, kPush = 1, kPull = 2, kEmit = 3, kNumCats = 4 };
-
 struct Profiler {
   bool on = false;
   hipStream_t st = nullptr;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  struct Mark {
+    int cat;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
   std::vector<hipEvent_t> pool;
   hipEvent_t open_ev = nullptr;
   int open_cat = -1;
+  double bytes[XB_K_CLASSES] = {};
+  int64_t launches[XB_K_CLASSES] = {};
 
   hipEvent_t get() {
     if (!pool.empty()) {
@@ -453,6 +518,7 @@ struct Profiler {
     return e;
   }
   void begin(int cat) {
+    launches[cat]++;
     if (!on) return;
     open_ev = get();
     open_cat = cat;
@@ -462,24 +528,30 @@ struct Profiler {
     if (!on || open_cat < 0) return;
     hipEvent_t e = get();
     HIPCHK(hipEventRecord(e, st));
-    marks.push_back({open_cat, {open_ev, e}});
+    marks.push_back(Mark{open_cat, open_ev, e});
     open_cat = -1;
   }
-  void collect(double* ms) {
+  void add(int cat, double b) { bytes[cat] += b; }
+  void collect(xb_run_stats* s) {
     for (auto& mk : marks) {
       float t = 0;
-      HIPCHK(hipEventSynchronize(mk.second.second));
-      HIPCHK(hipEventElapsedTime(&t, mk.second.first, mk.second.second));
-      ms[mk.first] += t;
-      pool.push_back(mk.second.first);
-      pool.push_back(mk.second.second);
+      HIPCHK(hipEventSynchronize(mk.b));
+      HIPCHK(hipEventElapsedTime(&t, mk.a, mk.b));
+      if (s) s->kernel_ms[mk.cat] += t;
+      pool.push_back(mk.a);
+      pool.push_back(mk.b);
     }
     marks.clear();
+    if (s)
+      for (int c = 0; c < XB_K_CLASSES; ++c) {
+        s->kernel_bytes[c] = bytes[c];
+        s->kernel_launches[c] = launches[c];
+      }
   }
   ~Profiler() {
     for (auto& mk : marks) {
-      (void)hipEventDestroy(mk.second.first);
-      (void)hipEventDestroy(mk.second.second);
+      (void)hipEventDestroy(mk.a);
+      (void)hipEventDestroy(mk.b);
     }
     for (auto e : pool) (void)hipEventDestroy(e);
   }
@@ -503,29 +575,27 @@ void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows
   Workspace& w = g->ws;
   const int32_t n = g->n;
   int32_t* stage = nullptr;
-  if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)n);
-  int32_t* q0 = ensure_q0_host(g, 2);
+  if (!dev_out || !lvl_rows) stage = ensure_stage(g, 2 * (int64_t)n);
   for (int32_t s = 0; s < S; ++s) {
     const int32_t src = sources[s];
-    int32_t* lv = dev_out ? (lvl_rows ? lvl_rows + (int64_t)s * n : stage) : stage;
+    int32_t* lv = (dev_out && lvl_rows) ? lvl_rows + (int64_t)s * n : stage;
     int32_t* pa = dev_out ? par_rows + (int64_t)s * n : stage + n;
-    if (dev_out && !lvl_rows) lv = ensure_stage(g, n);
-    prof.begin(kInit);
-    xb::launch_plain_init(g->stream, lv, pa, n, src);
-    q0[0] = src;
-    q0[1] = 0;
-    HIPCHK(hipMemcpyAsync(w.q[0].v, &q0[0], sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
-    HIPCHK(hipMemcpyAsync(w.q[0].eo, &q0[1], sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+    prof.begin(XB_K_PLAIN_INIT);
+    xb::launch_plain_init(g->stream, lv, pa, n);
+    xb::launch_plain_set_source(g->stream, lv, pa, src, w.q[0]);
     prof.end();
+    prof.add(XB_K_PLAIN_INIT, 8.0 * n);
     int32_t qn = 1, E = g->outdeg(src);
     int cur = 0, d = 0;
     while (qn > 0) {
       HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), g->stream));
-      prof.begin(kPush);
+      prof.begin(XB_K_PLAIN_PUSH);
       xb::launch_plain_push(g->stream, g->d_off, g->d_col, w.q[cur], qn, E, lv, pa, d, w.q[cur ^ 1]);
       prof.end();
-      if (st) st->launches_push++;
+      const int32_t E_prev = E;
       read_ctr(g, w.q[cur ^ 1].ctr, &qn, &E);
+      // col + level gather per frontier edge; level + parent + queue write per new vertex.
+      prof.add(XB_K_PLAIN_PUSH, 8.0 * E_prev + 16.0 * qn);
       cur ^= 1;
       if (qn > 0) ++d;
     }
@@ -547,12 +617,6 @@ void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows
 // ---------------------------------------------------------------------------
 // Batched bit-parallel engine
 // ---------------------------------------------------------------------------
-struct BPOpts {
-  int T;
-  int alpha;
-  int pull_cut;
-  int emit_cut;
-};
 
 // Returns false when the tile got deeper than the 8-bit level code allows.
 bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, int32_t* lvl_rows,
@@ -562,16 +626,19 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   const int32_t n = g->n;
   const size_t nw = (size_t)n * (size_t)T;
   xb::BPState& s = w.bp;
-  prof.begin(kInit);
+  prof.begin(XB_K_MEMSET);
   if (w.bp_dirty) {
     HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * nw, g->stream));
     HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * nw, g->stream));
     HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * nw, g->stream));
     HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), g->stream));
+    prof.add(XB_K_MEMSET, 24.0 * nw + 4.0 * n);
     w.bp_dirty = false;
   }
   HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, g->stream));
   HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, g->stream));
+  prof.end();
+  prof.add(XB_K_MEMSET, 72.0 * nw);
   // First frontier: the distinct source vertices with their degree prefix.
   std::vector<int32_t> uniq(src, src + cnt);
   std::sort(uniq.begin(), uniq.end());
@@ -589,8 +656,10 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   HIPCHK(hipMemcpyAsync(w.q[0].v, q0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(w.q[0].eo, q0 + qn0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(w.d_src, q0 + 2 * qn0, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice, g->stream));
+  prof.begin(XB_K_SEED);
   xb::launch_bp_seed(g->stream, T, s, w.d_src, cnt);
   prof.end();
+  prof.add(XB_K_SEED, 28.0 * cnt);
 
   int32_t qn = qn0, E = (int32_t)E0;
   int cur = 0, d = 0;
@@ -606,24 +675,46 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
     xb::Queue& qc = w.q[cur];
     xb::Queue& qx = w.q[cur ^ 1];
     HIPCHK(hipMemsetAsync(qx.ctr, 0, sizeof(unsigned long long), g->stream));
+    HIPCHK(hipMemsetAsync(w.tcount, 0, sizeof(uint32_t), g->stream));
     if ((int64_t)qn + E > thresh) {
-      prof.begin(kPull);
-      xb::launch_bp_pull(g->stream, T, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off, n, o.pull_cut,
-                         g->d_heavy_pull, g->n_heavy_pull, qx);
-      xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
-      prof.end();
+      int32_t begin = 0;
+      for (int c = 0; c < 3; ++c) {
+        const int32_t end = g->bucket_end[c];
+        if (end > begin) {
+          prof.begin(XB_K_PULL);
+          xb::launch_bp_pull_bucket(g->stream, T, c, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off,
+                                    g->d_perm, begin, end, qx);
+          prof.end();
+          // offsets + in-edges + (front, seen r/w) per vertex and word
+          prof.add(XB_K_PULL, 4.0 * (end - begin + 1) + 4.0 * g->bucket_edges[c] + 24.0 * T * (end - begin));
+        }
+        begin = end;
+      }
+      if (g->n_hub_chunks > 0) {
+        prof.begin(XB_K_PULL_HUB);
+        xb::launch_bp_pull_hub(g->stream, T, s, cur, cnt, g->d_in_col, g->d_hub_chunks, g->n_hub_chunks);
+        prof.end();
+        prof.add(XB_K_PULL_HUB, 4.0 * g->hub_edges + 24.0 * T * g->n_hubs);
+        prof.begin(XB_K_COMMIT);
+        xb::launch_bp_commit(g->stream, T, s, cur, code, g->d_off, qx, g->n_hubs);
+        prof.end();
+      }
       ++pulls;
-      if (st) st->launches_pull += 2 + (g->n_heavy_pull > 0);
     } else {
-      prof.begin(kPush);
-      HIPCHK(hipMemsetAsync(w.tcount, 0, sizeof(uint32_t), g->stream));
+      prof.begin(XB_K_PUSH);
       xb::launch_bp_push(g->stream, T, s, cur, g->d_off, g->d_col, qc, qn, E);
+      prof.end();
+      // queue + frontier words of the queue, then col + seen words per edge
+      prof.add(XB_K_PUSH, (8.0 + 8.0 * T) * qn + (4.0 + 8.0 * T) * E);
+      prof.begin(XB_K_COMMIT);
       xb::launch_bp_commit(g->stream, T, s, cur, code, g->d_off, qx, (int32_t)std::min<int64_t>(E, n));
-      xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
       prof.end();
       ++pushes;
-      if (st) st->launches_push += 3;
     }
+    prof.begin(XB_K_CLEAR);
+    xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
+    prof.end();
+    prof.add(XB_K_CLEAR, (4.0 + 8.0 * T) * qn);
     read_ctr(g, qx.ctr, &qn, &E);
     cur ^= 1;
     if (qn > 0) ++d;
@@ -636,15 +727,23 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   int K = 0;
   while ((1 << K) <= d + 1) ++K;
   ensure_side(g);
-  prof.begin(kEmit);
   const int words = (cnt + 63) / 64;
   for (int t = 0; t < words; ++t) {
-    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, g->d_in_off, g->d_in_col, n, o.emit_cut, g->d_heavy_emit,
-                       g->d_hidx_emit, g->n_heavy_emit, w.side, lvl_rows, par_rows);
+    if (g->n_heavy_emit > 0) {
+      prof.begin(XB_K_EMIT_HEAVY);
+      xb::launch_bp_emit_heavy(g->stream, T, s, t, K, w.d_src, g->d_in_off, g->d_in_col, g->d_heavy_emit,
+                               g->n_heavy_emit, w.side);
+      prof.end();
+    }
+    prof.begin(XB_K_EMIT);
+    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.emit_cut,
+                       g->d_hidx_emit, w.side, lvl_rows, par_rows);
+    prof.end();
+    const int rows_w = std::min(64, cnt - 64 * t);
+    // level + parent int32 per (source, vertex) of this word
+    prof.add(XB_K_EMIT, (lvl_rows ? 8.0 : 4.0) * rows_w * (double)n);
   }
-  prof.end();
   if (st) {
-    st->launches_emit += (int64_t)words * (1 + (g->n_heavy_emit > 0));
     st->max_level = std::max(st->max_level, d);
     st->push_levels += pushes;
     st->pull_levels += pulls;
@@ -657,7 +756,7 @@ void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S
                  bool dev_out, Profiler& prof, xb_run_stats* st) {
   ensure_queues(g);
   ensure_bp(g, o.T);
-  ensure_heavy(g, o.pull_cut, o.emit_cut);
+  ensure_layout(g, o);
   const int32_t n = g->n;
   const int32_t tile = 64 * o.T;
   ensure_src(g, tile);
@@ -665,15 +764,14 @@ void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S
   if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)tile * n);
   for (int32_t base = 0; base < S; base += tile) {
     const int32_t cnt = std::min(tile, S - base);
-    int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : stage;
+    int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : (lvl ? stage : nullptr);
     int32_t* pr = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
-    if (!dev_out && !lvl) lr = nullptr;
     bool ok = run_bp_tile(g, o, sources + base, cnt, lr, pr, prof, st);
     if (!ok) {
       // Deeper than 254 levels: these sources take the plain path.
       if (st) st->fallback_sources += cnt;
-      run_plain(g, sources + base, cnt, dev_out ? lr : (lvl ? lvl + (int64_t)base * n : nullptr),
-                dev_out ? pr : par + (int64_t)base * n, dev_out, prof, st);
+      run_plain(g, sources + base, cnt, lvl ? lvl + (int64_t)base * n : nullptr, par + (int64_t)base * n,
+                dev_out, prof, st);
       continue;
     }
     if (!dev_out) {
@@ -703,8 +801,11 @@ int32_t xb_opts_default(xb_msbfs_opts* o) {
   o->mode = XB_MODE_AUTO;
   o->words_per_tile = 8;
   o->alpha = 20;
-  o->pull_heavy_cut = 512;
-  o->emit_heavy_cut = 256;
+  o->pull_cut_small = 8;
+  o->pull_cut_medium = 64;
+  o->pull_cut_hub = 1024;
+  o->hub_chunk = 1024;
+  o->emit_heavy_cut = 64;
   return XB_OK;
 }
 
@@ -757,14 +858,19 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
       o.flags = opts->flags;
       if (opts->words_per_tile) o.words_per_tile = opts->words_per_tile;
       if (opts->alpha) o.alpha = opts->alpha;
-      if (opts->pull_heavy_cut) o.pull_heavy_cut = opts->pull_heavy_cut;
+      if (opts->pull_cut_small) o.pull_cut_small = opts->pull_cut_small;
+      if (opts->pull_cut_medium) o.pull_cut_medium = opts->pull_cut_medium;
+      if (opts->pull_cut_hub) o.pull_cut_hub = opts->pull_cut_hub;
+      if (opts->hub_chunk) o.hub_chunk = opts->hub_chunk;
       if (opts->emit_heavy_cut) o.emit_heavy_cut = opts->emit_heavy_cut;
     }
     if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
     if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
     const int T = o.words_per_tile;
     if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
-    if (o.alpha < 1 || o.pull_heavy_cut < 1 || o.emit_heavy_cut < 1) fail(XB_ERR_INVALID_ARG, "bad tunable");
+    if (o.alpha < 1 || o.pull_cut_small < 1 || o.pull_cut_medium < o.pull_cut_small ||
+        o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1 || o.emit_heavy_cut < 1)
+      fail(XB_ERR_INVALID_ARG, "bad tunable");
     for (int32_t i = 0; i < S; ++i)
       if (sources[i] < 0 || sources[i] >= g->n)
         fail(XB_ERR_SOURCE_RANGE, "source " + std::to_string(i) + " = " + std::to_string(sources[i]) +
@@ -781,7 +887,7 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     if (o.mode == XB_MODE_PLAIN) {
       run_plain(g, sources, S, levels, parents, dev_out, prof, stats);
     } else {
-      BPOpts bo{T, o.alpha, o.pull_heavy_cut, o.emit_heavy_cut};
+      BPOpts bo{T, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk, o.emit_heavy_cut};
       run_batched(g, bo, sources, S, levels, parents, dev_out, prof, stats);
     }
     HIPCHK(hipEventRecord(e1, g->stream));
@@ -791,13 +897,8 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (stats) {
-      double cats[kNumCats] = {0, 0, 0, 0};
-      prof.collect(cats);
+      prof.collect(stats);
       stats->ms_total = ms;
-      stats->ms_init = cats[kInit];
-      stats->ms_push = cats[kPush];
-      stats->ms_pull = cats[kPull];
-      stats->ms_emit = cats[kEmit];
     }
   });
 }

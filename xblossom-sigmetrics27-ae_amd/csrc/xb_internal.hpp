@@ -18,8 +18,8 @@ struct BPState {
   uint64_t* nextw;
   uint64_t* planes;
   uint32_t* tflag;
-  int32_t* tlist;          // touched vertices of a push level
-  uint32_t* tcount;        // device count of tlist
+  int32_t* tlist;    // vertices whose next words were OR-ed atomically
+  uint32_t* tcount;  // device count of tlist
 };
 
 // Frontier queue: vertex ids plus the exclusive prefix of their out-degrees
@@ -31,8 +31,14 @@ struct Queue {
   unsigned long long* ctr;
 };
 
+// A hub adjacency piece for the pull: in-edges [jb, je) of vertex v.
+struct HubChunk {
+  int32_t v, jb, je, pad;
+};
+
 // ---- plain per-source engine ----
-void launch_plain_init(hipStream_t st, int32_t* level, int32_t* parent, int32_t n, int32_t src);
+void launch_plain_init(hipStream_t st, int32_t* level, int32_t* parent, int32_t n);
+void launch_plain_set_source(hipStream_t st, int32_t* level, int32_t* parent, int32_t src, Queue q);
 void launch_plain_push(hipStream_t st, const int32_t* off, const int32_t* col, Queue cur, int32_t qn,
                        int32_t E, int32_t* level, int32_t* parent, int32_t d, Queue nxt);
 
@@ -42,14 +48,20 @@ void launch_bp_push(hipStream_t st, int T, BPState s, int cur, const int32_t* of
                     Queue q, int32_t qn, int32_t E);
 void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, const int32_t* out_off,
                       Queue nxt, int32_t max_touched);
-void launch_bp_pull(hipStream_t st, int T, BPState s, int cur, int32_t code, int32_t cnt,
-                    const int32_t* in_off, const int32_t* in_col, const int32_t* out_off, int32_t n,
-                    int32_t heavy_cut, const int32_t* heavy, int32_t n_heavy, Queue nxt);
+// cls 0: T lanes per vertex, 1: 4T lanes, 2: one wave per vertex.
+void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
+                           const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
+                           const int32_t* perm, int32_t begin, int32_t end, Queue nxt);
+void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
+                        const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
-void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows,
+// src: the tile's source ids (level-1 parents are the sources themselves).
+void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* src,
+                          const int32_t* in_off, const int32_t* in_col, const int32_t* heavy, int32_t n_heavy,
+                          int32_t* side);
+void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
-                    const int32_t* heavy, const int32_t* hidx, int32_t n_heavy, int32_t* side,
-                    int32_t* lvl_out, int32_t* par_out);
+                    const int32_t* hidx, const int32_t* side, int32_t* lvl_out, int32_t* par_out);
 
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

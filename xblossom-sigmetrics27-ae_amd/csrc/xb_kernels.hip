@@ -4,24 +4,28 @@
 // Two engines share the frontier-queue machinery:
 //
 //  * plain per-source engine: one top-down push BFS per source.  Each level is
-//    one edge-balanced launch over the frontier queue: the queue carries the
-//    exclusive prefix of its vertices' out-degrees, and every lane finds the
-//    (frontier vertex, neighbour offset) of its edge by a rightmost-<= search
-//    (the mapping of apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:
-//    237-263 + apps/xb/src/include/utils/cuda_algo.h:7-20, here without a
-//    separate scan pass: the prefix is produced by the compaction itself).
-//    Claims use atomicCAS on the level, parents atomicMin (lowest id wins),
-//    replacing the first-winner atomicExch lock of
-//    apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:325-343.
+//    one edge-balanced launch over the frontier queue.  The queue carries the
+//    exclusive prefix of its vertices' out-degrees, so a block takes a fixed
+//    chunk of edges, finds the owners of the chunk's first and last edge by a
+//    rightmost-<= search, stages the owners' offsets in LDS and maps each edge
+//    to (frontier vertex, neighbour offset) with an LDS search: the mapping of
+//    apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:237-263 and
+//    apps/xb/src/include/utils/cuda_algo.h:7-20 without a separate scan pass
+//    (the prefix comes out of the compaction itself).  Claims use atomicCAS on
+//    the level, parents atomicMin (lowest id wins), replacing the first-winner
+//    atomicExch lock of apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:325-343.
 //
 //  * batched bit-parallel engine: 64 sources per 64-bit word, T words per
-//    tile.  Levels alternate between a top-down push (edge-balanced, atomicOr
-//    into a next-word array, then a commit over the touched vertices) and a
-//    bottom-up pull over bitmap frontiers (lane groups per vertex for light
-//    vertices in id order, one wave per heavy vertex).  Level codes are kept
-//    as 8 bit-sliced planes per (vertex, word).  After the last level an emit
-//    pass writes the int32 level and parent tables row by row, resolving each
-//    parent as the lowest-id in-neighbour whose code is one less.
+//    tile.  Levels are either a top-down push (edge-balanced as above,
+//    atomicOr into next words, then a commit over the touched vertices) or a
+//    bottom-up pull over the frontier words: vertices are taken in in-degree
+//    bucket order (T, 4T or 64 lanes per vertex, each lane issuing U
+//    independent neighbour loads per step, OR-reduced across the vertex's
+//    lanes), and hub adjacency lists are cut into fixed pieces with one wave
+//    each.  Level codes are kept as 8 bit-sliced planes per (vertex, word).
+//    After the last level an emit pass writes the int32 level and parent
+//    tables in vertex order, resolving each parent as the lowest-id
+//    in-neighbour whose code is one less.
 //
 // Frontier compaction is wave-aggregated: a ballot gives each lane its rank,
 // a wave prefix sum gives its edge offset, and one 64-bit atomicAdd on a
@@ -39,7 +43,11 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 4096;
+constexpr int kMaxBlocks = 8192;
+constexpr int kPushR = 4;                 // edges per thread per chunk
+constexpr int kChunk = kBlock * kPushR;   // edges per block chunk
+constexpr int kPullU = 4;                 // independent neighbour loads per lane per step
+constexpr int kEmitU = 4;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
@@ -104,10 +112,9 @@ __device__ __forceinline__ void wave_push_list(bool pred, int32_t v, int32_t* li
   if (pred) list[old + rank] = v;
 }
 
-// Rightmost i in [0, qn) with eo[i] <= e (eo[0] == 0 <= e).  Zero-degree
+// Rightmost i in [lo, hi) with eo[i] <= e, given eo[lo] <= e.  Zero-degree
 // entries share their successor's offset and are skipped by "rightmost".
-__device__ __forceinline__ int owner_of(const int32_t* __restrict__ eo, int qn, int e) {
-  int lo = 0, hi = qn;
+__device__ __forceinline__ int owner_in(const int32_t* __restrict__ eo, int lo, int hi, int e) {
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (eo[mid] <= e)
@@ -116,6 +123,120 @@ __device__ __forceinline__ int owner_of(const int32_t* __restrict__ eo, int qn, 
       hi = mid;
   }
   return lo;
+}
+
+// Block-level owner staging for an edge chunk [c0, c1) of a frontier queue.
+// On return s_eo[0..cnt) = eo[lo..lo+cnt) when cnt <= kChunk + 1 (else the
+// owners are searched in global memory between lo and lo+cnt).
+struct ChunkOwners {
+  int lo;
+  int cnt;
+};
+
+__device__ __forceinline__ ChunkOwners stage_owners(const int32_t* __restrict__ eo, int qn, int c0, int c1,
+                                                    int32_t* s_eo, int* s_lohi) {
+  if (threadIdx.x == 0) s_lohi[0] = owner_in(eo, 0, qn, c0);
+  if (threadIdx.x == kWave) s_lohi[1] = owner_in(eo, 0, qn, c1 - 1);
+  __syncthreads();
+  ChunkOwners co{s_lohi[0], s_lohi[1] - s_lohi[0] + 1};
+  if (co.cnt <= kChunk + 1)
+    for (int i = threadIdx.x; i < co.cnt; i += blockDim.x) s_eo[i] = eo[co.lo + i];
+  __syncthreads();
+  return co;
+}
+
+__device__ __forceinline__ int chunk_owner(const ChunkOwners& co, const int32_t* __restrict__ eo,
+                                           const int32_t* s_eo, int e) {
+  if (co.cnt <= kChunk + 1) return co.lo + owner_in(s_eo, 0, co.cnt, e);
+  return owner_in(eo, co.lo, co.lo + co.cnt, e);
+}
+
+// ---- block-level LDS staging of queue appends ----
+// Lanes stage (vertex, degree) entries in LDS (one LDS atomic per wave); the
+// block flushes the stage to the global queue with ONE global atomic per
+// flush, after a block prefix sum over the staged degrees.  Callers run in
+// block-uniform loops and call stage_sync_flush() once per iteration, which
+// flushes when another iteration's worth of entries might not fit.
+constexpr int kStageCap = 2048;
+
+struct AppendStage {
+  int32_t v[kStageCap];
+  int32_t d[kStageCap];
+  int cnt;
+  int wsum[kBlock / kWave];
+  unsigned long long base;
+};
+
+__device__ __forceinline__ void stage_init(AppendStage& sg) {
+  if (threadIdx.x == 0) sg.cnt = 0;
+  __syncthreads();
+}
+
+// Wave-uniform call.
+__device__ __forceinline__ void stage_put(AppendStage& sg, bool pred, int32_t v, int32_t deg) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int rank = __popcll(mask & lanemask_lt());
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(&sg.cnt, (int)__popcll(mask));
+  base = __shfl(base, leader, kWave);
+  if (pred) {
+    sg.v[base + rank] = v;
+    sg.d[base + rank] = deg;
+  }
+}
+
+// Block-uniform call.  with_eo: write the degree prefix too (frontier
+// queues); otherwise only ids (touched lists, 32-bit counter in ctr32).
+__device__ void stage_flush(AppendStage& sg, int cnt, Queue q, int32_t* list, uint32_t* ctr32) {
+  const int per = (cnt + kBlock - 1) / kBlock;
+  const int i0 = min((int)threadIdx.x * per, cnt);
+  const int i1 = min(i0 + per, cnt);
+  int sum = 0;
+  if (!list)
+    for (int i = i0; i < i1; ++i) sum += sg.d[i];
+  const int incl = wave_incl_scan(sum);
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) sg.wsum[wv] = incl;
+  __syncthreads();
+  int woff = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wv) woff += sg.wsum[w];
+    total += sg.wsum[w];
+  }
+  if (threadIdx.x == 0) {
+    if (list)
+      sg.base = atomicAdd(ctr32, (uint32_t)cnt);
+    else
+      sg.base = atomicAdd(q.ctr, ((unsigned long long)(uint32_t)total << 32) | (unsigned long long)(uint32_t)cnt);
+  }
+  __syncthreads();
+  const uint32_t bc = (uint32_t)(sg.base & 0xFFFFFFFFull);
+  if (list) {
+    for (int i = threadIdx.x; i < cnt; i += kBlock) list[bc + i] = sg.v[i];
+  } else {
+    int run = (int)(sg.base >> 32) + woff + incl - sum;
+    for (int i = i0; i < i1; ++i) {
+      q.v[bc + i] = sg.v[i];
+      q.eo[bc + i] = run;
+      run += sg.d[i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sg.cnt = 0;
+  __syncthreads();
+}
+
+// Block-uniform call once per loop iteration (force = last iteration).
+__device__ __forceinline__ void stage_sync_flush(AppendStage& sg, bool force, Queue q, int32_t* list = nullptr,
+                                                 uint32_t* ctr32 = nullptr) {
+  __syncthreads();
+  const int cnt = sg.cnt;
+  __syncthreads();
+  if (cnt == 0 || (!force && cnt <= kStageCap - kBlock)) return;
+  stage_flush(sg, cnt, q, list, ctr32);
 }
 
 inline int grid_for(int64_t items, int per_block) {
@@ -130,47 +251,80 @@ inline int grid_for(int64_t items, int per_block) {
 // ============================================================================
 
 __global__ __launch_bounds__(kBlock) void k_plain_init(int32_t* __restrict__ level, int32_t* __restrict__ parent,
-                                                       int32_t n, int32_t src) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const bool s = (i == src);
-    level[i] = s ? 0 : -1;
-    parent[i] = s ? src : -1;
+                                                       int32_t n) {
+  const int64_t n4 = n / 4;
+  int4* l4 = reinterpret_cast<int4*>(level);
+  int4* p4 = reinterpret_cast<int4*>(parent);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(level) | reinterpret_cast<uintptr_t>(parent)) & 15) == 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t done = 0;
+  if (aligned) {
+    const int4 m1 = make_int4(-1, -1, -1, -1);
+    for (int64_t i = tid; i < n4; i += stride) {
+      l4[i] = m1;
+      p4[i] = m1;
+    }
+    done = n4 * 4;
+  }
+  for (int64_t i = done + tid; i < n; i += stride) {
+    level[i] = -1;
+    parent[i] = -1;
   }
 }
 
-// One top-down level: every lane takes one frontier edge (v -> u).
+__global__ void k_plain_set_source(int32_t* __restrict__ level, int32_t* __restrict__ parent, int32_t src,
+                                   Queue q) {
+  if (threadIdx.x == 0) {
+    level[src] = 0;
+    parent[src] = src;
+    q.v[0] = src;
+    q.eo[0] = 0;
+  }
+}
+
+// One top-down level: every thread takes kPushR frontier edges (v -> u).
 __global__ __launch_bounds__(kBlock) void k_plain_push(const int32_t* __restrict__ off,
                                                        const int32_t* __restrict__ col, Queue cur, int32_t qn,
                                                        int32_t E, int32_t* __restrict__ level,
                                                        int32_t* __restrict__ parent, int32_t d, Queue nxt) {
-  const int l = lane_id();
-  const int64_t stride = num_waves() * kWave;
-  for (int64_t e0 = global_wave() * kWave; e0 < E; e0 += stride) {
-    const int e = (int)(e0 + l);
-    bool fresh = false;
-    int32_t u = 0, du = 0;
-    if (e < E) {
-      const int i = owner_of(cur.eo, qn, e);
-      const int32_t v = cur.v[i];
-      u = col[off[v] + (e - cur.eo[i])];
-      int32_t lu = level[u];
-      if (lu < 0) {
-        const int32_t old = atomicCAS(&level[u], -1, d + 1);
-        if (old < 0) {
-          fresh = true;
-          lu = d + 1;
-        } else {
-          lu = old;
+  __shared__ int32_t s_eo[kChunk + 1];
+  __shared__ int s_lohi[2];
+  __shared__ AppendStage sg;
+  stage_init(sg);
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+    const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
+    const ChunkOwners co = stage_owners(cur.eo, qn, (int)c0, c1, s_eo, s_lohi);
+#pragma unroll 1
+    for (int r = 0; r < kPushR; ++r) {
+      const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
+      bool fresh = false;
+      int32_t u = 0, du = 0;
+      if (e < c1) {
+        const int i = chunk_owner(co, cur.eo, s_eo, e);
+        const int32_t v = cur.v[i];
+        u = col[off[v] + (e - cur.eo[i])];
+        int32_t lu = level[u];
+        if (lu < 0) {
+          const int32_t old = atomicCAS(&level[u], -1, d + 1);
+          if (old < 0) {
+            fresh = true;
+            lu = d + 1;
+          } else {
+            lu = old;
+          }
         }
+        if (lu == d + 1) {
+          // Parents only decrease, so a stale read can only overestimate.
+          if ((uint32_t)parent[u] > (uint32_t)v) atomicMin((uint32_t*)&parent[u], (uint32_t)v);
+        }
+        if (fresh) du = off[u + 1] - off[u];
       }
-      if (lu == d + 1) {
-        // Parents only decrease, so a stale read can only overestimate.
-        if ((uint32_t)parent[u] > (uint32_t)v) atomicMin((uint32_t*)&parent[u], (uint32_t)v);
-      }
-      if (fresh) du = off[u + 1] - off[u];
+      stage_put(sg, fresh, u, du);
+      stage_sync_flush(sg, false, nxt);
     }
-    wave_append(fresh, u, du, nxt);
   }
+  stage_sync_flush(sg, true, nxt);
 }
 
 // ============================================================================
@@ -202,53 +356,59 @@ __global__ __launch_bounds__(kBlock) void k_bp_seed(BPState s, const int32_t* __
   }
 }
 
-// Top-down push over the frontier queue, one edge per lane.
+// Top-down push over the frontier queue, kPushR edges per thread.
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_push(BPState s, int cur, const int32_t* __restrict__ off,
                                                     const int32_t* __restrict__ col, Queue q, int32_t qn,
                                                     int32_t E) {
-  const int l = lane_id();
+  __shared__ int32_t s_eo[kChunk + 1];
+  __shared__ int s_lohi[2];
+  __shared__ AppendStage sg;
+  stage_init(sg);
   const uint64_t* __restrict__ front = s.front[cur];
-  const int64_t stride = num_waves() * kWave;
-  for (int64_t e0 = global_wave() * kWave; e0 < E; e0 += stride) {
-    const int e = (int)(e0 + l);
-    bool first = false;
-    int32_t u = 0;
-    if (e < E) {
-      const int i = owner_of(q.eo, qn, e);
-      const int32_t v = q.v[i];
-      u = col[off[v] + (e - q.eo[i])];
-      bool any = false;
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+    const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, s_eo, s_lohi);
+#pragma unroll 1
+    for (int r = 0; r < kPushR; ++r) {
+      const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
+      bool first = false;
+      int32_t u = 0;
+      if (e < c1) {
+        const int i = chunk_owner(co, q.eo, s_eo, e);
+        const int32_t v = q.v[i];
+        u = col[off[v] + (e - q.eo[i])];
+        bool any = false;
 #pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const uint64_t f = front[(int64_t)v * T + t];
-        if (!f) continue;
-        uint64_t nb = f & ~s.seen[(int64_t)u * T + t];
-        if (!nb) continue;
-        nb &= ~s.nextw[(int64_t)u * T + t];
-        if (!nb) {
-          any = true;  // another lane already offered these bits
-          continue;
+        for (int t = 0; t < T; ++t) {
+          const uint64_t f = front[(int64_t)v * T + t];
+          if (!f) continue;
+          uint64_t nb = f & ~s.seen[(int64_t)u * T + t];
+          if (!nb) continue;
+          any = true;
+          nb &= ~s.nextw[(int64_t)u * T + t];
+          if (nb) atomicOr((unsigned long long*)&s.nextw[(int64_t)u * T + t], (unsigned long long)nb);
         }
-        atomicOr((unsigned long long*)&s.nextw[(int64_t)u * T + t], (unsigned long long)nb);
-        any = true;
+        if (any && s.tflag[u] == 0u) first = (atomicExch(&s.tflag[u], 1u) == 0u);
       }
-      if (any && s.tflag[u] == 0u) first = (atomicExch(&s.tflag[u], 1u) == 0u);
+      stage_put(sg, first, u, 0);
+      stage_sync_flush(sg, false, Queue{}, s.tlist, s.tcount);
     }
-    wave_push_list(first, u, s.tlist, s.tcount);
   }
+  stage_sync_flush(sg, true, Queue{}, s.tlist, s.tcount);
 }
 
-// Commit a push level: move next-words into seen/front, record level codes.
+// Commit atomically gathered next words (push levels and pulled hubs): move
+// them into seen/front, record level codes, append to the next queue.
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32_t code,
                                                       const int32_t* __restrict__ out_off, Queue nxt) {
-  const int l = lane_id();
+  __shared__ AppendStage sg;
+  stage_init(sg);
   const uint32_t cnt = *s.tcount;
   uint64_t* __restrict__ fnext = s.front[cur ^ 1];
-  const int64_t stride = num_waves() * kWave;
-  for (int64_t b0 = global_wave() * kWave; b0 < (int64_t)cnt; b0 += stride) {
-    const int64_t idx = b0 + l;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < (int64_t)cnt; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t idx = b0 + threadIdx.x;
     bool any = false;
     int32_t u = 0, du = 0;
     if (idx < (int64_t)cnt) {
@@ -270,103 +430,129 @@ __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32
       s.tflag[u] = 0u;
       if (any) du = out_off[u + 1] - out_off[u];
     }
-    wave_append(any, u, du, nxt);
+    stage_put(sg, any, u, du);
+    stage_sync_flush(sg, false, nxt);
   }
+  stage_sync_flush(sg, true, nxt);
 }
 
-// Bottom-up pull for light vertices (in-degree <= heavy_cut), in id order.
-// T lanes per vertex, one source word each; lanes of a vertex walk the same
-// in-neighbour list and stop when their word has nothing left to find.
-template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_pull_light(BPState s, int cur, uint32_t code, int32_t cnt,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ out_off, int32_t n,
-                                                          int32_t heavy_cut, Queue nxt) {
-  constexpr int VPW = kWave / T;
+// Bottom-up pull over one in-degree bucket of the vertex order perm[begin,
+// end).  G = T*P lanes per vertex: lane (slot, p, t) handles source word t and
+// in-neighbours j = b + k + i*P + p (i < kPullU) of step k; the P partitions
+// are OR-reduced after every step so every lane of the vertex sees the same
+// remaining need and the group stops together.
+template <int T, int P>
+__global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
+                                                           const int32_t* __restrict__ in_off,
+                                                           const int32_t* __restrict__ in_col,
+                                                           const int32_t* __restrict__ out_off,
+                                                           const int32_t* __restrict__ perm, int32_t begin,
+                                                           int32_t end, Queue nxt) {
+  constexpr int G = T * P;
+  constexpr int VPW = kWave / G;
+  constexpr int VPB = (kBlock / kWave) * VPW;
+  __shared__ AppendStage sg;
+  stage_init(sg);
   const int l = lane_id();
   const int t = l % T;
-  const int slot = l / T;
+  const int p = (l / T) % P;
+  const int slot = l / G;
+  const int wv = threadIdx.x / kWave;
   const uint64_t vmask = word_mask(cnt, t);
   const uint64_t* __restrict__ front = s.front[cur];
   uint64_t* __restrict__ fnext = s.front[cur ^ 1];
-  const int64_t stride = num_waves() * VPW;
-  for (int64_t vb = global_wave() * VPW; vb < n; vb += stride) {
-    const int32_t v = (int32_t)(vb + slot);
-    uint64_t got = 0;
-    bool act = v < n;
-    int32_t b = 0, e = 0;
+  for (int64_t bb = begin + (int64_t)blockIdx.x * VPB; bb < end; bb += (int64_t)gridDim.x * VPB) {
+    const int64_t idx = bb + wv * VPW + slot;
+    const bool act = idx < end;
+    int32_t v = 0, b = 0, e = 0;
+    uint64_t old = 0, need = 0;
     if (act) {
+      v = perm[idx];
       b = in_off[v];
       e = in_off[v + 1];
-      if (e - b > heavy_cut) act = false;
+      old = s.seen[(int64_t)v * T + t];
+      need = ~old & vmask;
     }
-    if (act) {
+    uint64_t got = 0;
+    for (int32_t k = 0;; k += P * kPullU) {
+      const bool more = need != 0 && (b + k + p < e);
+      if (!__any(more)) break;
+      uint64_t g = 0;
+      if (more) {
+        int32_t us[kPullU];
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i) {
+          const int32_t j = b + k + i * P + p;
+          us[i] = (j < e) ? in_col[j] : -1;
+        }
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i)
+          if (us[i] >= 0) g |= front[(int64_t)us[i] * T + t];
+        g &= need;
+      }
+#pragma unroll
+      for (int x = T; x < G; x <<= 1) g |= __shfl_xor(g, x, kWave);
+      got |= g;
+      need &= ~g;
+    }
+    if (act && p == 0 && got) {
       const int64_t w = (int64_t)v * T + t;
-      const uint64_t old = s.seen[w];
-      uint64_t need = ~old & vmask;
-      for (int32_t j = b; j < e && need; ++j) {
-        const int32_t u = in_col[j];
-        const uint64_t g = front[(int64_t)u * T + t] & need;
-        got |= g;
-        need &= ~g;
-      }
-      if (got) {
-        s.seen[w] = old | got;
-        fnext[w] = got;
-        planes_or(&s.planes[w * 8], code, got);
-      }
+      s.seen[w] = old | got;
+      fnext[w] = got;
+      planes_or(&s.planes[w * 8], code, got);
     }
-    const uint64_t bal = __ballot(got != 0);
-    const uint64_t grp = (T == 64) ? bal : ((bal >> (slot * T)) & ((1ull << T) - 1ull));
-    const bool lead = act && t == 0 && grp != 0;
+    const uint64_t bal = __ballot(act && p == 0 && got != 0);
+    const uint64_t grp = (bal >> (slot * G)) & ((T == 64) ? ~0ull : ((1ull << T) - 1ull));
+    const bool lead = act && (l % G) == 0 && grp != 0;
     const int32_t dv = lead ? out_off[v + 1] - out_off[v] : 0;
-    wave_append(lead, v, dv, nxt);
+    stage_put(sg, lead, v, dv);
+    stage_sync_flush(sg, false, nxt);
   }
+  stage_sync_flush(sg, true, nxt);
 }
 
-// Bottom-up pull for heavy vertices: one wave per vertex, 64/T in-neighbours
-// per step, OR-reduced across the wave.
+// Bottom-up pull over hub adjacency pieces: one wave per piece, results
+// OR-ed atomically into the next words and committed by k_bp_commit.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_pull_heavy(BPState s, int cur, uint32_t code, int32_t cnt,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ out_off,
-                                                          const int32_t* __restrict__ heavy, int32_t n_heavy,
-                                                          Queue nxt) {
-  constexpr int NS = kWave / T;
+__global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int32_t cnt,
+                                                        const int32_t* __restrict__ in_col,
+                                                        const HubChunk* __restrict__ chunks, int32_t n_chunks) {
+  constexpr int P = kWave / T;
   const int l = lane_id();
   const int t = l % T;
-  const int slot = l / T;
+  const int p = l / T;
   const uint64_t vmask = word_mask(cnt, t);
   const uint64_t* __restrict__ front = s.front[cur];
-  uint64_t* __restrict__ fnext = s.front[cur ^ 1];
-  for (int64_t h = global_wave(); h < n_heavy; h += num_waves()) {
-    const int32_t v = heavy[h];
-    const int32_t b = in_off[v], e = in_off[v + 1];
-    const int64_t w = (int64_t)v * T + t;
-    const uint64_t old = s.seen[w];
-    uint64_t need = ~old & vmask;
+  for (int64_t c = global_wave(); c < n_chunks; c += num_waves()) {
+    const HubChunk hc = chunks[c];
+    const int64_t w = (int64_t)hc.v * T + t;
+    uint64_t need = ~s.seen[w] & vmask;
     uint64_t got = 0;
-    for (int32_t j0 = b; j0 < e; j0 += NS) {
+    for (int32_t k = hc.jb; k < hc.je; k += P * kPullU) {
       if (!__any(need != 0)) break;
-      const int32_t j = j0 + slot;
       uint64_t g = 0;
-      if (j < e && need) g = front[(int64_t)in_col[j] * T + t] & need;
+      if (need) {
+        int32_t us[kPullU];
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i) {
+          const int32_t j = k + i * P + p;
+          us[i] = (j < hc.je) ? in_col[j] : -1;
+        }
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i)
+          if (us[i] >= 0) g |= front[(int64_t)us[i] * T + t];
+        g &= need;
+      }
 #pragma unroll
       for (int x = T; x < kWave; x <<= 1) g |= __shfl_xor(g, x, kWave);
       got |= g;
       need &= ~g;
     }
-    if (slot == 0 && got) {
-      s.seen[w] = old | got;
-      fnext[w] = got;
-      planes_or(&s.planes[w * 8], code, got);
-    }
-    const bool vany = __ballot(slot == 0 && got != 0) != 0;
-    const bool lead = (l == 0) && vany;
-    const int32_t dv = lead ? out_off[v + 1] - out_off[v] : 0;
-    wave_append(lead, v, dv, nxt);
+    if (p == 0 && got) atomicOr((unsigned long long*)&s.nextw[w], (unsigned long long)got);
+    const bool any = __ballot(p == 0 && got != 0) != 0;
+    bool first = false;
+    if (l == 0 && any && s.tflag[hc.v] == 0u) first = (atomicExch(&s.tflag[hc.v], 1u) == 0u);
+    wave_push_list(first, hc.v, s.tlist, s.tcount);
   }
 }
 
@@ -388,6 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
 // lowest lane offering a bit (exclusive prefix-OR) owns that bit's parent.
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word, int32_t K,
+                                                          const int32_t* __restrict__ src,
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col,
                                                           const int32_t* __restrict__ heavy, int32_t n_heavy,
@@ -404,8 +591,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
       reached |= pk[k];
       if (k > 0) hi |= pk[k];
     }
+    uint64_t hi2 = 0;
+#pragma unroll
+    for (int k = 2; k < 8; ++k) hi2 |= pk[k];
     const uint64_t lvl0 = pk[0] & ~hi;
-    uint64_t need = reached & ~lvl0;
+    // Level 1 (code 2): the only in-neighbour at level 0 is the source itself;
+    // the light pass writes those parents directly.
+    const uint64_t lvl1 = pk[1] & ~pk[0] & ~hi2;
+    uint64_t need = reached & ~lvl0 & ~lvl1;
     uint64_t borrow = need;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -440,9 +633,11 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
 
 // Light vertices in id order: 4 lanes per vertex, each owning 16 sources of
 // the word, 16 vertices per wave, so each store instruction writes four full
-// 64-byte row segments of the level and parent tables.
+// 64-byte row segments of the level and parent tables.  Each lane issues
+// kEmitU neighbour loads per step and applies them in ascending order.
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t K, int32_t rows,
+                                                          const int32_t* __restrict__ src,
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col, int32_t n,
                                                           int32_t heavy_cut, const int32_t* __restrict__ hidx,
@@ -467,8 +662,12 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
       reached |= pk[k];
       if (k > 0) hi |= pk[k];
     }
+    uint32_t hi2 = 0;
+#pragma unroll
+    for (int k = 2; k < 8; ++k) hi2 |= pk[k];
     const uint32_t lvl0 = pk[0] & ~hi;
-    uint32_t need = reached & ~lvl0;
+    const uint32_t lvl1 = pk[1] & ~pk[0] & ~hi2;  // code 2: the parent is the source itself
+    uint32_t need = reached & ~lvl0 & ~lvl1;
     uint32_t borrow = need;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -486,17 +685,31 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
         for (int i = 0; i < 16; ++i) par[i] = sp[i];
       }
     } else {
-      for (int32_t j = b; j < e && need; ++j) {
-        const int32_t u = in_col[j];
-        const uint64_t* pu = &s.planes[((int64_t)u * T + word) * 8];
-        uint32_t eq = need;
+      for (int32_t j0 = b; j0 < e && need; j0 += kEmitU) {
+        int32_t us[kEmitU];
+        uint32_t eqs[kEmitU];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (k < K) eq &= ~((uint32_t)((pu[k] >> sh) & 0xFFFFull) ^ tk[k]);
-        if (eq) {
+        for (int i = 0; i < kEmitU; ++i) us[i] = (j0 + i < e) ? in_col[j0 + i] : -1;
 #pragma unroll
-          for (int i = 0; i < 16; ++i) par[i] = ((eq >> i) & 1u) ? u : par[i];
-          need &= ~eq;
+        for (int i = 0; i < kEmitU; ++i) {
+          uint32_t eq = 0;
+          if (us[i] >= 0) {
+            const uint64_t* pu = &s.planes[((int64_t)us[i] * T + word) * 8];
+            eq = 0xFFFFu;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+              if (k < K) eq &= ~((uint32_t)((pu[k] >> sh) & 0xFFFFull) ^ tk[k]);
+          }
+          eqs[i] = eq;
+        }
+#pragma unroll
+        for (int i = 0; i < kEmitU; ++i) {
+          const uint32_t eq = eqs[i] & need;
+          if (eq) {
+#pragma unroll
+            for (int x = 0; x < 16; ++x) par[x] = ((eq >> x) & 1u) ? us[i] : par[x];
+            need &= ~eq;
+          }
         }
       }
     }
@@ -509,21 +722,21 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
       for (int k = 0; k < 8; ++k) c |= ((pk[k] >> i) & 1u) << k;
       const int64_t o = (int64_t)r * n + v;
       if (lvl_out) lvl_out[o] = (int32_t)c - 1;
-      par_out[o] = (c == 0) ? -1 : (c == 1 ? v : par[i]);
+      par_out[o] = (c == 0) ? -1 : (c == 1 ? v : (c == 2 ? src[r] : par[i]));
     }
   }
 }
 
 // Sum over (source, vertex) with level >= 0 of the vertex's out-degree.
+// blockIdx.y strides over sources, the x dimension over vertices.
 __global__ __launch_bounds__(kBlock) void k_count_traversed(const int32_t* __restrict__ off,
-                                                            const int32_t* __restrict__ levels, int64_t total,
+                                                            const int32_t* __restrict__ levels, int32_t S,
                                                             int32_t n, unsigned long long* out) {
   unsigned long long acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    if (levels[i] >= 0) {
-      const int32_t v = (int32_t)(i % n);
-      acc += (unsigned long long)(off[v + 1] - off[v]);
-    }
+  for (int32_t s = blockIdx.y; s < S; s += gridDim.y) {
+    const int32_t* row = levels + (int64_t)s * n;
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+      if (row[v] >= 0) acc += (unsigned long long)(off[v + 1] - off[v]);
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kWave);
@@ -536,24 +749,42 @@ __global__ __launch_bounds__(kBlock) void k_count_traversed(const int32_t* __res
 // Launch wrappers
 // ============================================================================
 
-void launch_plain_init(hipStream_t st, int32_t* level, int32_t* parent, int32_t n, int32_t src) {
-  hipLaunchKernelGGL(k_plain_init, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, level, parent, n, src);
+void launch_plain_init(hipStream_t st, int32_t* level, int32_t* parent, int32_t n) {
+  hipLaunchKernelGGL(k_plain_init, dim3(grid_for(((int64_t)n + 3) / 4, kBlock)), dim3(kBlock), 0, st, level,
+                     parent, n);
+}
+
+void launch_plain_set_source(hipStream_t st, int32_t* level, int32_t* parent, int32_t src, Queue q) {
+  hipLaunchKernelGGL(k_plain_set_source, dim3(1), dim3(64), 0, st, level, parent, src, q);
 }
 
 void launch_plain_push(hipStream_t st, const int32_t* off, const int32_t* col, Queue cur, int32_t qn, int32_t E,
                        int32_t* level, int32_t* parent, int32_t d, Queue nxt) {
   if (E <= 0) return;
-  hipLaunchKernelGGL(k_plain_push, dim3(grid_for(E, kBlock)), dim3(kBlock), 0, st, off, col, cur, qn, E, level,
+  hipLaunchKernelGGL(k_plain_push, dim3(grid_for(E, kChunk)), dim3(kBlock), 0, st, off, col, cur, qn, E, level,
                      parent, d, nxt);
 }
 
-#define XB_DISPATCH_T(T, CALL)      \
-  switch (T) {                      \
-    case 1: { constexpr int TT = 1; CALL; } break; \
-    case 2: { constexpr int TT = 2; CALL; } break; \
-    case 4: { constexpr int TT = 4; CALL; } break; \
-    case 8: { constexpr int TT = 8; CALL; } break; \
-    default: break;                 \
+#define XB_DISPATCH_T(T, CALL)                   \
+  switch (T) {                                   \
+    case 1: {                                    \
+      constexpr int TT = 1;                      \
+      CALL;                                      \
+    } break;                                     \
+    case 2: {                                    \
+      constexpr int TT = 2;                      \
+      CALL;                                      \
+    } break;                                     \
+    case 4: {                                    \
+      constexpr int TT = 4;                      \
+      CALL;                                      \
+    } break;                                     \
+    case 8: {                                    \
+      constexpr int TT = 8;                      \
+      CALL;                                      \
+    } break;                                     \
+    default:                                     \
+      break;                                     \
   }
 
 void launch_bp_seed(hipStream_t st, int T, BPState s, const int32_t* src, int32_t cnt) {
@@ -564,7 +795,7 @@ void launch_bp_seed(hipStream_t st, int T, BPState s, const int32_t* src, int32_
 void launch_bp_push(hipStream_t st, int T, BPState s, int cur, const int32_t* off, const int32_t* col, Queue q,
                     int32_t qn, int32_t E) {
   if (E <= 0) return;
-  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_push<TT>, dim3(grid_for(E, kBlock)), dim3(kBlock), 0, st, s, cur, off,
+  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_push<TT>, dim3(grid_for(E, kChunk)), dim3(kBlock), 0, st, s, cur, off,
                                       col, q, qn, E));
 }
 
@@ -574,19 +805,37 @@ void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, c
                                       s, cur, (uint32_t)code, out_off, nxt));
 }
 
-void launch_bp_pull(hipStream_t st, int T, BPState s, int cur, int32_t code, int32_t cnt, const int32_t* in_off,
-                    const int32_t* in_col, const int32_t* out_off, int32_t n, int32_t heavy_cut,
-                    const int32_t* heavy, int32_t n_heavy, Queue nxt) {
-  const int vpw = kWave / T;
-  const int64_t waves = ((int64_t)n + vpw - 1) / vpw;
-  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_pull_light<TT>, dim3(grid_for(waves * kWave, kBlock)), dim3(kBlock),
-                                      0, st, s, cur, (uint32_t)code, cnt, in_off, in_col, out_off, n, heavy_cut,
-                                      nxt));
-  if (n_heavy > 0) {
-    XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_pull_heavy<TT>, dim3(grid_for((int64_t)n_heavy * kWave, kBlock)),
-                                        dim3(kBlock), 0, st, s, cur, (uint32_t)code, cnt, in_off, in_col, out_off,
-                                        heavy, n_heavy, nxt));
+template <int T, int P>
+static void pull_bucket_launch(hipStream_t st, BPState s, int cur, int32_t code, int32_t cnt, const int32_t* in_off,
+                               const int32_t* in_col, const int32_t* out_off, const int32_t* perm, int32_t begin,
+                               int32_t end, Queue nxt) {
+  constexpr int VPB = (kBlock / kWave) * (kWave / (T * P));
+  hipLaunchKernelGGL((k_bp_pull_bucket<T, P>), dim3(grid_for((int64_t)(end - begin), VPB)), dim3(kBlock), 0, st, s, cur,
+                     (uint32_t)code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
+}
+
+void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
+                           const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
+                           const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
+  if (end <= begin) return;
+  if (cls == 0) {
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, 1>(st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end,
+                                                nxt)));
+  } else if (cls == 1) {
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, (TT * 4 <= kWave ? 4 : kWave / TT)>(st, s, cur, code, cnt, in_off,
+                                                                                 in_col, out_off, perm, begin,
+                                                                                 end, nxt)));
+  } else {
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, kWave / TT>(st, s, cur, code, cnt, in_off, in_col, out_off, perm,
+                                                         begin, end, nxt)));
   }
+}
+
+void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
+                        const HubChunk* chunks, int32_t n_chunks) {
+  if (n_chunks <= 0) return;
+  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_pull_hub<TT>, dim3(grid_for((int64_t)n_chunks * kWave, kBlock)),
+                                      dim3(kBlock), 0, st, s, cur, cnt, in_col, chunks, n_chunks));
 }
 
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn) {
@@ -595,24 +844,28 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
                                       st, s, cur, q.v, qn));
 }
 
-void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* in_off,
-                    const int32_t* in_col, int32_t n, int32_t heavy_cut, const int32_t* heavy,
-                    const int32_t* hidx, int32_t n_heavy, int32_t* side, int32_t* lvl_out, int32_t* par_out) {
-  if (n_heavy > 0) {
-    XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_emit_heavy<TT>, dim3(grid_for((int64_t)n_heavy * kWave, kBlock)),
-                                        dim3(kBlock), 0, st, s, word, K, in_off, in_col, heavy, n_heavy, side));
-  }
+void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* src,
+                          const int32_t* in_off, const int32_t* in_col, const int32_t* heavy, int32_t n_heavy,
+                          int32_t* side) {
+  if (n_heavy <= 0) return;
+  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_emit_heavy<TT>, dim3(grid_for((int64_t)n_heavy * kWave, kBlock)),
+                                      dim3(kBlock), 0, st, s, word, K, src, in_off, in_col, heavy, n_heavy, side));
+}
+
+void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
+                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, const int32_t* hidx,
+                    const int32_t* side, int32_t* lvl_out, int32_t* par_out) {
   const int64_t waves = ((int64_t)n + 15) / 16;
   XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_emit_light<TT>, dim3(grid_for(waves * kWave, kBlock)), dim3(kBlock), 0,
-                                      st, s, word, K, rows, in_off, in_col, n, heavy_cut, hidx, side, lvl_out,
+                                      st, s, word, K, rows, src, in_off, in_col, n, heavy_cut, hidx, side, lvl_out,
                                       par_out));
 }
 
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,
                             unsigned long long* out) {
-  const int64_t total = (int64_t)S * n;
-  hipLaunchKernelGGL(k_count_traversed, dim3(grid_for(total, kBlock)), dim3(kBlock), 0, st, off, levels, total, n,
-                     out);
+  const int gx = grid_for(n, kBlock) < 64 ? grid_for(n, kBlock) : 64;
+  const int gy = S < 256 ? S : 256;
+  hipLaunchKernelGGL(k_count_traversed, dim3(gx, gy), dim3(kBlock), 0, st, off, levels, S, n, out);
 }
 
 }  // namespace xb

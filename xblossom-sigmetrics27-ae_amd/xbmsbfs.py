@@ -79,32 +79,49 @@ class Opts(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("words_per_tile", ctypes.c_int32),
         ("alpha", ctypes.c_int32),
-        ("pull_heavy_cut", ctypes.c_int32),
+        ("pull_cut_small", ctypes.c_int32),
+        ("pull_cut_medium", ctypes.c_int32),
+        ("pull_cut_hub", ctypes.c_int32),
+        ("hub_chunk", ctypes.c_int32),
         ("emit_heavy_cut", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("reserved", ctypes.c_int32 * 7),
     ]
+
+
+# Kernel classes of xb_run_stats (XB_K_* in include/xb/xb_capi.h).
+KERNEL_CLASSES = ("memset", "seed", "push", "commit", "pull", "pull_hub", "clear", "emit_heavy", "emit",
+                  "plain_init", "plain_push")
+K_CLASSES = 16
 
 
 class RunStats(ctypes.Structure):
     _fields_ = [
         ("ms_total", ctypes.c_double),
-        ("ms_init", ctypes.c_double),
-        ("ms_push", ctypes.c_double),
-        ("ms_pull", ctypes.c_double),
-        ("ms_emit", ctypes.c_double),
-        ("launches_push", ctypes.c_int64),
-        ("launches_pull", ctypes.c_int64),
-        ("launches_emit", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double * K_CLASSES),
+        ("kernel_bytes", ctypes.c_double * K_CLASSES),
+        ("kernel_launches", ctypes.c_int64 * K_CLASSES),
         ("max_level", ctypes.c_int32),
         ("tiles", ctypes.c_int32),
         ("push_levels", ctypes.c_int32),
         ("pull_levels", ctypes.c_int32),
         ("fallback_sources", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("reserved", ctypes.c_int32 * 11),
     ]
 
+    def kernels(self) -> dict:
+        """Per kernel class: summed HIP-event ms, launches and algorithmic bytes."""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            if self.kernel_launches[i]:
+                out[name] = {"ms": self.kernel_ms[i], "launches": int(self.kernel_launches[i]),
+                             "bytes": self.kernel_bytes[i]}
+        return out
+
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: getattr(self, k) for k, _ in self._fields_
+             if k not in ("reserved", "kernel_ms", "kernel_bytes", "kernel_launches")}
+        d["kernels"] = self.kernels()
+        return d
 
 
 _lib = None
