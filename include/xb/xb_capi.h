@@ -150,6 +150,16 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
 int32_t xb_count_traversed(xb_graph* g, const int32_t* levels_dev, int32_t S,
                            int64_t* edges_out);
 
+/* Test hook (mirrors apps/xb/test/device/scan_bs_test.cu:58-137, which tests
+ * the reference's exclusive scan + FindRightmostLTEQ mapping): appends
+ * (i, degrees[i]) for i < k through the device frontier compaction, then maps
+ * every tasklet t < sum(degrees) to (owner[t], offset[t]) with the device
+ * owner search the push kernels use.  Outputs are host arrays of
+ * sum(degrees) entries; tasklet order follows the (unordered) device queue,
+ * so the pairs, not their positions, are the result. */
+int32_t xb_test_edge_map(const int32_t* degrees, int32_t k, int32_t* owner_out,
+                         int32_t* offset_out, int64_t* total_out);
+
 /* Device memory helpers for callers without their own allocator. */
 int32_t xb_device_alloc(size_t bytes, void** ptr);
 int32_t xb_device_free(void* ptr);

@@ -1,0 +1,105 @@
+"""CPU-side checks of the C ABI library (no GPU calls that compute).
+
+* lib/libxbmsbfs.so loads and exports every function include/xb/xb_capi.h declares;
+* CSR validation errors are reported with status codes before any device work;
+* without a visible GPU, a valid graph is refused with XB_ERR_NO_DEVICE (the
+  product never falls back to a CPU path);
+* the text loader rejects malformed files with XB_ERR_IO.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import xbmsbfs as xb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", "xb", h) for h in os.listdir(os.path.join(ROOT, "include", "xb"))
+           if h.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        txt = open(h).read()
+        names |= set(re.findall(r"^\s*(?:int32_t|void|uint64_t|int64_t)\s+(xb_\w+)\s*\(", txt, re.M))
+    return sorted(names)
+
+
+def test_header_declarations_match_binding():
+    assert set(declared_functions()) == set(xb.EXPORTED_SYMBOLS)
+
+
+@pytest.mark.parametrize("name", declared_functions())
+def test_library_exports(name):
+    lib = xb.load_engine()
+    assert hasattr(lib, name), f"{name} not exported"
+
+
+def test_opts_default():
+    o = xb.default_opts()
+    assert o.words_per_tile in (1, 2, 4, 8) and o.alpha > 0
+    assert o.pull_cut_small <= o.pull_cut_medium <= o.pull_cut_hub
+
+
+def _create(ro, ci, directed=False):
+    lib = xb.load_engine()
+    h = ctypes.c_void_p()
+    ro = np.ascontiguousarray(ro, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    rc = lib.xb_graph_create_csr(xb._ptr(ro), ro.size, xb._ptr(ci), ci.size, 1 if directed else 0, ctypes.byref(h))
+    buf = ctypes.create_string_buffer(512)
+    lib.xb_last_error(buf, 512)
+    return rc, buf.value.decode()
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.mark.parametrize("ro,ci,msg", [
+    ([0, 2, 1], [1, 0], "monotone"),
+    ([1, 2, 3], [1, 0, 0], "row_offsets[0]"),
+    ([0, 1, 2], [1, 0, 0], "number of column"),
+    ([0, 1, 2], [1, 5], "out of range"),
+    ([0, 1, 2], [1, -1], "out of range"),
+])
+def test_bad_csr_rejected(ro, ci, msg):
+    rc, err = _create(ro, ci)
+    assert rc == xb.XB_ERR_BAD_CSR and msg in err
+
+
+def test_null_and_empty_arguments():
+    lib = xb.load_engine()
+    h = ctypes.c_void_p()
+    assert lib.xb_graph_create_csr(None, 0, None, 0, 0, ctypes.byref(h)) == xb.XB_ERR_INVALID_ARG
+    ro = np.array([0, 0], dtype=np.int32)
+    assert lib.xb_graph_create_csr(xb._ptr(ro), 2, None, 0, 7, ctypes.byref(h)) == xb.XB_ERR_INVALID_ARG
+    assert lib.xb_msbfs_run(None, None, 1, None, None, None, None) == xb.XB_ERR_INVALID_ARG
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the no-device refusal")
+def test_valid_graph_without_gpu_is_refused():
+    rc, err = _create([0, 1, 2], [1, 0])
+    assert rc == xb.XB_ERR_NO_DEVICE, err
+
+
+def test_text_loader_errors(tmp_path):
+    lib = xb.load_engine()
+    h = ctypes.c_void_p()
+    missing = str(tmp_path / "nope.txt").encode()
+    assert lib.xb_graph_load_text(missing, missing, 0, ctypes.byref(h)) == xb.XB_ERR_IO
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 1 x 2\n")
+    good = tmp_path / "col.txt"
+    good.write_text("1 0\n")
+    assert lib.xb_graph_load_text(str(bad).encode(), str(good).encode(), 0, ctypes.byref(h)) == xb.XB_ERR_IO
+    ro = tmp_path / "ro.txt"
+    ro.write_text("0 1\n3\n")  # row_offsets[n] != number of column indices
+    assert lib.xb_graph_load_text(str(ro).encode(), str(good).encode(), 0, ctypes.byref(h)) == xb.XB_ERR_BAD_CSR

@@ -117,3 +117,113 @@ def test_bucket_mix(T):
     assert (deg > 200).sum() > 0 and ((deg > 4) & (deg <= 20)).sum() > 0
     _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=4, pull_cut_medium=20,
            pull_cut_hub=200, hub_chunk=64, emit_heavy_cut=30, alpha=100)
+
+
+# ---- pinned cases: networkx fixtures, closed forms, reference known answers ----
+import graphs  # noqa: E402
+
+
+@pytest.mark.parametrize("case", list(graphs.golden_cases()), ids=lambda c: c[0])
+def test_golden_fixtures(case):
+    name, ro, ci, src, lv_ref, pa_ref, directed = case
+    hg = xb.HGraph(ro, ci, directed=directed)
+    eng = xb.XBlossomGpuEngine10(hg)
+    for kw in ({"mode": xb.XB_MODE_PLAIN}, {"mode": xb.XB_MODE_BATCHED, "words_per_tile": 1},
+               {"mode": xb.XB_MODE_BATCHED, "words_per_tile": 8, "alpha": 1_000_000}):
+        lv, pa = eng.MultiSourceBFS(src, **kw)
+        assert np.array_equal(lv, lv_ref), (name, kw)
+        assert np.array_equal(pa, pa_ref), (name, kw)
+
+
+@pytest.mark.parametrize("case", list(graphs.closed_form_cases()), ids=lambda c: c[0])
+def test_closed_forms(case):
+    name, ro, ci, src, lv_ref, pa_ref = case
+    hg = xb.HGraph(ro, ci)
+    eng = xb.XBlossomGpuEngine10(hg)
+    for mode in (xb.XB_MODE_PLAIN, xb.XB_MODE_BATCHED):
+        lv, pa = eng.MultiSourceBFS(src, mode=mode)
+        assert np.array_equal(lv, lv_ref) and np.array_equal(pa, pa_ref), (name, mode)
+
+
+def test_path_table_known_answers():
+    # apps/xb/test/device/path_table_test.cu:80-124 and :130-156: the root
+    # distances of ExpandTwoNodes trees are BFS hop distances from the root.
+    def tree(n, triples):
+        adj = [[] for _ in range(n)]
+        for v, w, x in triples:
+            adj[v] += [w]
+            adj[w] += [v, x]
+            adj[x] += [w]
+        return graphs.csr_from_adj(adj)
+
+    ro, ci = tree(8, [(0, 1, 2), (0, 3, 4), (4, 5, 6)])
+    lv, pa = xb.XBlossomGpuEngine10(xb.HGraph(ro, ci)).MultiSourceBFS([0], mode=xb.XB_MODE_BATCHED)
+    assert [int(lv[0, v]) for v in (0, 2, 4, 6)] == [0, 2, 2, 4] and pa[0, 6] == 5 and lv[0, 7] == -1
+    ro, ci = tree(19, [(0, 1, 2), (2, 3, 4), (2, 5, 6), (4, 7, 8), (8, 9, 10), (8, 11, 12), (6, 13, 14),
+                       (6, 15, 16), (16, 17, 18)])
+    lv, _ = xb.XBlossomGpuEngine10(xb.HGraph(ro, ci)).MultiSourceBFS([0], mode=xb.XB_MODE_BATCHED)
+    exp = {0: 0, 2: 2, 4: 4, 6: 4, 8: 6, 10: 8, 12: 8, 18: 8}
+    assert {v: int(lv[0, v]) for v in exp} == exp
+
+
+def _pairs(owner, offset):
+    return sorted(zip(owner.tolist(), offset.tolist()))
+
+
+def test_device_edge_map_known_answer():
+    # apps/xb/test/device/scan_bs_test.cu:58-137: degrees {5,4,3,2,1} -> 15 tasklets.
+    deg = np.array([5, 4, 3, 2, 1], dtype=np.int32)
+    owner, offset = xb.test_edge_map(deg)
+    ro, oo = oracle.edge_map(deg)
+    assert owner.size == 15 and _pairs(owner, offset) == _pairs(ro, oo)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_device_edge_map_large_and_zero_runs(seed):
+    rng = np.random.default_rng(seed)
+    deg = rng.integers(0, 40, size=50_000).astype(np.int32)
+    deg[rng.random(deg.size) < 0.7] = 0  # long runs of empty segments force the global-search path
+    deg[123] = 5000
+    owner, offset = xb.test_edge_map(deg)
+    ro, oo = oracle.edge_map(deg)
+    assert _pairs(owner, offset) == _pairs(ro, oo)
+
+
+def test_runs_are_bit_identical(rmat12):
+    """Deterministic lowest-id rule: repeated runs give identical tables (race check)."""
+    g, hg = rmat12
+    src = xb.sample_sources(g, 300, seed=9)
+    eng = xb.XBlossomGpuEngine10(hg)
+    a = eng.MultiSourceBFS(src, mode=xb.XB_MODE_BATCHED, words_per_tile=4)
+    b = eng.MultiSourceBFS(src, mode=xb.XB_MODE_BATCHED, words_per_tile=4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_source_out_of_range_rejected(rmat12):
+    g, hg = rmat12
+    eng = xb.XBlossomGpuEngine10(hg)
+    with pytest.raises(xb.XbError) as ei:
+        eng.MultiSourceBFS([0, g.n], mode=xb.XB_MODE_BATCHED)
+    assert ei.value.code == xb.XB_ERR_SOURCE_RANGE
+    with pytest.raises(xb.XbError) as ei:
+        eng.MultiSourceBFS([-1], mode=xb.XB_MODE_PLAIN)
+    assert ei.value.code == xb.XB_ERR_SOURCE_RANGE
+
+
+def test_text_loader_roundtrip(tmp_path, rmat12):
+    g, _ = rmat12
+    (tmp_path / "ro.txt").write_text(" ".join(map(str, g.row_offsets.tolist())) + "\n")
+    (tmp_path / "ci.txt").write_text("\n".join(map(str, g.col_indices.tolist())) + "\n")
+    hg = xb.HGraph(str(tmp_path / "ro.txt"), str(tmp_path / "ci.txt"))
+    assert (hg.nnodes, hg.nedges) == (g.n, g.m)
+    _check(g, hg, xb.sample_sources(g, 70, seed=1), mode=xb.XB_MODE_BATCHED)
+
+
+def test_unsorted_adjacency_is_sorted_on_ingest(rmat12):
+    g, _ = rmat12
+    ci = g.col_indices.copy()
+    for v in range(0, g.n, 7):
+        a, b = g.row_offsets[v], g.row_offsets[v + 1]
+        ci[a:b] = ci[a:b][::-1]
+    hg = xb.HGraph(g.row_offsets, ci)
+    _check(g, hg, xb.sample_sources(g, 100, seed=2), mode=xb.XB_MODE_BATCHED)

@@ -917,6 +917,45 @@ int32_t xb_count_traversed(xb_graph* g, const int32_t* levels_dev, int32_t S, in
   });
 }
 
+int32_t xb_test_edge_map(const int32_t* degrees, int32_t k, int32_t* owner_out, int32_t* offset_out,
+                         int64_t* total_out) {
+  return guarded([&] {
+    if (!degrees || k < 0 || !total_out) fail(XB_ERR_INVALID_ARG, "bad argument");
+    int64_t total = 0;
+    for (int32_t i = 0; i < k; ++i) {
+      if (degrees[i] < 0) fail(XB_ERR_INVALID_ARG, "negative degree");
+      total += degrees[i];
+    }
+    if (total > INT32_MAX) fail(XB_ERR_INVALID_ARG, "total tasklets exceed int32");
+    *total_out = total;
+    if (k == 0 || total == 0) return;
+    if (!owner_out || !offset_out) fail(XB_ERR_INVALID_ARG, "null outputs");
+    ensure_device();
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int32_t* d_deg = dalloc<int32_t>((size_t)k);
+    int32_t* d_v = dalloc<int32_t>((size_t)k);
+    int32_t* d_eo = dalloc<int32_t>((size_t)k);
+    unsigned long long* d_ctr = dalloc<unsigned long long>(1);
+    int32_t* d_owner = dalloc<int32_t>((size_t)total);
+    int32_t* d_off = dalloc<int32_t>((size_t)total);
+    HIPCHK(hipMemcpy(d_deg, degrees, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(d_ctr, 0, sizeof(unsigned long long)));
+    xb::Queue q{d_v, d_eo, d_ctr};
+    xb::launch_test_build_queue(st, d_deg, k, q);
+    xb::launch_test_edge_map(st, q, k, (int32_t)total, d_owner, d_off);
+    HIPCHK(hipStreamSynchronize(st));
+    unsigned long long ctr = 0;
+    HIPCHK(hipMemcpy(&ctr, d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(owner_out, d_owner, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(offset_out, d_off, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
+    for (void* p : {(void*)d_deg, (void*)d_v, (void*)d_eo, (void*)d_ctr, (void*)d_owner, (void*)d_off}) dfree(p);
+    (void)hipStreamDestroy(st);
+    if ((int64_t)(ctr & 0xFFFFFFFFull) != k || (int64_t)(ctr >> 32) != total)
+      fail(XB_ERR_DEVICE, "queue counter does not match the input");
+  });
+}
+
 int32_t xb_device_alloc(size_t bytes, void** ptr) {
   return guarded([&] {
     if (!ptr) fail(XB_ERR_INVALID_ARG, "null ptr");

@@ -67,4 +67,8 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, i
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,
                             unsigned long long* out);
 
+// ---- test hooks ----
+void launch_test_build_queue(hipStream_t st, const int32_t* deg, int32_t k, Queue q);
+void launch_test_edge_map(hipStream_t st, Queue q, int32_t qn, int32_t E, int32_t* owner, int32_t* offset);
+
 }  // namespace xb

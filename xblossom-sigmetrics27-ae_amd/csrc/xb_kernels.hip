@@ -743,6 +743,42 @@ __global__ __launch_bounds__(kBlock) void k_count_traversed(const int32_t* __res
   if (lane_id() == 0) atomicAdd(out, acc);
 }
 
+// ---- test hooks: the compaction + edge-mapping device code on its own ----
+
+// Appends (i, deg[i]) for i < k through the block-staged compaction.
+__global__ __launch_bounds__(kBlock) void k_test_build_queue(const int32_t* __restrict__ deg, int32_t k, Queue q) {
+  __shared__ AppendStage sg;
+  stage_init(sg);
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < k; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = b0 + threadIdx.x;
+    const bool p = i < k;
+    stage_put(sg, p, p ? (int32_t)i : 0, p ? deg[i] : 0);
+    stage_sync_flush(sg, false, q);
+  }
+  stage_sync_flush(sg, true, q);
+}
+
+// Maps every edge tasklet e < E to (queue vertex, offset) with the chunked
+// owner staging the push kernels use.
+__global__ __launch_bounds__(kBlock) void k_test_edge_map(Queue q, int32_t qn, int32_t E, int32_t* __restrict__ owner,
+                                                          int32_t* __restrict__ offset) {
+  __shared__ int32_t s_eo[kChunk + 1];
+  __shared__ int s_lohi[2];
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+    const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, s_eo, s_lohi);
+    for (int r = 0; r < kPushR; ++r) {
+      const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
+      if (e < c1) {
+        const int i = chunk_owner(co, q.eo, s_eo, e);
+        owner[e] = q.v[i];
+        offset[e] = e - q.eo[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // ============================================================================
@@ -866,6 +902,15 @@ void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* l
   const int gx = grid_for(n, kBlock) < 64 ? grid_for(n, kBlock) : 64;
   const int gy = S < 256 ? S : 256;
   hipLaunchKernelGGL(k_count_traversed, dim3(gx, gy), dim3(kBlock), 0, st, off, levels, S, n, out);
+}
+
+void launch_test_build_queue(hipStream_t st, const int32_t* deg, int32_t k, Queue q) {
+  hipLaunchKernelGGL(k_test_build_queue, dim3(grid_for(k, kBlock)), dim3(kBlock), 0, st, deg, k, q);
+}
+
+void launch_test_edge_map(hipStream_t st, Queue q, int32_t qn, int32_t E, int32_t* owner, int32_t* offset) {
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_test_edge_map, dim3(grid_for(E, kChunk)), dim3(kBlock), 0, st, q, qn, E, owner, offset);
 }
 
 }  // namespace xb

@@ -58,6 +58,7 @@ EXPORTED_SYMBOLS = (
     "xb_graph_destroy",
     "xb_msbfs_run",
     "xb_count_traversed",
+    "xb_test_edge_map",
     "xb_device_alloc",
     "xb_device_free",
     "xb_memcpy_d2h",
@@ -155,6 +156,7 @@ def load_engine():
     lib.xb_graph_destroy.argtypes = [VP]
     lib.xb_msbfs_run.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts), ctypes.POINTER(RunStats)]
     lib.xb_count_traversed.argtypes = [VP, _P32, I32, ctypes.POINTER(I64)]
+    lib.xb_test_edge_map.argtypes = [_P32, I32, _P32, _P32, ctypes.POINTER(I64)]
     lib.xb_device_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(VP)]
     lib.xb_device_free.argtypes = [VP]
     lib.xb_memcpy_d2h.argtypes = [VP, VP, ctypes.c_size_t]
@@ -256,6 +258,18 @@ class XBlossomGpuEngine10:
         out = ctypes.c_int64()
         _check(lib.xb_count_traversed(self.graph.handle, _ptr(levels_dev), S, ctypes.byref(out)))
         return out.value
+
+
+def test_edge_map(degrees):
+    """Device compaction + edge mapping on a degree list: (owner, offset) per tasklet."""
+    lib = load_engine()
+    d = np.ascontiguousarray(degrees, dtype=np.int32)
+    total = int(d.astype(np.int64).sum())
+    owner = np.empty(max(total, 1), dtype=np.int32)
+    offset = np.empty(max(total, 1), dtype=np.int32)
+    t = ctypes.c_int64()
+    _check(lib.xb_test_edge_map(_ptr(d), d.size, _ptr(owner), _ptr(offset), ctypes.byref(t)))
+    return owner[:total], offset[:total]
 
 
 # ---------------------------------------------------------------------------
