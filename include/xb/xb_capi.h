@@ -66,7 +66,9 @@ typedef struct xb_graph xb_graph;
  * degree buckets (by in-degree): deg <= pull_cut_small gets T lanes per
  * vertex (one per source word), deg <= pull_cut_medium 4T lanes, deg <=
  * pull_cut_hub a whole wave, larger ones are split into hub_chunk-edge pieces
- * with one wave each. */
+ * with one wave each.  Vertices above pull_cut_medium are "heavy": the pull
+ * records their parents as it finds them, and the emit pass resolves only
+ * what push levels discovered. */
 typedef struct xb_msbfs_opts {
   int32_t mode;            /* XB_MODE_*                                        */
   uint32_t flags;          /* XB_OUT_DEVICE | XB_PROFILE | ...                 */
@@ -76,8 +78,7 @@ typedef struct xb_msbfs_opts {
   int32_t pull_cut_medium;
   int32_t pull_cut_hub;
   int32_t hub_chunk;
-  int32_t emit_heavy_cut;  /* in-degree above which emit uses a wave/vertex    */
-  int32_t reserved[7];
+  int32_t reserved[8];
 } xb_msbfs_opts;
 
 /* Kernel classes timed by XB_PROFILE (index into the xb_run_stats arrays). */

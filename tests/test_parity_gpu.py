@@ -52,7 +52,7 @@ def test_batched_rmat(rmat12, T, alpha):
 def test_batched_heavy_kernels(rmat12, T):
     g, hg = rmat12
     src = xb.sample_sources(g, 200, seed=4)
-    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=1, pull_cut_medium=3, pull_cut_hub=6, hub_chunk=5, emit_heavy_cut=2)
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=1, pull_cut_medium=3, pull_cut_hub=6, hub_chunk=5)
 
 
 def test_plain_rmat(rmat12):
@@ -78,7 +78,7 @@ def test_directed_rmat():
     src = xb.sample_sources(g, 130, seed=1)
     for mode in (xb.XB_MODE_BATCHED, xb.XB_MODE_PLAIN):
         _check(g, hg, src, mode=mode)
-    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=2, pull_cut_small=2, pull_cut_medium=2, pull_cut_hub=4, hub_chunk=3, emit_heavy_cut=2,
+    _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=2, pull_cut_small=2, pull_cut_medium=2, pull_cut_hub=4, hub_chunk=3,
            alpha=1_000_000)
 
 
@@ -116,7 +116,7 @@ def test_bucket_mix(T):
     deg = g.degrees()
     assert (deg > 200).sum() > 0 and ((deg > 4) & (deg <= 20)).sum() > 0
     _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, pull_cut_small=4, pull_cut_medium=20,
-           pull_cut_hub=200, hub_chunk=64, emit_heavy_cut=30, alpha=100)
+           pull_cut_hub=200, hub_chunk=64, alpha=100)
 
 
 # ---- pinned cases: networkx fixtures, closed forms, reference known answers ----

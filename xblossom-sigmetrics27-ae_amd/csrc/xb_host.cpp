@@ -164,7 +164,7 @@ struct xb_graph {
   hipStream_t stream = nullptr;
   Pinned pinned;
   // In-degree layout for the pull and emit, rebuilt when a cut changes.
-  int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1, emit_cut = -1;
+  int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1;
   int32_t* d_perm = nullptr;          // in-degree > 0, bucket-major, ascending id per bucket
   int32_t bucket_end[3] = {0, 0, 0};  // perm ranges of the small / medium / large buckets
   int64_t bucket_edges[3] = {0, 0, 0};
@@ -172,9 +172,9 @@ struct xb_graph {
   int32_t n_hub_chunks = 0;
   int32_t n_hubs = 0;
   int64_t hub_edges = 0;
-  int32_t* d_heavy_emit = nullptr;
-  int32_t* d_hidx_emit = nullptr;
-  int32_t n_heavy_emit = 0;
+  int32_t* d_heavy = nullptr;  // in-degree > cut_medium, ascending id (the emit heavy list)
+  int32_t* d_hidx = nullptr;   // vertex -> index in d_heavy, else -1
+  int32_t n_heavy = 0;
   Workspace ws;
 
   const std::vector<int32_t>& in_off() const { return h_in_off.empty() ? h_off : h_in_off; }
@@ -188,8 +188,8 @@ struct xb_graph {
     dfree(d_col);
     dfree(d_perm);
     dfree(d_hub_chunks);
-    dfree(d_heavy_emit);
-    dfree(d_hidx_emit);
+    dfree(d_heavy);
+    dfree(d_hidx);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -338,7 +338,6 @@ struct BPOpts {
   int cut_medium;
   int cut_hub;
   int hub_chunk;
-  int emit_cut;
 };
 
 // ---------------------------------------------------------------------------
@@ -389,29 +388,27 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
     g->n_hub_chunks = (int32_t)chunks.size();
     g->n_hubs = hubs;
     g->hub_edges = hub_edges;
+    // Heavy vertices (bucket 2 and hubs): their parents are recorded by the pull.
+    std::vector<int32_t> hv, hidx((size_t)n, -1);
+    for (int32_t v = 0; v < n; ++v)
+      if (io[v + 1] - io[v] > o.cut_medium) {
+        hidx[v] = (int32_t)hv.size();
+        hv.push_back(v);
+      }
+    dfree(g->d_heavy);
+    dfree(g->d_hidx);
+    g->d_heavy = nullptr;
+    g->d_hidx = nullptr;
+    g->d_heavy = dalloc<int32_t>(hv.size());
+    g->d_hidx = dalloc<int32_t>((size_t)n);
+    if (!hv.empty())
+      HIPCHK(hipMemcpy(g->d_heavy, hv.data(), sizeof(int32_t) * hv.size(), hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(g->d_hidx, hidx.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    g->n_heavy = (int32_t)hv.size();
     g->cut_small = o.cut_small;
     g->cut_medium = o.cut_medium;
     g->cut_hub = o.cut_hub;
     g->hub_chunk = o.hub_chunk;
-  }
-  if (o.emit_cut != g->emit_cut) {
-    std::vector<int32_t> h, idx((size_t)n, -1);
-    for (int32_t v = 0; v < n; ++v)
-      if (io[v + 1] - io[v] > o.emit_cut) {
-        idx[v] = (int32_t)h.size();
-        h.push_back(v);
-      }
-    dfree(g->d_heavy_emit);
-    dfree(g->d_hidx_emit);
-    g->d_heavy_emit = nullptr;
-    g->d_hidx_emit = nullptr;
-    g->d_heavy_emit = dalloc<int32_t>(h.size());
-    g->d_hidx_emit = dalloc<int32_t>((size_t)n);
-    if (!h.empty())
-      HIPCHK(hipMemcpy(g->d_heavy_emit, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
-    if (n) HIPCHK(hipMemcpy(g->d_hidx_emit, idx.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
-    g->n_heavy_emit = (int32_t)h.size();
-    g->emit_cut = o.emit_cut;
   }
 }
 
@@ -458,14 +455,18 @@ void ensure_src(xb_graph* g, int64_t cnt) {
   w.src_cap = cnt;
 }
 
-void ensure_side(xb_graph* g) {
+// Heavy-parent table: n_heavy x T x 64 entries.
+void ensure_hpar(xb_graph* g, int T) {
   Workspace& w = g->ws;
-  const int64_t need = (int64_t)std::max(g->n_heavy_emit, 1) * 64;
-  if (w.side_cap >= need) return;
-  dfree(w.side);
-  w.side = nullptr;
-  w.side = dalloc<int32_t>((size_t)need);
-  w.side_cap = need;
+  const int64_t need = (int64_t)std::max(g->n_heavy, 1) * T * 64;
+  if (w.side_cap < need) {
+    dfree(w.side);
+    w.side = nullptr;
+    w.side = dalloc<int32_t>((size_t)need);
+    w.side_cap = need;
+  }
+  w.bp.hpar = (uint32_t*)w.side;
+  w.bp.hidx = g->d_hidx;
 }
 
 int32_t* ensure_stage(xb_graph* g, int64_t ints) {
@@ -637,8 +638,10 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   }
   HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, g->stream));
   HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, g->stream));
+  if (g->n_heavy > 0)
+    HIPCHK(hipMemsetAsync(s.hpar, 0xFF, sizeof(uint32_t) * (size_t)g->n_heavy * T * 64, g->stream));
   prof.end();
-  prof.add(XB_K_MEMSET, 72.0 * nw);
+  prof.add(XB_K_MEMSET, 72.0 * nw + 256.0 * T * g->n_heavy);
   // First frontier: the distinct source vertices with their degree prefix.
   std::vector<int32_t> uniq(src, src + cnt);
   std::sort(uniq.begin(), uniq.end());
@@ -726,18 +729,16 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   // Emit: K planes cover codes up to d+1.
   int K = 0;
   while ((1 << K) <= d + 1) ++K;
-  ensure_side(g);
   const int words = (cnt + 63) / 64;
   for (int t = 0; t < words; ++t) {
-    if (g->n_heavy_emit > 0) {
+    if (g->n_heavy > 0) {
       prof.begin(XB_K_EMIT_HEAVY);
-      xb::launch_bp_emit_heavy(g->stream, T, s, t, K, w.d_src, g->d_in_off, g->d_in_col, g->d_heavy_emit,
-                               g->n_heavy_emit, w.side);
+      xb::launch_bp_emit_heavy(g->stream, T, s, t, K, g->d_in_off, g->d_in_col, g->d_heavy, g->n_heavy);
       prof.end();
     }
     prof.begin(XB_K_EMIT);
-    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.emit_cut,
-                       g->d_hidx_emit, w.side, lvl_rows, par_rows);
+    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.cut_medium, lvl_rows,
+                       par_rows);
     prof.end();
     const int rows_w = std::min(64, cnt - 64 * t);
     // level + parent int32 per (source, vertex) of this word
@@ -757,6 +758,7 @@ void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S
   ensure_queues(g);
   ensure_bp(g, o.T);
   ensure_layout(g, o);
+  ensure_hpar(g, o.T);
   const int32_t n = g->n;
   const int32_t tile = 64 * o.T;
   ensure_src(g, tile);
@@ -805,7 +807,6 @@ int32_t xb_opts_default(xb_msbfs_opts* o) {
   o->pull_cut_medium = 64;
   o->pull_cut_hub = 1024;
   o->hub_chunk = 1024;
-  o->emit_heavy_cut = 64;
   return XB_OK;
 }
 
@@ -862,14 +863,13 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
       if (opts->pull_cut_medium) o.pull_cut_medium = opts->pull_cut_medium;
       if (opts->pull_cut_hub) o.pull_cut_hub = opts->pull_cut_hub;
       if (opts->hub_chunk) o.hub_chunk = opts->hub_chunk;
-      if (opts->emit_heavy_cut) o.emit_heavy_cut = opts->emit_heavy_cut;
     }
     if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
     if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
     const int T = o.words_per_tile;
     if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
     if (o.alpha < 1 || o.pull_cut_small < 1 || o.pull_cut_medium < o.pull_cut_small ||
-        o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1 || o.emit_heavy_cut < 1)
+        o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1)
       fail(XB_ERR_INVALID_ARG, "bad tunable");
     for (int32_t i = 0; i < S; ++i)
       if (sources[i] < 0 || sources[i] >= g->n)
@@ -887,7 +887,7 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     if (o.mode == XB_MODE_PLAIN) {
       run_plain(g, sources, S, levels, parents, dev_out, prof, stats);
     } else {
-      BPOpts bo{T, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk, o.emit_heavy_cut};
+      BPOpts bo{T, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk};
       run_batched(g, bo, sources, S, levels, parents, dev_out, prof, stats);
     }
     HIPCHK(hipEventRecord(e1, g->stream));

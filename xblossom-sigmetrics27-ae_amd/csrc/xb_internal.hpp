@@ -18,8 +18,10 @@ struct BPState {
   uint64_t* nextw;
   uint64_t* planes;
   uint32_t* tflag;
-  int32_t* tlist;    // vertices whose next words were OR-ed atomically
-  uint32_t* tcount;  // device count of tlist
+  int32_t* tlist;       // vertices whose next words were OR-ed atomically
+  uint32_t* tcount;     // device count of tlist
+  uint32_t* hpar;       // heavy-vertex parents [(h*T + t)*64 + bit], 0xFFFFFFFF = unresolved
+  const int32_t* hidx;  // vertex -> heavy index (in-degree above the medium pull cut), else -1
 };
 
 // Frontier queue: vertex ids plus the exclusive prefix of their out-degrees
@@ -55,13 +57,13 @@ void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, i
 void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
+// Heavy vertices: resolve the parents the pull did not record (push levels).
+void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
+                          const int32_t* in_col, const int32_t* heavy, int32_t n_heavy);
 // src: the tile's source ids (level-1 parents are the sources themselves).
-void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* src,
-                          const int32_t* in_off, const int32_t* in_col, const int32_t* heavy, int32_t n_heavy,
-                          int32_t* side);
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
-                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
-                    const int32_t* hidx, const int32_t* side, int32_t* lvl_out, int32_t* par_out);
+                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
+                    int32_t* par_out);
 
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

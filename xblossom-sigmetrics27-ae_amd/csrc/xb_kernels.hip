@@ -440,8 +440,11 @@ __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32
 // end).  G = T*P lanes per vertex: lane (slot, p, t) handles source word t and
 // in-neighbours j = b + k + i*P + p (i < kPullU) of step k; the P partitions
 // are OR-reduced after every step so every lane of the vertex sees the same
-// remaining need and the group stops together.
-template <int T, int P>
+// remaining need and the group stops together.  With REC (heavy vertices,
+// one wave each) the neighbours of a step are taken in ascending position and
+// the first provider of every new bit -- the lowest-id parent -- is recorded
+// in the heavy-parent table s.hpar.
+template <int T, int P, bool REC>
 __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
                                                            const int32_t* __restrict__ in_off,
                                                            const int32_t* __restrict__ in_col,
@@ -466,34 +469,60 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, u
     const bool act = idx < end;
     int32_t v = 0, b = 0, e = 0;
     uint64_t old = 0, need = 0;
+    uint32_t* hp = nullptr;
     if (act) {
       v = perm[idx];
       b = in_off[v];
       e = in_off[v + 1];
       old = s.seen[(int64_t)v * T + t];
       need = ~old & vmask;
+      if (REC) hp = s.hpar + ((int64_t)s.hidx[v] * T + t) * 64;
     }
     uint64_t got = 0;
     for (int32_t k = 0;; k += P * kPullU) {
       const bool more = need != 0 && (b + k + p < e);
       if (!__any(more)) break;
-      uint64_t g = 0;
-      if (more) {
-        int32_t us[kPullU];
+      int32_t us[kPullU];
+      uint64_t fs[kPullU];
 #pragma unroll
-        for (int i = 0; i < kPullU; ++i) {
-          const int32_t j = b + k + i * P + p;
-          us[i] = (j < e) ? in_col[j] : -1;
-        }
-#pragma unroll
-        for (int i = 0; i < kPullU; ++i)
-          if (us[i] >= 0) g |= front[(int64_t)us[i] * T + t];
-        g &= need;
+      for (int i = 0; i < kPullU; ++i) {
+        const int32_t j = b + k + i * P + p;
+        us[i] = (more && j < e) ? in_col[j] : -1;
       }
 #pragma unroll
-      for (int x = T; x < G; x <<= 1) g |= __shfl_xor(g, x, kWave);
-      got |= g;
-      need &= ~g;
+      for (int i = 0; i < kPullU; ++i) fs[i] = (us[i] >= 0) ? front[(int64_t)us[i] * T + t] : 0ull;
+      if (!REC) {
+        uint64_t g = 0;
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i) g |= fs[i];
+        g &= need;
+#pragma unroll
+        for (int x = T; x < G; x <<= 1) g |= __shfl_xor(g, x, kWave);
+        got |= g;
+        need &= ~g;
+      } else {
+#pragma unroll
+        for (int i = 0; i < kPullU; ++i) {
+          const uint64_t c = fs[i] & need;
+          uint64_t incl = c;
+#pragma unroll
+          for (int d = 1; d < P; d <<= 1) {
+            const uint64_t y = __shfl_up(incl, d * T, kWave);
+            if (p >= d) incl |= y;
+          }
+          uint64_t excl = __shfl_up(incl, T, kWave);
+          if (p == 0) excl = 0;
+          uint64_t first = c & ~excl;
+          while (first) {
+            const int bit = __builtin_ctzll(first);
+            first &= first - 1;
+            hp[bit] = (uint32_t)us[i];
+          }
+          const uint64_t all = __shfl(incl, slot * G + (P - 1) * T + t, kWave);
+          got |= all;
+          need &= ~all;
+        }
+      }
     }
     if (act && p == 0 && got) {
       const int64_t w = (int64_t)v * T + t;
@@ -512,7 +541,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, u
 }
 
 // Bottom-up pull over hub adjacency pieces: one wave per piece, results
-// OR-ed atomically into the next words and committed by k_bp_commit.
+// OR-ed atomically into the next words and committed by k_bp_commit.  Each
+// piece records its first provider of every new bit with atomicMin into the
+// heavy-parent table, so the lowest id over all pieces wins.
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int32_t cnt,
                                                         const int32_t* __restrict__ in_col,
@@ -526,27 +557,41 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int3
   for (int64_t c = global_wave(); c < n_chunks; c += num_waves()) {
     const HubChunk hc = chunks[c];
     const int64_t w = (int64_t)hc.v * T + t;
+    uint32_t* hp = s.hpar + ((int64_t)s.hidx[hc.v] * T + t) * 64;
     uint64_t need = ~s.seen[w] & vmask;
     uint64_t got = 0;
     for (int32_t k = hc.jb; k < hc.je; k += P * kPullU) {
       if (!__any(need != 0)) break;
-      uint64_t g = 0;
-      if (need) {
-        int32_t us[kPullU];
+      int32_t us[kPullU];
+      uint64_t fs[kPullU];
 #pragma unroll
-        for (int i = 0; i < kPullU; ++i) {
-          const int32_t j = k + i * P + p;
-          us[i] = (j < hc.je) ? in_col[j] : -1;
-        }
-#pragma unroll
-        for (int i = 0; i < kPullU; ++i)
-          if (us[i] >= 0) g |= front[(int64_t)us[i] * T + t];
-        g &= need;
+      for (int i = 0; i < kPullU; ++i) {
+        const int32_t j = k + i * P + p;
+        us[i] = (need && j < hc.je) ? in_col[j] : -1;
       }
 #pragma unroll
-      for (int x = T; x < kWave; x <<= 1) g |= __shfl_xor(g, x, kWave);
-      got |= g;
-      need &= ~g;
+      for (int i = 0; i < kPullU; ++i) fs[i] = (us[i] >= 0) ? front[(int64_t)us[i] * T + t] : 0ull;
+#pragma unroll
+      for (int i = 0; i < kPullU; ++i) {
+        const uint64_t cbits = fs[i] & need;
+        uint64_t incl = cbits;
+#pragma unroll
+        for (int d = 1; d < P; d <<= 1) {
+          const uint64_t y = __shfl_up(incl, d * T, kWave);
+          if (p >= d) incl |= y;
+        }
+        uint64_t excl = __shfl_up(incl, T, kWave);
+        if (p == 0) excl = 0;
+        uint64_t first = cbits & ~excl;
+        while (first) {
+          const int bit = __builtin_ctzll(first);
+          first &= first - 1;
+          atomicMin(&hp[bit], (uint32_t)us[i]);
+        }
+        const uint64_t all = __shfl(incl, (P - 1) * T + t, kWave);
+        got |= all;
+        need &= ~all;
+      }
     }
     if (p == 0 && got) atomicOr((unsigned long long*)&s.nextw[w], (unsigned long long)got);
     const bool any = __ballot(p == 0 && got != 0) != 0;
@@ -570,41 +615,55 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
 
 // ---- emit: level/parent tables for one source word ----
 
-// Heavy vertices: one wave per vertex; lane i tests in-neighbour j0+i, and the
-// lowest lane offering a bit (exclusive prefix-OR) owns that bit's parent.
-template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word, int32_t K,
-                                                          const int32_t* __restrict__ src,
+// Level-code masks of 64 sources from K bit-sliced planes.
+template <int K>
+struct CodeMasks {
+  uint64_t reached, lvl0, lvl1;
+  uint64_t tk[K];  // planes of (code - 1), valid for bits with code >= 2
+};
+
+template <int K>
+__device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
+  CodeMasks<K> m;
+  uint64_t hi = 0, hi2 = 0;
+  m.reached = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m.reached |= pk[k];
+    if (k >= 1) hi |= pk[k];
+    if (k >= 2) hi2 |= pk[k];
+  }
+  m.lvl0 = pk[0] & ~hi;
+  m.lvl1 = (K >= 2) ? (pk[K >= 2 ? 1 : 0] & ~pk[0] & ~hi2) : 0ull;
+  uint64_t borrow = m.reached & ~m.lvl0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m.tk[k] = pk[k] ^ borrow;
+    borrow &= ~pk[k];
+  }
+  return m;
+}
+
+// Heavy vertices (in-degree above the medium pull cut): the pull recorded the
+// parents of every bit it discovered; bits discovered by push levels are
+// resolved here, one wave per vertex, lane i testing in-neighbour j0+i, the
+// lowest lane offering a bit (exclusive prefix-OR) owning that bit's parent.
+template <int T, int K>
+__global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word,
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ heavy, int32_t n_heavy,
-                                                          int32_t* __restrict__ side) {
+                                                          const int32_t* __restrict__ heavy, int32_t n_heavy) {
   const int l = lane_id();
   for (int64_t h = global_wave(); h < n_heavy; h += num_waves()) {
     const int32_t v = heavy[h];
     const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
-    uint64_t pk[8], tk[8];
-    uint64_t reached = 0, hi = 0;
+    uint64_t pk[K];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      pk[k] = (k < K) ? pv[k] : 0ull;
-      reached |= pk[k];
-      if (k > 0) hi |= pk[k];
-    }
-    uint64_t hi2 = 0;
-#pragma unroll
-    for (int k = 2; k < 8; ++k) hi2 |= pk[k];
-    const uint64_t lvl0 = pk[0] & ~hi;
-    // Level 1 (code 2): the only in-neighbour at level 0 is the source itself;
-    // the light pass writes those parents directly.
-    const uint64_t lvl1 = pk[1] & ~pk[0] & ~hi2;
-    uint64_t need = reached & ~lvl0 & ~lvl1;
-    uint64_t borrow = need;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      tk[k] = pk[k] ^ borrow;
-      borrow &= ~pk[k];
-    }
+    for (int k = 0; k < K; ++k) pk[k] = pv[k];
+    const CodeMasks<K> cm = code_masks<K>(pk);
+    uint32_t* hp = s.hpar + ((int64_t)h * T + word) * 64;
+    const uint64_t unset = __ballot(hp[l] == 0xFFFFFFFFu);
+    uint64_t need = cm.reached & ~cm.lvl0 & ~cm.lvl1 & unset;
     const int32_t b = in_off[v], e = in_off[v + 1];
     for (int32_t j0 = b; j0 < e && need; j0 += kWave) {
       const int32_t j = j0 + l;
@@ -615,15 +674,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
         const uint64_t* pu = &s.planes[((int64_t)u * T + word) * 8];
         eq = need;
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (k < K) eq &= ~(pu[k] ^ tk[k]);
+        for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
       }
       const uint64_t ex = wave_excl_or(eq);
       uint64_t first = eq & ~ex;
       while (first) {
         const int bit = __builtin_ctzll(first);
         first &= first - 1;
-        side[h * 64 + bit] = u;
+        hp[bit] = (uint32_t)u;
       }
       const uint64_t all = __shfl(ex | eq, kWave - 1, kWave);
       need &= ~all;
@@ -631,99 +689,110 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
   }
 }
 
-// Light vertices in id order: 4 lanes per vertex, each owning 16 sources of
-// the word, 16 vertices per wave, so each store instruction writes four full
-// 64-byte row segments of the level and parent tables.  Each lane issues
-// kEmitU neighbour loads per step and applies them in ascending order.
-template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t K, int32_t rows,
+// Light vertices in id order, 16 per wave, 4 lanes per vertex.  Each lane
+// takes every 4th in-neighbour (kEmitU per step) and tests all 64 sources of
+// the word at once; a segmented prefix-OR over the vertex's 4 lanes, in
+// ascending neighbour position, gives the first (lowest-id) provider of every
+// bit, staged in an LDS tile.  Lane p then writes rows 16p..16p+15 for its
+// vertex, so a store instruction covers four full 64-byte row segments.
+template <int T, int K>
+__global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
                                                           const int32_t* __restrict__ src,
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col, int32_t n,
-                                                          int32_t heavy_cut, const int32_t* __restrict__ hidx,
-                                                          const int32_t* __restrict__ side,
-                                                          int32_t* __restrict__ lvl_out,
+                                                          int32_t heavy_cut, int32_t* __restrict__ lvl_out,
                                                           int32_t* __restrict__ par_out) {
+  __shared__ int32_t lds_par[kBlock / kWave][64][17];
   const int l = lane_id();
   const int g = l >> 2;
-  const int q = l & 3;
-  const int sh = 16 * q;
-  const int row0 = 64 * word + sh;
+  const int p = l & 3;
+  int32_t(*lp)[17] = lds_par[threadIdx.x / kWave];
   const int64_t stride = num_waves() * 16;
   for (int64_t vb = global_wave() * 16; vb < n; vb += stride) {
     const int32_t v = (int32_t)(vb + g);
-    if (v >= n) continue;
-    uint32_t pk[8], tk[8];
-    uint32_t reached = 0, hi = 0;
-    const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
+    const bool act = v < n;
+    uint64_t pk[K];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      pk[k] = (k < K) ? (uint32_t)((pv[k] >> sh) & 0xFFFFull) : 0u;
-      reached |= pk[k];
-      if (k > 0) hi |= pk[k];
+    for (int k = 0; k < K; ++k) pk[k] = 0;
+    int32_t b = 0, e = 0;
+    if (act) {
+      const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
+#pragma unroll
+      for (int k = 0; k < K; ++k) pk[k] = pv[k];
+      b = in_off[v];
+      e = in_off[v + 1];
     }
-    uint32_t hi2 = 0;
+    const CodeMasks<K> cm = code_masks<K>(pk);
+    const bool heavy = act && (e - b > heavy_cut);
+    uint64_t need = heavy ? 0ull : (cm.reached & ~cm.lvl0 & ~cm.lvl1);
+    for (int32_t k0 = 0;; k0 += 4 * kEmitU) {
+      const bool more = need != 0 && (b + k0 + p < e);
+      if (!__any(more)) break;
+      int32_t us[kEmitU];
+      uint64_t eqs[kEmitU];
 #pragma unroll
-    for (int k = 2; k < 8; ++k) hi2 |= pk[k];
-    const uint32_t lvl0 = pk[0] & ~hi;
-    const uint32_t lvl1 = pk[1] & ~pk[0] & ~hi2;  // code 2: the parent is the source itself
-    uint32_t need = reached & ~lvl0 & ~lvl1;
-    uint32_t borrow = need;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      tk[k] = pk[k] ^ borrow;
-      borrow &= ~pk[k];
-    }
-    int32_t par[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) par[i] = -1;
-    const int32_t b = in_off[v], e = in_off[v + 1];
-    if (e - b > heavy_cut) {
-      if (need) {
-        const int32_t* sp = side + (int64_t)hidx[v] * 64 + sh;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) par[i] = sp[i];
+      for (int i = 0; i < kEmitU; ++i) {
+        const int32_t j = b + k0 + i * 4 + p;
+        us[i] = (more && j < e) ? in_col[j] : -1;
       }
-    } else {
-      for (int32_t j0 = b; j0 < e && need; j0 += kEmitU) {
-        int32_t us[kEmitU];
-        uint32_t eqs[kEmitU];
 #pragma unroll
-        for (int i = 0; i < kEmitU; ++i) us[i] = (j0 + i < e) ? in_col[j0 + i] : -1;
+      for (int i = 0; i < kEmitU; ++i) {
+        uint64_t eq = 0;
+        if (us[i] >= 0) {
+          const uint64_t* pu = &s.planes[((int64_t)us[i] * T + word) * 8];
+          eq = ~0ull;
 #pragma unroll
-        for (int i = 0; i < kEmitU; ++i) {
-          uint32_t eq = 0;
-          if (us[i] >= 0) {
-            const uint64_t* pu = &s.planes[((int64_t)us[i] * T + word) * 8];
-            eq = 0xFFFFu;
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-              if (k < K) eq &= ~((uint32_t)((pu[k] >> sh) & 0xFFFFull) ^ tk[k]);
-          }
-          eqs[i] = eq;
+          for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
         }
+        eqs[i] = eq;
+      }
 #pragma unroll
-        for (int i = 0; i < kEmitU; ++i) {
-          const uint32_t eq = eqs[i] & need;
-          if (eq) {
-#pragma unroll
-            for (int x = 0; x < 16; ++x) par[x] = ((eq >> x) & 1u) ? us[i] : par[x];
-            need &= ~eq;
-          }
+      for (int i = 0; i < kEmitU; ++i) {
+        const uint64_t c = eqs[i] & need;
+        uint64_t incl = c;
+        uint64_t y = __shfl_up(incl, 1, kWave);
+        if (p >= 1) incl |= y;
+        y = __shfl_up(incl, 2, kWave);
+        if (p >= 2) incl |= y;
+        uint64_t excl = __shfl_up(incl, 1, kWave);
+        if (p == 0) excl = 0;
+        uint64_t first = c & ~excl;
+        while (first) {
+          const int bit = __builtin_ctzll(first);
+          first &= first - 1;
+          lp[bit][g] = us[i];
         }
+        need &= ~__shfl(incl, (l & ~3) | 3, kWave);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (act) {
+      const uint32_t* hp = heavy ? s.hpar + ((int64_t)s.hidx[v] * T + word) * 64 : nullptr;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = row0 + i;
-      if (r >= rows) break;
-      uint32_t c = 0;
+      for (int i = 0; i < 16; ++i) {
+        const int bit = 16 * p + i;
+        const int r = 64 * word + bit;
+        if (r >= rows) break;
+        uint32_t c = 0;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) c |= ((pk[k] >> i) & 1u) << k;
-      const int64_t o = (int64_t)r * n + v;
-      if (lvl_out) lvl_out[o] = (int32_t)c - 1;
-      par_out[o] = (c == 0) ? -1 : (c == 1 ? v : (c == 2 ? src[r] : par[i]));
+        for (int k = 0; k < K; ++k) c |= (uint32_t)((pk[k] >> bit) & 1ull) << k;
+        int32_t par;
+        if (c == 0)
+          par = -1;
+        else if (c == 1)
+          par = v;
+        else if (c == 2)
+          par = src[r];
+        else
+          par = heavy ? (int32_t)hp[bit] : lp[bit][g];
+        const int64_t o = (int64_t)r * n + v;
+        if (lvl_out) lvl_out[o] = (int32_t)c - 1;
+        par_out[o] = par;
+      }
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -841,13 +910,13 @@ void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, c
                                       s, cur, (uint32_t)code, out_off, nxt));
 }
 
-template <int T, int P>
+template <int T, int P, bool REC>
 static void pull_bucket_launch(hipStream_t st, BPState s, int cur, int32_t code, int32_t cnt, const int32_t* in_off,
                                const int32_t* in_col, const int32_t* out_off, const int32_t* perm, int32_t begin,
                                int32_t end, Queue nxt) {
   constexpr int VPB = (kBlock / kWave) * (kWave / (T * P));
-  hipLaunchKernelGGL((k_bp_pull_bucket<T, P>), dim3(grid_for((int64_t)(end - begin), VPB)), dim3(kBlock), 0, st, s, cur,
-                     (uint32_t)code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
+  hipLaunchKernelGGL((k_bp_pull_bucket<T, P, REC>), dim3(grid_for((int64_t)(end - begin), VPB)), dim3(kBlock), 0,
+                     st, s, cur, (uint32_t)code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
 }
 
 void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
@@ -855,15 +924,14 @@ void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, i
                            const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
   if (end <= begin) return;
   if (cls == 0) {
-    XB_DISPATCH_T(T, (pull_bucket_launch<TT, 1>(st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end,
-                                                nxt)));
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, 1, false>(st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin,
+                                                       end, nxt)));
   } else if (cls == 1) {
-    XB_DISPATCH_T(T, (pull_bucket_launch<TT, (TT * 4 <= kWave ? 4 : kWave / TT)>(st, s, cur, code, cnt, in_off,
-                                                                                 in_col, out_off, perm, begin,
-                                                                                 end, nxt)));
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, (TT * 4 <= kWave ? 4 : kWave / TT), false>(
+                         st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end, nxt)));
   } else {
-    XB_DISPATCH_T(T, (pull_bucket_launch<TT, kWave / TT>(st, s, cur, code, cnt, in_off, in_col, out_off, perm,
-                                                         begin, end, nxt)));
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, kWave / TT, true>(st, s, cur, code, cnt, in_off, in_col, out_off, perm,
+                                                               begin, end, nxt)));
   }
 }
 
@@ -880,21 +948,69 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
                                       st, s, cur, q.v, qn));
 }
 
-void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* src,
-                          const int32_t* in_off, const int32_t* in_col, const int32_t* heavy, int32_t n_heavy,
-                          int32_t* side) {
+#define XB_DISPATCH_K(K, CALL)   \
+  switch (K) {                   \
+    case 1: {                    \
+      constexpr int KK = 1;      \
+      CALL;                      \
+    } break;                     \
+    case 2: {                    \
+      constexpr int KK = 2;      \
+      CALL;                      \
+    } break;                     \
+    case 3: {                    \
+      constexpr int KK = 3;      \
+      CALL;                      \
+    } break;                     \
+    case 4: {                    \
+      constexpr int KK = 4;      \
+      CALL;                      \
+    } break;                     \
+    case 5: {                    \
+      constexpr int KK = 5;      \
+      CALL;                      \
+    } break;                     \
+    case 6: {                    \
+      constexpr int KK = 6;      \
+      CALL;                      \
+    } break;                     \
+    case 7: {                    \
+      constexpr int KK = 7;      \
+      CALL;                      \
+    } break;                     \
+    default: {                   \
+      constexpr int KK = 8;      \
+      CALL;                      \
+    } break;                     \
+  }
+
+template <int T, int K>
+static void emit_heavy_launch(hipStream_t st, BPState s, int32_t word, const int32_t* in_off, const int32_t* in_col,
+                              const int32_t* heavy, int32_t n_heavy) {
+  hipLaunchKernelGGL((k_bp_emit_heavy<T, K>), dim3(grid_for((int64_t)n_heavy * kWave, kBlock)), dim3(kBlock), 0, st,
+                     s, word, in_off, in_col, heavy, n_heavy);
+}
+
+template <int T, int K>
+static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
+                              const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
+                              int32_t* lvl_out, int32_t* par_out) {
+  const int64_t waves = ((int64_t)n + 15) / 16;
+  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(waves * kWave, kBlock)), dim3(kBlock), 0, st, s, word,
+                     rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+}
+
+void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
+                          const int32_t* in_col, const int32_t* heavy, int32_t n_heavy) {
   if (n_heavy <= 0) return;
-  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_emit_heavy<TT>, dim3(grid_for((int64_t)n_heavy * kWave, kBlock)),
-                                      dim3(kBlock), 0, st, s, word, K, src, in_off, in_col, heavy, n_heavy, side));
+  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_heavy_launch<TT, KK>(st, s, word, in_off, in_col, heavy, n_heavy))));
 }
 
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
-                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, const int32_t* hidx,
-                    const int32_t* side, int32_t* lvl_out, int32_t* par_out) {
-  const int64_t waves = ((int64_t)n + 15) / 16;
-  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_emit_light<TT>, dim3(grid_for(waves * kWave, kBlock)), dim3(kBlock), 0,
-                                      st, s, word, K, rows, src, in_off, in_col, n, heavy_cut, hidx, side, lvl_out,
-                                      par_out));
+                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
+                    int32_t* par_out) {
+  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, n, heavy_cut,
+                                                               lvl_out, par_out))));
 }
 
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

@@ -84,8 +84,7 @@ class Opts(ctypes.Structure):
         ("pull_cut_medium", ctypes.c_int32),
         ("pull_cut_hub", ctypes.c_int32),
         ("hub_chunk", ctypes.c_int32),
-        ("emit_heavy_cut", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("reserved", ctypes.c_int32 * 8),
     ]
 
 
