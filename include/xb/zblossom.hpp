@@ -1,0 +1,159 @@
+// zblossom.hpp -- header-only C++ shim over the C ABI (xb_capi.h) that keeps
+// the reference's entry points, so existing C++ call sites compile unchanged
+// and gain MultiSourceBFS():
+//
+//   zblossom::HGraph(row_offsets_path, col_indices_path)
+//       apps/xb/src/include/graph/hgraph_gpu.h:37-48 (host CSR, move-only;
+//       nnodes = |rowOffsets| - 1, nedges = |colIndices| / 2 as at :44-45)
+//   zblossom::XBlossomGpuEngine10(const HGraph&, size_t path_table_buffer_ratio)
+//       apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:25 (the engine
+//       owns the device copy of the graph, as DGraph does in
+//       apps/xb/src/include/graph/dgraph.h:50-63)
+//
+// Status codes become exceptions (zblossom::XbException), replacing the
+// reference's CUDA_CHECK terminate (apps/xb/src/include/utils/utils_gpu.h:54-63).
+#pragma once
+
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "xb_capi.h"
+
+namespace zblossom {
+
+class XbException : public std::runtime_error {
+ public:
+  XbException(int32_t code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+  int32_t code() const { return code_; }
+
+ private:
+  int32_t code_;
+};
+
+inline void XbCheck(int32_t rc) {
+  if (rc != XB_OK) {
+    char buf[1024];
+    xb_last_error(buf, sizeof(buf));
+    throw XbException(rc, std::string("xb error ") + std::to_string(rc) + ": " + buf);
+  }
+}
+
+// Reads whitespace-separated int32 values, the file format of the
+// reference's ReadFileIntoHVector (apps/xb/src/include/common/utils_gpu.h:9-25),
+// but reports a missing file instead of returning an empty vector.
+inline std::vector<int32_t> ReadIntFile(const std::string& path) {
+  std::vector<int32_t> out;
+  std::FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) throw XbException(XB_ERR_IO, "cannot open " + path);
+  long long v;
+  while (std::fscanf(f, "%lld", &v) == 1) {
+    if (v < INT32_MIN || v > INT32_MAX) {
+      std::fclose(f);
+      throw XbException(XB_ERR_IO, "integer out of range in " + path);
+    }
+    out.push_back((int32_t)v);
+  }
+  std::fclose(f);
+  return out;
+}
+
+// Host CSR graph (int32 rowOffsets / colIndices).
+class HGraph {
+ public:
+  HGraph() = default;
+  HGraph(const std::string& row_offsets_path, const std::string& col_indices_path, bool directed = false)
+      : row_offsets_(ReadIntFile(row_offsets_path)), col_indices_(ReadIntFile(col_indices_path)),
+        directed_(directed) {
+    if (row_offsets_.empty()) throw XbException(XB_ERR_IO, "no row offsets in " + row_offsets_path);
+    Bind();
+  }
+  HGraph(std::vector<int32_t> row_offsets, std::vector<int32_t> col_indices, bool directed = false)
+      : row_offsets_(std::move(row_offsets)), col_indices_(std::move(col_indices)), directed_(directed) {
+    if (row_offsets_.empty()) throw XbException(XB_ERR_INVALID_ARG, "empty row offsets");
+    Bind();
+  }
+  HGraph(const HGraph&) = delete;
+  HGraph& operator=(const HGraph&) = delete;
+  HGraph(HGraph&& o) noexcept { *this = std::move(o); }
+  HGraph& operator=(HGraph&& o) noexcept {
+    row_offsets_ = std::move(o.row_offsets_);
+    col_indices_ = std::move(o.col_indices_);
+    directed_ = o.directed_;
+    Bind();
+    return *this;
+  }
+
+  size_t nnodes = 0, nedges = 0;
+  const int32_t* row_offsets = nullptr;
+  const int32_t* col_indices = nullptr;
+
+  const std::vector<int32_t>& GetRowOffsetsVector() const { return row_offsets_; }
+  const std::vector<int32_t>& GetColIndicesVector() const { return col_indices_; }
+  bool directed() const { return directed_; }
+
+ private:
+  void Bind() {
+    nnodes = row_offsets_.empty() ? 0 : row_offsets_.size() - 1;
+    nedges = directed_ ? col_indices_.size() : col_indices_.size() / 2;
+    row_offsets = row_offsets_.data();
+    col_indices = col_indices_.data();
+  }
+  std::vector<int32_t> row_offsets_, col_indices_;
+  bool directed_ = false;
+};
+
+// Engine bound to one device-resident copy of a graph.
+class XBlossomGpuEngine10 {
+ public:
+  XBlossomGpuEngine10(const HGraph& g, size_t path_table_buffer_ratio)
+      : n_(g.nnodes), path_table_buffer_ratio_(path_table_buffer_ratio) {
+    XbCheck(xb_graph_create_csr(g.row_offsets, (int64_t)g.GetRowOffsetsVector().size(), g.col_indices,
+                                (int64_t)g.GetColIndicesVector().size(),
+                                g.directed() ? XB_GRAPH_DIRECTED : XB_GRAPH_SYMMETRIC, &graph_));
+    XbCheck(xb_opts_default(&opts_));
+  }
+  ~XBlossomGpuEngine10() {
+    if (graph_) xb_graph_destroy(graph_);
+  }
+  XBlossomGpuEngine10(const XBlossomGpuEngine10&) = delete;
+  XBlossomGpuEngine10& operator=(const XBlossomGpuEngine10&) = delete;
+
+  xb_msbfs_opts& Options() { return opts_; }
+  const xb_run_stats& LastStats() const { return stats_; }
+  size_t PathTableBufferRatio() const { return path_table_buffer_ratio_; }
+
+  // Host tables, S x n row-major (row s = sources[s]).
+  void MultiSourceBFS(const std::vector<int32_t>& sources, std::vector<int32_t>& levels,
+                      std::vector<int32_t>& parents) {
+    const size_t cells = sources.size() * n_;
+    levels.assign(cells, 0);
+    parents.assign(cells, 0);
+    xb_msbfs_opts o = opts_;
+    o.flags &= ~XB_OUT_DEVICE;
+    XbCheck(xb_msbfs_run(graph_, sources.data(), (int32_t)sources.size(), levels.data(), parents.data(), &o,
+                         &stats_));
+  }
+
+  // Device tables (caller-allocated, S x n int32 each).
+  void MultiSourceBFSDevice(const int32_t* sources, int32_t S, int32_t* levels_dev, int32_t* parents_dev) {
+    xb_msbfs_opts o = opts_;
+    o.flags |= XB_OUT_DEVICE;
+    XbCheck(xb_msbfs_run(graph_, sources, S, levels_dev, parents_dev, &o, &stats_));
+  }
+
+  xb_graph* handle() const { return graph_; }
+
+ private:
+  xb_graph* graph_ = nullptr;
+  size_t n_ = 0;
+  size_t path_table_buffer_ratio_ = 0;
+  xb_msbfs_opts opts_{};
+  xb_run_stats stats_{};
+};
+
+}  // namespace zblossom

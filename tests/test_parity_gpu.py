@@ -227,3 +227,15 @@ def test_unsorted_adjacency_is_sorted_on_ingest(rmat12):
         ci[a:b] = ci[a:b][::-1]
     hg = xb.HGraph(g.row_offsets, ci)
     _check(g, hg, xb.sample_sources(g, 100, seed=2), mode=xb.XB_MODE_BATCHED)
+
+
+def test_cpp_shim_call_site(tmp_path):
+    """The C++ shim (zblossom::HGraph / XBlossomGpuEngine10) from a C++ call site."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(xb.ENGINE_LIB), "shim_test")
+    assert os.path.exists(exe), "shim_test not built"
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "shim_test ok" in r.stdout

@@ -689,12 +689,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
   }
 }
 
-// Light vertices in id order, 16 per wave, 4 lanes per vertex.  Each lane
-// takes every 4th in-neighbour (kEmitU per step) and tests all 64 sources of
-// the word at once; a segmented prefix-OR over the vertex's 4 lanes, in
-// ascending neighbour position, gives the first (lowest-id) provider of every
-// bit, staged in an LDS tile.  Lane p then writes rows 16p..16p+15 for its
-// vertex, so a store instruction covers four full 64-byte row segments.
+// Light vertices: a block takes 64 consecutive vertices.  Compute phase: 4
+// lanes per vertex, each taking every 4th in-neighbour (kEmitU per step) and
+// testing all 64 sources of the word at once; a segmented prefix-OR over the
+// vertex's 4 lanes, in ascending neighbour position, gives the first
+// (lowest-id) provider of every bit, staged in an LDS tile [row][vertex].
+// Heavy vertices copy their recorded parents into the tile.  Write phase: each
+// wave takes 16 rows and each lane one vertex, so every store writes 256
+// contiguous bytes of one row of the level and parent tables.
 template <int T, int K>
 __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
                                                           const int32_t* __restrict__ src,
@@ -702,14 +704,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
                                                           const int32_t* __restrict__ in_col, int32_t n,
                                                           int32_t heavy_cut, int32_t* __restrict__ lvl_out,
                                                           int32_t* __restrict__ par_out) {
-  __shared__ int32_t lds_par[kBlock / kWave][64][17];
+  __shared__ int32_t tile[64][65];
+  __shared__ uint64_t pl[64][K];
   const int l = lane_id();
-  const int g = l >> 2;
+  const int wv = threadIdx.x / kWave;
+  const int vi = wv * 16 + (l >> 2);
   const int p = l & 3;
-  int32_t(*lp)[17] = lds_par[threadIdx.x / kWave];
-  const int64_t stride = num_waves() * 16;
-  for (int64_t vb = global_wave() * 16; vb < n; vb += stride) {
-    const int32_t v = (int32_t)(vb + g);
+  for (int64_t vb = (int64_t)blockIdx.x * 64; vb < n; vb += (int64_t)gridDim.x * 64) {
+    // ---- compute phase ----
+    const int32_t v = (int32_t)(vb + vi);
     const bool act = v < n;
     uint64_t pk[K];
 #pragma unroll
@@ -722,9 +725,18 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
       b = in_off[v];
       e = in_off[v + 1];
     }
+    if (p == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) pl[vi][k] = pk[k];
+    }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
-    uint64_t need = heavy ? 0ull : (cm.reached & ~cm.lvl0 & ~cm.lvl1);
+    uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
+    if (heavy) {
+      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[v] * T + word) * 64;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) tile[16 * p + i][vi] = (int32_t)hp[16 * p + i];
+    }
     for (int32_t k0 = 0;; k0 += 4 * kEmitU) {
       const bool more = need != 0 && (b + k0 + p < e);
       if (!__any(more)) break;
@@ -760,39 +772,33 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
         while (first) {
           const int bit = __builtin_ctzll(first);
           first &= first - 1;
-          lp[bit][g] = us[i];
+          tile[bit][vi] = us[i];
         }
         need &= ~__shfl(incl, (l & ~3) | 3, kWave);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (act) {
-      const uint32_t* hp = heavy ? s.hpar + ((int64_t)s.hidx[v] * T + word) * 64 : nullptr;
+    __syncthreads();
+    // ---- write phase ----
+    const int32_t v2 = (int32_t)(vb + l);
+    if (v2 < n) {
+      uint64_t q[K];
 #pragma unroll
+      for (int k = 0; k < K; ++k) q[k] = pl[l][k];
+#pragma unroll 4
       for (int i = 0; i < 16; ++i) {
-        const int bit = 16 * p + i;
+        const int bit = 16 * wv + i;
         const int r = 64 * word + bit;
         if (r >= rows) break;
         uint32_t c = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) c |= (uint32_t)((pk[k] >> bit) & 1ull) << k;
-        int32_t par;
-        if (c == 0)
-          par = -1;
-        else if (c == 1)
-          par = v;
-        else if (c == 2)
-          par = src[r];
-        else
-          par = heavy ? (int32_t)hp[bit] : lp[bit][g];
-        const int64_t o = (int64_t)r * n + v;
-        if (lvl_out) lvl_out[o] = (int32_t)c - 1;
-        par_out[o] = par;
+        for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
+        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : tile[bit][l]));
+        const int64_t o = (int64_t)r * n + v2;
+        if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[o]);
+        __builtin_nontemporal_store(par, &par_out[o]);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    __syncthreads();
   }
 }
 
@@ -995,9 +1001,8 @@ template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
                               int32_t* lvl_out, int32_t* par_out) {
-  const int64_t waves = ((int64_t)n + 15) / 16;
-  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(waves * kWave, kBlock)), dim3(kBlock), 0, st, s, word,
-                     rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64)), dim3(kBlock), 0, st, s, word, rows, src, in_off,
+                     in_col, n, heavy_cut, lvl_out, par_out);
 }
 
 void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
