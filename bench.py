@@ -102,12 +102,105 @@ def cpu_baseline(g, src, seconds: float, threads: int):
     }
 
 
+def bench_uneven(args) -> int:  # noqa: C901
+    """The uneven batch: 32 R-MAT graphs (scales 12-22) with 1-512 sources each.
+
+    Batched = one persistent launch for the whole batch (hipGraph replay);
+    reported beside the host-driven single-graph engine run graph by graph and
+    the plain per-source engine on every source."""
+    import numpy as np
+    import torch
+
+    import xbmsbfs as xb
+
+    t0 = time.perf_counter()
+    work = xb.workload_uneven_batch()
+    log(f"uneven batch: {len(work)} graphs, {sum(g.m for g, _ in work)} edges, "
+        f"{sum(s.size for _, s in work)} sources ({time.perf_counter() - t0:.1f}s)")
+    hgs = [xb.HGraph(g.row_offsets, g.col_indices) for g, _ in work]
+    srcs = [s for _, s in work]
+    lv = [torch.empty((s.size, g.nnodes), dtype=torch.int32, device="cuda") for s, g in zip(srcs, hgs)]
+    pa = [torch.empty((s.size, g.nnodes), dtype=torch.int32, device="cuda") for s, g in zip(srcs, hgs)]
+    tun = {"words_per_tile": args.words_per_tile} if args.words_per_tile else {}
+    batch = xb.Batch(hgs, srcs, [t.data_ptr() for t in lv], [t.data_ptr() for t in pa], **tun)
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ms_graph = []
+    for _ in range(args.steps):
+        ms_graph.append(batch.run().ms_total)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t_start) / args.steps
+    edges = sum(xb.XBlossomGpuEngine10(h).CountTraversed(l.data_ptr(), s.size) for h, l, s in zip(hgs, lv, srcs))
+    gteps = edges / t_batch / 1e9
+    log(f"persistent batch: {t_batch * 1e3:.2f} ms, {gteps:.1f} GTEPS")
+    sums = [(l.sum().item(), p.sum().item()) for l, p in zip(lv, pa)]
+    # Host-driven engine, graph by graph, into the same tables.
+    engines = [xb.XBlossomGpuEngine10(h) for h in hgs]
+    for e, s, l, p in zip(engines, srcs, lv, pa):
+        e.MultiSourceBFS(s, l.data_ptr(), p.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True, **tun)
+    torch.cuda.synchronize()
+    th0 = time.perf_counter()
+    for e, s, l, p in zip(engines, srcs, lv, pa):
+        e.MultiSourceBFS(s, l.data_ptr(), p.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True, **tun)
+    torch.cuda.synchronize()
+    t_host = time.perf_counter() - th0
+    exact = sums == [(l.sum().item(), p.sum().item()) for l, p in zip(lv, pa)]
+    log(f"host-driven per graph: {t_host * 1e3:.2f} ms, equal sums: {exact}")
+    plain = None
+    if not args.no_plain:
+        tp0 = time.perf_counter()
+        for e, s, l, p in zip(engines, srcs, lv, pa):
+            e.MultiSourceBFS(s, l.data_ptr(), p.data_ptr(), mode=xb.XB_MODE_PLAIN, device_out=True)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - tp0
+        plain = {"gteps": round(edges / t_plain / 1e9, 3), "ms_per_batch": round(t_plain * 1e3, 2)}
+        exact = exact and sums == [(l.sum().item(), p.sum().item()) for l, p in zip(lv, pa)]
+        log(f"plain: {t_plain * 1e3:.1f} ms")
+    st = batch.last_stats
+    achieved = st.kernel_bytes[11] / (st.kernel_ms[11] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "uneven batch: 32 R-MAT graphs, scales 12-22, edge factor 8-32 (seed 5), "
+                               "1-512 sources each (seed 6)",
+                   "graphs": len(work), "m_total": int(sum(g.m for g, _ in work)),
+                   "sources": int(sum(s.size for s in srcs)), "edges_traversed_per_step": int(edges),
+                   "engine": "persistent"},
+        "host_driven_per_graph": {"gteps": round(edges / t_host / 1e9, 3), "ms_per_batch": round(t_host * 1e3, 2)},
+        "plain_per_source": plain,
+        "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,
+        "bit_exact_sums_vs_host_driven_and_plain": exact,
+        "roofline": {"bound": "hbm", "kernel": "k_bp_persistent", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_launch": st.kernel_bytes[11],
+                     "avg_launch_ms": round(float(np.mean(ms_graph)), 3), "launches": args.steps},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16"])
+    ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven"])
+    ap.add_argument("--engine", default="hostloop", choices=["hostloop", "persistent"],
+                    help="hostloop: one launch per phase, host-driven level loop; persistent: the batch API "
+                         "(one persistent kernel, level loop on the device, replayed hipGraph)")
     ap.add_argument("--words-per-tile", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -128,6 +221,9 @@ def main() -> int:
 
     import xbmsbfs as xb
 
+    if args.workload == "uneven":
+        return bench_uneven(args)
+
     t0 = time.perf_counter()
     g, src, desc = make_workload(args.workload)
     S, n = int(src.size), g.n
@@ -142,7 +238,13 @@ def main() -> int:
     pa = torch.empty((S, n), dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
 
+    batch = None
+    if args.engine == "persistent":
+        batch = xb.Batch([hg], [src], [lv.data_ptr()], [pa.data_ptr()], **tun)
+
     def run_batched(profile: bool):
+        if batch is not None:
+            return batch.run()
         eng.MultiSourceBFS(src, lv.data_ptr(), pa.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True,
                            profile=profile, **tun)
         return eng.last_stats
@@ -254,6 +356,7 @@ def main() -> int:
             "edges_traversed_per_step": int(edges),
             "max_level": last["max_level"] if last else None,
             "words_per_tile": args.words_per_tile or 8,
+            "engine": args.engine,
         },
         "plain_per_source": plain,
         "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,

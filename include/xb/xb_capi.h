@@ -93,6 +93,7 @@ typedef struct xb_msbfs_opts {
 #define XB_K_EMIT 8       /* level/parent table write-out                       */
 #define XB_K_PLAIN_INIT 9 /* plain engine: row initialisation                   */
 #define XB_K_PLAIN_PUSH 10 /* plain engine: top-down push level                 */
+#define XB_K_PERSISTENT 11 /* persistent batch kernel (whole batch, one launch) */
 #define XB_K_CLASSES 16
 
 /* Per-run statistics (filled when the pointer is non-null).  kernel_ms and
@@ -145,6 +146,33 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
                      int32_t* levels, int32_t* parents,
                      const xb_msbfs_opts* opts, xb_run_stats* stats);
 
+/* ---- batched multi-graph runs: one persistent, hipGraph-captured launch ----
+ *
+ * A batch is G graphs, each with its own source set and device output
+ * tables.  xb_batch_create plans it once: every graph gets a team of
+ * workgroups sized by its estimated cost, its source tiles (64*T sources
+ * each) run on that team with the level loop on the device, and the whole
+ * batch is one persistent kernel launch captured in a hipGraph.
+ * xb_batch_run replays the graph; tiles deeper than the 8-bit level code are
+ * re-run on the plain path afterwards.  The graphs must stay alive, and their
+ * tuning (bucket cuts) unchanged, while the batch exists.  Results are those
+ * of xb_msbfs_run on each graph. */
+typedef struct xb_batch xb_batch;
+
+int32_t xb_batch_create(xb_graph* const* graphs, int32_t G,
+                        const int32_t* const* sources, const int32_t* S,
+                        int32_t* const* levels_dev, int32_t* const* parents_dev,
+                        const xb_msbfs_opts* opts, xb_batch** out);
+int32_t xb_batch_run(xb_batch* b, xb_run_stats* stats);
+int32_t xb_batch_destroy(xb_batch* b);
+
+/* Convenience: create + run + destroy.  Outputs are host arrays unless
+ * opts->flags has XB_OUT_DEVICE. */
+int32_t xb_msbfs_run_batched(xb_graph* const* graphs, int32_t G,
+                             const int32_t* const* sources, const int32_t* S,
+                             int32_t* const* levels, int32_t* const* parents,
+                             const xb_msbfs_opts* opts, xb_run_stats* stats);
+
 /* Sum over sources of the out-degrees of the vertices each source reaches
  * (the traversed-edge count of a full top-down BFS), computed on the device
  * from a levels table already in device memory. */
@@ -160,6 +188,11 @@ int32_t xb_count_traversed(xb_graph* g, const int32_t* levels_dev, int32_t S,
  * so the pairs, not their positions, are the result. */
 int32_t xb_test_edge_map(const int32_t* degrees, int32_t k, int32_t* owner_out,
                          int32_t* offset_out, int64_t* total_out);
+
+/* Order-independent checksum of an int32 device array (sum over i of a 64-bit
+ * mix of (i, a[i])): equal tables give equal sums; used to compare S x n
+ * tables without copying them to the host. */
+int32_t xb_device_checksum(const int32_t* a_dev, int64_t count, uint64_t* out);
 
 /* Device memory helpers for callers without their own allocator. */
 int32_t xb_device_alloc(size_t bytes, void** ptr);

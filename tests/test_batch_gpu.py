@@ -1,0 +1,112 @@
+"""Multi-graph batches: one persistent, hipGraph-captured launch (xb_batch_*).
+
+Every graph of a batch must come out bit-exact against the oracle (small
+graphs) or against the single-graph engine plus oracle rows (large ones), on
+the first run and on every replay of the captured graph.
+"""
+import numpy as np
+import pytest
+
+import graphs as tg
+import oracle
+import xbmsbfs as xb
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_graphs():
+    out = []
+    out.append((xb.gen_rmat(8, 8, 0.57, 0.19, 0.19, seed=1), False))
+    out.append((xb.gen_rmat(11, 16, 0.57, 0.19, 0.19, seed=2), False))
+    out.append((xb.gen_rmat(13, 8, 0.45, 0.25, 0.15, seed=3), False))
+    out.append((xb.gen_chung_lu(20_000, 2.1, 12.0, 2_000, seed=4), False))
+    out.append((xb.gen_rmat(12, 8, 0.6, 0.15, 0.15, seed=5, symmetric=False), True))
+    ro, ci = tg.path(320)  # deeper than the 8-bit level code: plain re-run of its tile
+    out.append((xb.Csr(ro, ci), False))
+    return out
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    rng = np.random.default_rng(7)
+    gs = _mixed_graphs()
+    hgs = [xb.HGraph(g.row_offsets, g.col_indices, directed=d) for g, d in gs]
+    srcs = []
+    for g, _ in gs:
+        S = int(rng.integers(1, 300))
+        s = xb.sample_sources(g, S, seed=int(rng.integers(1 << 30)))
+        srcs.append(s if s.size else np.zeros(1, dtype=np.int32))
+    srcs[-1] = np.array([0, 319, 160], dtype=np.int32)
+    refs = [oracle.msbfs(g.row_offsets, g.col_indices, s, threads=8)[:2] for (g, _), s in zip(gs, srcs)]
+    return gs, hgs, srcs, refs
+
+
+@pytest.mark.parametrize("T", [1, 8])
+def test_batch_mixed_vs_oracle(mixed, T):
+    gs, hgs, srcs, refs = mixed
+    res, st = xb.run_batched(hgs, srcs, words_per_tile=T, pull_cut_small=4, pull_cut_medium=16, pull_cut_hub=128,
+                             hub_chunk=64)
+    for i, ((lv, pa), (lr, pr)) in enumerate(zip(res, refs)):
+        assert np.array_equal(lv, lr), f"graph {i}: levels"
+        assert np.array_equal(pa, pr), f"graph {i}: parents"
+    assert st.fallback_sources == 3
+    assert st.kernel_launches[11] == 1
+
+
+def test_batch_replay_and_single_graph_agree(mixed):
+    gs, hgs, srcs, refs = mixed
+    lv = [xb.DeviceBuffer((s.size, g.nnodes), fill=7) for s, g in zip(srcs, hgs)]
+    pa = [xb.DeviceBuffer((s.size, g.nnodes), fill=7) for s, g in zip(srcs, hgs)]
+    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa], words_per_tile=4)
+    for rep in range(3):
+        for t in lv + pa:
+            t.upload(np.full(t.count, 7, dtype=np.int32))
+        st = b.run()
+        for i, (lr, pr) in enumerate(refs):
+            assert np.array_equal(lv[i].numpy(), lr), f"replay {rep} graph {i}"
+            assert np.array_equal(pa[i].numpy(), pr), f"replay {rep} graph {i}"
+        assert st.ms_total > 0
+    b.close()
+    # The single-graph engine on the same inputs.
+    for i, (h, s) in enumerate(zip(hgs, srcs)):
+        l1, p1 = xb.XBlossomGpuEngine10(h).MultiSourceBFS(s, mode=xb.XB_MODE_BATCHED)
+        assert np.array_equal(l1, refs[i][0]) and np.array_equal(p1, refs[i][1])
+
+
+def test_batch_rejects_bad_sources(mixed):
+    gs, hgs, srcs, refs = mixed
+    bad = [s.copy() for s in srcs]
+    bad[1][0] = hgs[1].nnodes
+    with pytest.raises(xb.XbError) as ei:
+        xb.run_batched(hgs, bad)
+    assert ei.value.code == xb.XB_ERR_SOURCE_RANGE
+
+
+@pytest.mark.timeout(600)
+def test_uneven_batch_full_size():
+    """32 R-MAT graphs (scales 12-22, edge factor 8-32, 1-512 sources each) in one
+    persistent launch: every graph's tables equal the single-graph engine's
+    (device checksums over all S x n entries), and sampled rows equal the oracle."""
+    work = xb.workload_uneven_batch()
+    hgs = [xb.HGraph(g.row_offsets, g.col_indices) for g, _ in work]
+    srcs = [s for _, s in work]
+    lv = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    pa = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa])
+    st = b.run()
+    assert st.kernel_launches[11] == 1 and st.tiles >= len(work)
+    for i, ((g, s), h) in enumerate(zip(work, hgs)):
+        eng = xb.XBlossomGpuEngine10(h)
+        l1 = xb.DeviceBuffer((s.size, g.n))
+        p1 = xb.DeviceBuffer((s.size, g.n))
+        eng.MultiSourceBFS(s, l1.ptr, p1.ptr, mode=xb.XB_MODE_BATCHED, device_out=True)
+        assert lv[i].checksum() == l1.checksum(), f"graph {i}: levels"
+        assert pa[i].checksum() == p1.checksum(), f"graph {i}: parents"
+        l1.free()
+        p1.free()
+        rows = sorted({0, s.size - 1, s.size // 2})
+        lr, pr, _ = oracle.msbfs(g.row_offsets, g.col_indices, s[rows], threads=16)
+        got_l = np.concatenate([lv[i].rows(r, r + 1) for r in rows])
+        got_p = np.concatenate([pa[i].rows(r, r + 1) for r in rows])
+        assert np.array_equal(got_l, lr) and np.array_equal(got_p, pr), f"graph {i}: oracle rows"
+    b.close()

@@ -56,11 +56,8 @@ def _create(ro, ci, directed=False):
 
 
 def _has_gpu():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    # Probe without initialising any HIP runtime in the test process.
+    return os.path.exists("/dev/kfd")
 
 
 @pytest.mark.parametrize("ro,ci,msg", [

@@ -29,32 +29,28 @@ def test_full_parity(which):
 
 
 def test_livejournal_shaped_sampled():
-    import torch
-
     g, src = xb.workload_livejournal_like()
     S, n = src.size, g.n
     eng = xb.XBlossomGpuEngine10(xb.HGraph(g.row_offsets, g.col_indices))
-    lv = torch.empty((S, n), dtype=torch.int32, device="cuda")
-    pa = torch.empty((S, n), dtype=torch.int32, device="cuda")
-    eng.MultiSourceBFS(src, lv.data_ptr(), pa.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True)
-    torch.cuda.synchronize()
-    edges = eng.CountTraversed(lv.data_ptr(), S)
-    # Plain engine on a slice of sources that spans two source words and both tiles.
-    sl = np.r_[0:40, 600:624]
-    lvp = torch.empty((sl.size, n), dtype=torch.int32, device="cuda")
-    pap = torch.empty((sl.size, n), dtype=torch.int32, device="cuda")
-    eng.MultiSourceBFS(src[sl], lvp.data_ptr(), pap.data_ptr(), mode=xb.XB_MODE_PLAIN, device_out=True)
-    torch.cuda.synchronize()
-    idx = torch.as_tensor(sl, device="cuda")
-    assert torch.equal(lv[idx], lvp) and torch.equal(pa[idx], pap)
+    lv = xb.DeviceBuffer((S, n))
+    pa = xb.DeviceBuffer((S, n))
+    eng.MultiSourceBFS(src, lv.ptr, pa.ptr, mode=xb.XB_MODE_BATCHED, device_out=True)
+    edges = eng.CountTraversed(lv.ptr, S)
+    # Plain engine on slices of sources in different words and both tiles.
+    for r0, r1 in ((0, 12), (60, 70), (600, 606)):
+        lvp = xb.DeviceBuffer((r1 - r0, n))
+        pap = xb.DeviceBuffer((r1 - r0, n))
+        eng.MultiSourceBFS(src[r0:r1], lvp.ptr, pap.ptr, mode=xb.XB_MODE_PLAIN, device_out=True)
+        assert np.array_equal(lv.rows(r0, r1), lvp.numpy()) and np.array_equal(pa.rows(r0, r1), pap.numpy())
+        lvp.free()
+        pap.free()
     # Oracle and invariants on sampled rows.
     rows = [0, 63, 64, 511, 512, 1023]
-    lv_h = lv[rows].cpu().numpy()
-    pa_h = pa[rows].cpu().numpy()
+    lv_h = np.concatenate([lv.rows(r, r + 1) for r in rows])
+    pa_h = np.concatenate([pa.rows(r, r + 1) for r in rows])
     lv_ref, pa_ref, e_rows = oracle.msbfs(g.row_offsets, g.col_indices, src[rows], threads=6)
     assert np.array_equal(lv_h, lv_ref) and np.array_equal(pa_h, pa_ref)
     graphs.check_invariants(g.row_offsets, g.col_indices, src[rows], lv_h, pa_h, rows=[0, 3])
-    # Traversed-edge checksum: every row's count is within [0, m] and the total is consistent.
     assert 0 < edges <= S * g.m
     deg = g.degrees().astype(np.int64)
     assert e_rows == int(sum(deg[lv_h[i] >= 0].sum() for i in range(len(rows))))

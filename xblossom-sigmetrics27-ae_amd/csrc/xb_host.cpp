@@ -77,6 +77,21 @@ void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+// Copies on the owning (non-blocking) stream and waits, so later work on that
+// stream is ordered after the copy (a plain hipMemcpy/hipMemset runs on the
+// null stream, which non-blocking streams do not wait for).
+void h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (!bytes) return;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (!bytes) return;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 int resolve_threads() {
   unsigned hc = std::thread::hardware_concurrency();
   return (int)std::max(1u, std::min(hc, 16u));
@@ -265,8 +280,8 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
   HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   g->d_off = dalloc<int32_t>((size_t)n + 1);
   g->d_col = dalloc<int32_t>((size_t)m);
-  HIPCHK(hipMemcpy(g->d_off, off, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
-  if (m) HIPCHK(hipMemcpy(g->d_col, colv.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice));
+  h2d(g->d_off, off, sizeof(int32_t) * ((size_t)n + 1), g->stream);
+  h2d(g->d_col, colv.data(), sizeof(int32_t) * (size_t)m, g->stream);
   if (flags & XB_GRAPH_DIRECTED) {
     // Transpose by counting sort on the destination; sources are visited in
     // ascending order, so every in-list comes out ascending.
@@ -280,8 +295,8 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
       for (int32_t j = off[u]; j < off[u + 1]; ++j) icol[pos[colv[j]]++] = (int32_t)u;
     g->d_in_off = dalloc<int32_t>((size_t)n + 1);
     g->d_in_col = dalloc<int32_t>((size_t)m);
-    HIPCHK(hipMemcpy(g->d_in_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
-    if (m) HIPCHK(hipMemcpy(g->d_in_col, icol.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice));
+    h2d(g->d_in_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
+    h2d(g->d_in_col, icol.data(), sizeof(int32_t) * (size_t)m, g->stream);
   } else {
     g->d_in_off = g->d_off;
     g->d_in_col = g->d_col;
@@ -381,10 +396,10 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
     g->d_hub_chunks = nullptr;
     g->d_perm = dalloc<int32_t>(perm.size());
     if (!perm.empty())
-      HIPCHK(hipMemcpy(g->d_perm, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+      h2d(g->d_perm, perm.data(), sizeof(int32_t) * perm.size(), g->stream);
     g->d_hub_chunks = dalloc<xb::HubChunk>(chunks.size());
     if (!chunks.empty())
-      HIPCHK(hipMemcpy(g->d_hub_chunks, chunks.data(), sizeof(xb::HubChunk) * chunks.size(), hipMemcpyHostToDevice));
+      h2d(g->d_hub_chunks, chunks.data(), sizeof(xb::HubChunk) * chunks.size(), g->stream);
     g->n_hub_chunks = (int32_t)chunks.size();
     g->n_hubs = hubs;
     g->hub_edges = hub_edges;
@@ -402,8 +417,8 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
     g->d_heavy = dalloc<int32_t>(hv.size());
     g->d_hidx = dalloc<int32_t>((size_t)n);
     if (!hv.empty())
-      HIPCHK(hipMemcpy(g->d_heavy, hv.data(), sizeof(int32_t) * hv.size(), hipMemcpyHostToDevice));
-    if (n) HIPCHK(hipMemcpy(g->d_hidx, hidx.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+      h2d(g->d_heavy, hv.data(), sizeof(int32_t) * hv.size(), g->stream);
+    h2d(g->d_hidx, hidx.data(), sizeof(int32_t) * (size_t)n, g->stream);
     g->n_heavy = (int32_t)hv.size();
     g->cut_small = o.cut_small;
     g->cut_medium = o.cut_medium;
@@ -787,6 +802,39 @@ void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S
   }
 }
 
+xb_msbfs_opts parse_opts(const xb_msbfs_opts* opts) {
+  xb_msbfs_opts o;
+  xb_opts_default(&o);
+  if (opts) {
+    o.mode = opts->mode;
+    o.flags = opts->flags;
+    if (opts->words_per_tile) o.words_per_tile = opts->words_per_tile;
+    if (opts->alpha) o.alpha = opts->alpha;
+    if (opts->pull_cut_small) o.pull_cut_small = opts->pull_cut_small;
+    if (opts->pull_cut_medium) o.pull_cut_medium = opts->pull_cut_medium;
+    if (opts->pull_cut_hub) o.pull_cut_hub = opts->pull_cut_hub;
+    if (opts->hub_chunk) o.hub_chunk = opts->hub_chunk;
+  }
+  if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
+  const int T = o.words_per_tile;
+  if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
+  if (o.alpha < 1 || o.pull_cut_small < 1 || o.pull_cut_medium < o.pull_cut_small ||
+      o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1)
+    fail(XB_ERR_INVALID_ARG, "bad tunable");
+  return o;
+}
+
+BPOpts bp_opts(const xb_msbfs_opts& o) {
+  return BPOpts{o.words_per_tile, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk};
+}
+
+void check_sources(const xb_graph* g, const int32_t* sources, int32_t S) {
+  for (int32_t i = 0; i < S; ++i)
+    if (sources[i] < 0 || sources[i] >= g->n)
+      fail(XB_ERR_SOURCE_RANGE, "source " + std::to_string(i) + " = " + std::to_string(sources[i]) +
+                                    " outside [0, " + std::to_string(g->n) + ")");
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -852,29 +900,10 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     if (S < 0) fail(XB_ERR_INVALID_ARG, "S < 0");
     if (S == 0) return;
     if (!sources || !parents) fail(XB_ERR_INVALID_ARG, "null sources or parents");
-    xb_msbfs_opts o;
-    xb_opts_default(&o);
-    if (opts) {
-      o.mode = opts->mode;
-      o.flags = opts->flags;
-      if (opts->words_per_tile) o.words_per_tile = opts->words_per_tile;
-      if (opts->alpha) o.alpha = opts->alpha;
-      if (opts->pull_cut_small) o.pull_cut_small = opts->pull_cut_small;
-      if (opts->pull_cut_medium) o.pull_cut_medium = opts->pull_cut_medium;
-      if (opts->pull_cut_hub) o.pull_cut_hub = opts->pull_cut_hub;
-      if (opts->hub_chunk) o.hub_chunk = opts->hub_chunk;
-    }
-    if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
-    if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
+    const xb_msbfs_opts o = parse_opts(opts);
     const int T = o.words_per_tile;
-    if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
-    if (o.alpha < 1 || o.pull_cut_small < 1 || o.pull_cut_medium < o.pull_cut_small ||
-        o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1)
-      fail(XB_ERR_INVALID_ARG, "bad tunable");
-    for (int32_t i = 0; i < S; ++i)
-      if (sources[i] < 0 || sources[i] >= g->n)
-        fail(XB_ERR_SOURCE_RANGE, "source " + std::to_string(i) + " = " + std::to_string(sources[i]) +
-                                      " outside [0, " + std::to_string(g->n) + ")");
+    if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
+    check_sources(g, sources, S);
     const bool dev_out = (o.flags & XB_OUT_DEVICE) != 0;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     Profiler prof;
@@ -887,7 +916,7 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     if (o.mode == XB_MODE_PLAIN) {
       run_plain(g, sources, S, levels, parents, dev_out, prof, stats);
     } else {
-      BPOpts bo{T, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk};
+      const BPOpts bo = bp_opts(o);
       run_batched(g, bo, sources, S, levels, parents, dev_out, prof, stats);
     }
     HIPCHK(hipEventRecord(e1, g->stream));
@@ -939,20 +968,38 @@ int32_t xb_test_edge_map(const int32_t* degrees, int32_t k, int32_t* owner_out, 
     unsigned long long* d_ctr = dalloc<unsigned long long>(1);
     int32_t* d_owner = dalloc<int32_t>((size_t)total);
     int32_t* d_off = dalloc<int32_t>((size_t)total);
-    HIPCHK(hipMemcpy(d_deg, degrees, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(d_ctr, 0, sizeof(unsigned long long)));
+    h2d(d_deg, degrees, sizeof(int32_t) * (size_t)k, st);
+    HIPCHK(hipMemsetAsync(d_ctr, 0, sizeof(unsigned long long), st));
     xb::Queue q{d_v, d_eo, d_ctr};
     xb::launch_test_build_queue(st, d_deg, k, q);
     xb::launch_test_edge_map(st, q, k, (int32_t)total, d_owner, d_off);
     HIPCHK(hipStreamSynchronize(st));
     unsigned long long ctr = 0;
-    HIPCHK(hipMemcpy(&ctr, d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(owner_out, d_owner, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(offset_out, d_off, sizeof(int32_t) * (size_t)total, hipMemcpyDeviceToHost));
+    d2h(&ctr, d_ctr, sizeof(ctr), st);
+    d2h(owner_out, d_owner, sizeof(int32_t) * (size_t)total, st);
+    d2h(offset_out, d_off, sizeof(int32_t) * (size_t)total, st);
     for (void* p : {(void*)d_deg, (void*)d_v, (void*)d_eo, (void*)d_ctr, (void*)d_owner, (void*)d_off}) dfree(p);
     (void)hipStreamDestroy(st);
     if ((int64_t)(ctr & 0xFFFFFFFFull) != k || (int64_t)(ctr >> 32) != total)
       fail(XB_ERR_DEVICE, "queue counter does not match the input");
+  });
+}
+
+int32_t xb_device_checksum(const int32_t* a_dev, int64_t count, uint64_t* out) {
+  return guarded([&] {
+    if ((!a_dev && count > 0) || count < 0 || !out) fail(XB_ERR_INVALID_ARG, "bad argument");
+    ensure_device();
+    unsigned long long* d = dalloc<unsigned long long>(1);
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
+    if (count > 0) xb::launch_checksum(st, a_dev, count, d);
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipStreamDestroy(st);
+    dfree(d);
+    *out = (uint64_t)h;
   });
 }
 
@@ -972,17 +1019,407 @@ int32_t xb_device_free(void* ptr) {
 }
 
 int32_t xb_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-  return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+  return guarded([&] {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes) {
-  return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+  return guarded([&] {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 int32_t xb_last_error(char* buf, size_t len) {
   if (!buf || len == 0) return XB_ERR_INVALID_ARG;
   std::snprintf(buf, len, "%s", g_last_error.c_str());
   return XB_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched multi-graph runs: one persistent, hipGraph-captured launch
+// ===========================================================================
+struct xb_batch {
+  hipStream_t stream = nullptr;
+  int T = 8;
+  int alpha = 20;
+  int cut_small = 0, cut_medium = 0, cut_hub = 0, hub_chunk = 0;
+  std::vector<xb_graph*> graphs;
+  std::vector<void*> allocs;  // device memory owned by the batch
+  xb::Team* d_teams = nullptr;
+  xb::GraphView* d_views = nullptr;
+  xb::TileJob* d_jobs = nullptr;
+  xb::TeamCtl* d_ctl = nullptr;
+  unsigned* d_err = nullptr;
+  std::vector<xb::TileJob> h_jobs;
+  std::vector<xb::Team> h_teams;
+  std::vector<std::vector<int32_t>> job_sources;  // host copy of each tile's sources (plain re-runs)
+  int32_t n_teams = 0;
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  template <class U>
+  U* alloc(size_t count) {
+    U* p = dalloc<U>(count);
+    allocs.push_back(p);
+    return p;
+  }
+  ~xb_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) dfree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Clean-state invariant of team workspaces: fronts, next words and touched
+// flags are zero between tiles.  Re-established after an aborted launch.
+void batch_clear_state(xb_batch* b) {
+  for (size_t i = 0; i < b->h_teams.size(); ++i) {
+    const xb::Team& tm = b->h_teams[i];
+    const xb_graph* g = b->graphs[b->h_jobs[tm.job_begin].graph];
+    const size_t nw = (size_t)g->n * b->T + 1;
+    HIPCHK(hipMemsetAsync(tm.s.front[0], 0, sizeof(uint64_t) * nw, b->stream));
+    HIPCHK(hipMemsetAsync(tm.s.front[1], 0, sizeof(uint64_t) * nw, b->stream));
+    HIPCHK(hipMemsetAsync(tm.s.nextw, 0, sizeof(uint64_t) * nw, b->stream));
+    HIPCHK(hipMemsetAsync(tm.s.tflag, 0, sizeof(uint32_t) * ((size_t)g->n + 1), b->stream));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+}
+
+xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* sources, const int32_t* S,
+                      int32_t* const* levels_dev, int32_t* const* parents_dev, const xb_msbfs_opts& o) {
+  if (G <= 0 || !graphs || !sources || !S || !parents_dev) fail(XB_ERR_INVALID_ARG, "empty or null batch");
+  const bool no_levels = (o.flags & XB_NO_LEVELS) != 0;
+  if (!levels_dev && !no_levels) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
+  for (int32_t i = 0; i < G; ++i) {
+    if (!graphs[i]) fail(XB_ERR_INVALID_ARG, "null graph in batch");
+    if (S[i] < 0) fail(XB_ERR_INVALID_ARG, "negative source count");
+    if (S[i] > 0 && (!sources[i] || !parents_dev[i] || (!no_levels && !levels_dev[i])))
+      fail(XB_ERR_INVALID_ARG, "null sources or outputs for graph " + std::to_string(i));
+    check_sources(graphs[i], sources[i], S[i]);
+  }
+  std::unique_ptr<xb_batch> b(new xb_batch());
+  const BPOpts bo = bp_opts(o);
+  b->T = bo.T;
+  b->alpha = bo.alpha;
+  b->cut_small = bo.cut_small;
+  b->cut_medium = bo.cut_medium;
+  b->cut_hub = bo.cut_hub;
+  b->hub_chunk = bo.hub_chunk;
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  const int T = b->T;
+  const int32_t tile = 64 * T;
+
+  // Graph views (one per graph of the batch).
+  std::vector<xb::GraphView> views((size_t)G);
+  for (int32_t i = 0; i < G; ++i) {
+    xb_graph* g = graphs[i];
+    b->graphs.push_back(g);
+    ensure_layout(g, bo);
+    xb::GraphView& v = views[i];
+    v.off = g->d_off;
+    v.col = g->d_col;
+    v.in_off = g->d_in_off;
+    v.in_col = g->d_in_col;
+    v.perm = g->d_perm;
+    for (int c = 0; c < 3; ++c) v.bucket_end[c] = g->bucket_end[c];
+    v.n_hub_chunks = g->n_hub_chunks;
+    v.hubs = g->d_hub_chunks;
+    v.heavy = g->d_heavy;
+    v.hidx = g->d_hidx;
+    v.n_heavy = g->n_heavy;
+    v.n = g->n;
+    v.cut_medium = bo.cut_medium;
+    v.pad = 0;
+    v.m = g->m;
+  }
+
+  // Tiles: sources and first frontiers, packed into one device buffer.
+  std::vector<int32_t> packed;
+  struct TileHost {
+    int32_t graph, base, cnt;
+    size_t src_off, q0_off;
+    int32_t qn0, E0;
+  };
+  std::vector<TileHost> tiles;
+  for (int32_t i = 0; i < G; ++i) {
+    for (int32_t base = 0; base < S[i]; base += tile) {
+      const int32_t cnt = std::min(tile, S[i] - base);
+      TileHost th{i, base, cnt, packed.size(), 0, 0, 0};
+      packed.insert(packed.end(), sources[i] + base, sources[i] + base + cnt);
+      std::vector<int32_t> uniq(sources[i] + base, sources[i] + base + cnt);
+      std::sort(uniq.begin(), uniq.end());
+      uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+      th.q0_off = packed.size();
+      th.qn0 = (int32_t)uniq.size();
+      int64_t E = 0;
+      std::vector<int32_t> eo(uniq.size());
+      for (size_t k = 0; k < uniq.size(); ++k) {
+        eo[k] = (int32_t)E;
+        E += graphs[i]->outdeg(uniq[k]);
+      }
+      th.E0 = (int32_t)E;
+      packed.insert(packed.end(), uniq.begin(), uniq.end());
+      packed.insert(packed.end(), eo.begin(), eo.end());
+      tiles.push_back(th);
+      b->job_sources.emplace_back(sources[i] + base, sources[i] + base + cnt);
+    }
+  }
+  int32_t* d_packed = b->alloc<int32_t>(packed.size());
+  if (!packed.empty())
+    h2d(d_packed, packed.data(), sizeof(int32_t) * packed.size(), b->stream);
+
+  // Teams: one per graph with tiles, workgroups in proportion to estimated cost.
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  const int per_cu = std::max(1, xb::persistent_blocks_per_cu(T));
+  const int budget = per_cu * prop.multiProcessorCount;
+  std::vector<int32_t> team_graph;
+  std::vector<double> cost;
+  for (int32_t i = 0; i < G; ++i)
+    if (S[i] > 0) {
+      team_graph.push_back(i);
+      const double tiles_i = (double)((S[i] + tile - 1) / tile);
+      cost.push_back(tiles_i * ((double)graphs[i]->m + 8.0 * graphs[i]->n));
+    }
+  const int nt = (int)team_graph.size();
+  if (nt == 0) fail(XB_ERR_INVALID_ARG, "batch has no sources");
+  if (nt > budget) fail(XB_ERR_UNSUPPORTED, "more graphs in one batch than co-resident workgroups");
+  double total_cost = 0;
+  for (double c : cost) total_cost += c;
+  std::vector<int> blocks((size_t)nt);
+  int used = 0;
+  for (int t = 0; t < nt; ++t) {
+    const xb_graph* g = graphs[team_graph[t]];
+    const int useful = (int)std::max<int64_t>(1, (g->m + (int64_t)g->n) / 4096);
+    blocks[t] = std::max(1, std::min(useful, (int)(budget * cost[t] / total_cost)));
+    used += blocks[t];
+  }
+  while (used > budget) {
+    int big = (int)(std::max_element(blocks.begin(), blocks.end()) - blocks.begin());
+    --blocks[big];
+    --used;
+  }
+  // Hand leftover workgroups to the costliest teams that can use them.
+  for (bool grew = true; grew && used < budget;) {
+    grew = false;
+    for (int t = 0; t < nt && used < budget; ++t) {
+      const xb_graph* g = graphs[team_graph[t]];
+      const int useful = (int)std::max<int64_t>(1, (g->m + (int64_t)g->n) / 4096);
+      if (blocks[t] < useful && blocks[t] < (int)(budget * cost[t] / total_cost) + 1) {
+        ++blocks[t];
+        ++used;
+        grew = true;
+      }
+    }
+  }
+
+  b->d_ctl = b->alloc<xb::TeamCtl>((size_t)nt);
+  b->d_err = b->alloc<unsigned>(1);
+  int first = 0;
+  for (int t = 0; t < nt; ++t) {
+    const int32_t gi = team_graph[t];
+    xb_graph* g = graphs[gi];
+    xb::Team tm{};
+    tm.first_block = first;
+    tm.nblocks = blocks[t];
+    first += blocks[t];
+    tm.job_begin = (int32_t)b->h_jobs.size();
+    for (const TileHost& th : tiles) {
+      if (th.graph != gi) continue;
+      xb::TileJob j{};
+      j.graph = gi;
+      j.cnt = th.cnt;
+      j.src = d_packed + th.src_off;
+      j.q0v = d_packed + th.q0_off;
+      j.q0eo = d_packed + th.q0_off + th.qn0;
+      j.qn0 = th.qn0;
+      j.E0 = th.E0;
+      j.lvl_out = no_levels ? nullptr : levels_dev[gi] + (int64_t)th.base * g->n;
+      j.par_out = parents_dev[gi] + (int64_t)th.base * g->n;
+      j.status = -1;
+      j.levels = -1;
+      b->h_jobs.push_back(j);
+    }
+    tm.job_end = (int32_t)b->h_jobs.size();
+    const size_t nw = (size_t)g->n * T + 1;
+    tm.s.seen = b->alloc<uint64_t>(nw);
+    tm.s.front[0] = b->alloc<uint64_t>(nw);
+    tm.s.front[1] = b->alloc<uint64_t>(nw);
+    tm.s.nextw = b->alloc<uint64_t>(nw);
+    tm.s.planes = b->alloc<uint64_t>(nw * 8);
+    tm.s.tflag = b->alloc<uint32_t>((size_t)g->n + 1);
+    tm.s.tlist = b->alloc<int32_t>((size_t)g->n + 1);
+    tm.s.hpar = b->alloc<uint32_t>((size_t)std::max(g->n_heavy, 1) * T * 64);
+    tm.s.tcount = nullptr;  // set per level from the control block
+    for (int k = 0; k < 2; ++k) {
+      tm.qv[k] = b->alloc<int32_t>((size_t)g->n + 1);
+      tm.qeo[k] = b->alloc<int32_t>((size_t)g->n + 1);
+    }
+    tm.ctl = b->d_ctl + t;
+    b->h_teams.push_back(tm);
+  }
+  // Job order follows the teams; remap the host-side source copies to match.
+  {
+    std::vector<std::vector<int32_t>> srcs;
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < tiles.size(); ++k)
+        if (tiles[k].graph == team_graph[t]) srcs.push_back(b->job_sources[k]);
+    b->job_sources.swap(srcs);
+  }
+  b->n_teams = nt;
+  b->grid = first;
+  b->d_teams = b->alloc<xb::Team>((size_t)nt);
+  b->d_views = b->alloc<xb::GraphView>((size_t)G);
+  b->d_jobs = b->alloc<xb::TileJob>(b->h_jobs.size());
+  h2d(b->d_teams, b->h_teams.data(), sizeof(xb::Team) * nt, b->stream);
+  h2d(b->d_views, views.data(), sizeof(xb::GraphView) * G, b->stream);
+  h2d(b->d_jobs, b->h_jobs.data(), sizeof(xb::TileJob) * b->h_jobs.size(), b->stream);
+  batch_clear_state(b.get());
+
+  // Capture the launch (control reset + persistent kernel) once.
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  xb::launch_bp_persistent(b->stream, T, b->grid, b->d_teams, b->n_teams, b->d_views, b->d_jobs,
+                           (int32_t)b->h_jobs.size(), b->d_ctl, b->alpha, b->d_err);
+  HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return b.release();
+}
+
+void run_batch(xb_batch* b, xb_run_stats* st) {
+  for (xb_graph* g : b->graphs)
+    if (g->cut_small != b->cut_small || g->cut_medium != b->cut_medium || g->cut_hub != b->cut_hub ||
+        g->hub_chunk != b->hub_chunk)
+      fail(XB_ERR_INVALID_ARG, "a graph of the batch was re-tuned after the batch was created");
+  if (st) std::memset(st, 0, sizeof(*st));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, b->stream));
+  HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  HIPCHK(hipEventRecord(e1, b->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  unsigned err = 0;
+  d2h(&err, b->d_err, sizeof(err), b->stream);
+  std::vector<xb::TileJob> jobs(b->h_jobs.size());
+  d2h(jobs.data(), b->d_jobs, sizeof(xb::TileJob) * jobs.size(), b->stream);
+  if (err) {
+    batch_clear_state(b);
+    fail(XB_ERR_DEVICE, "persistent batch aborted: a team barrier timed out (workgroups not co-resident)");
+  }
+  double bytes = 0;
+  int32_t max_level = 0;
+  for (size_t k = 0; k < jobs.size(); ++k) {
+    const xb::TileJob& j = jobs[k];
+    xb_graph* g = b->graphs[j.graph];
+    if (j.status == 1) {
+      // Deeper than the level code: the plain path writes this tile's rows.
+      Profiler prof;
+      prof.st = g->stream;
+      run_plain(g, b->job_sources[k].data(), j.cnt, j.lvl_out, j.par_out, true, prof, st);
+      if (st) st->fallback_sources += j.cnt;
+      continue;
+    }
+    if (j.status != 0) fail(XB_ERR_DEVICE, "a batch tile did not complete");
+    max_level = std::max(max_level, j.levels);
+    // Section 8(d) bytes of the tile: L CSR streams, L state streams, outputs.
+    const double L = j.levels + 1.0, W = (j.cnt + 63) / 64, n = g->n, m = (double)g->m;
+    bytes += L * (4.0 * (n + 1) + 4.0 * m) + L * 24.0 * W * n + (j.lvl_out ? 8.0 : 4.0) * j.cnt * n;
+  }
+  if (st) {
+    st->ms_total = ms;
+    st->kernel_ms[XB_K_PERSISTENT] = ms;
+    st->kernel_launches[XB_K_PERSISTENT] = 1;
+    st->kernel_bytes[XB_K_PERSISTENT] = bytes;
+    st->max_level = std::max(st->max_level, max_level);
+    st->tiles = (int32_t)jobs.size();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t xb_batch_create(xb_graph* const* graphs, int32_t G, const int32_t* const* sources, const int32_t* S,
+                        int32_t* const* levels_dev, int32_t* const* parents_dev, const xb_msbfs_opts* opts,
+                        xb_batch** out) {
+  return guarded([&] {
+    if (!out) fail(XB_ERR_INVALID_ARG, "null out handle");
+    *out = build_batch(graphs, G, sources, S, levels_dev, parents_dev, parse_opts(opts));
+  });
+}
+
+int32_t xb_batch_run(xb_batch* b, xb_run_stats* stats) {
+  return guarded([&] {
+    if (!b) fail(XB_ERR_INVALID_ARG, "null batch");
+    run_batch(b, stats);
+  });
+}
+
+int32_t xb_batch_destroy(xb_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int32_t xb_msbfs_run_batched(xb_graph* const* graphs, int32_t G, const int32_t* const* sources, const int32_t* S,
+                             int32_t* const* levels, int32_t* const* parents, const xb_msbfs_opts* opts,
+                             xb_run_stats* stats) {
+  return guarded([&] {
+    const xb_msbfs_opts o = parse_opts(opts);
+    if (G <= 0 || !graphs || !S || !parents) fail(XB_ERR_INVALID_ARG, "empty or null batch");
+    const bool no_levels = (o.flags & XB_NO_LEVELS) != 0;
+    if (o.flags & XB_OUT_DEVICE) {
+      std::unique_ptr<xb_batch> b(build_batch(graphs, G, sources, S, levels, parents, o));
+      run_batch(b.get(), stats);
+      return;
+    }
+    // Host outputs: stage the tables on the device, then copy back.
+    std::vector<int32_t*> dl((size_t)G, nullptr), dp((size_t)G, nullptr);
+    std::vector<void*> owned;
+    struct Free {
+      std::vector<void*>& v;
+      ~Free() {
+        for (void* p : v) dfree(p);
+      }
+    } freer{owned};
+    for (int32_t i = 0; i < G; ++i) {
+      if (!graphs[i]) fail(XB_ERR_INVALID_ARG, "null graph in batch");
+      const size_t cells = (size_t)std::max(S[i], 0) * (size_t)graphs[i]->n;
+      dp[i] = dalloc<int32_t>(cells);
+      owned.push_back(dp[i]);
+      if (!no_levels) {
+        dl[i] = dalloc<int32_t>(cells);
+        owned.push_back(dl[i]);
+      }
+    }
+    {
+      std::unique_ptr<xb_batch> b(build_batch(graphs, G, sources, S, no_levels ? nullptr : dl.data(), dp.data(), o));
+      run_batch(b.get(), stats);
+    }
+    for (int32_t i = 0; i < G; ++i) {
+      const size_t cells = (size_t)std::max(S[i], 0) * (size_t)graphs[i]->n;
+      if (cells == 0) continue;
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipMemcpy(parents[i], dp[i], sizeof(int32_t) * cells, hipMemcpyDeviceToHost));
+      if (!no_levels) HIPCHK(hipMemcpy(levels[i], dl[i], sizeof(int32_t) * cells, hipMemcpyDeviceToHost));
+    }
+  });
 }
 
 }  // extern "C"

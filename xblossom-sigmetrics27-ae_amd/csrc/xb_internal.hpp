@@ -38,6 +38,64 @@ struct HubChunk {
   int32_t v, jb, je, pad;
 };
 
+// ---- persistent batch kernel ----
+// Read-only view of one device graph and its in-degree layout.
+struct GraphView {
+  const int32_t* off;
+  const int32_t* col;
+  const int32_t* in_off;
+  const int32_t* in_col;
+  const int32_t* perm;
+  int32_t bucket_end[3];
+  int32_t n_hub_chunks;
+  const HubChunk* hubs;
+  const int32_t* heavy;
+  const int32_t* hidx;
+  int32_t n_heavy;
+  int32_t n;
+  int32_t cut_medium;
+  int32_t pad;
+  int64_t m;
+};
+
+// One tile (up to 64*T sources) of one graph.
+struct TileJob {
+  int32_t graph;           // index into the GraphView array
+  int32_t cnt;             // sources in the tile
+  const int32_t* src;      // device, cnt ids
+  const int32_t* q0v;      // first frontier: distinct source vertices
+  const int32_t* q0eo;     // and the exclusive prefix of their out-degrees
+  int32_t qn0, E0;
+  int32_t* lvl_out;        // rows of this tile (row stride n); may be null
+  int32_t* par_out;
+  int32_t status;          // out: 0 done, 1 deeper than the level code (re-run on the plain path)
+  int32_t levels;          // out: deepest level reached
+};
+
+// Control block of one team (zeroed by a kernel at the start of every launch).
+struct TeamCtl {
+  unsigned bar_count;
+  unsigned bar_gen;
+  unsigned long long ctr[3];  // packed (edges << 32 | count) queue counters, level d uses ctr[d % 3]
+  uint32_t tcount[2];         // touched-list counts, level d uses tcount[d & 1]
+};
+
+// A team: a contiguous range of workgroups running a contiguous range of jobs
+// (the tiles of one graph) with its own workspace.
+struct Team {
+  int32_t first_block, nblocks;
+  int32_t job_begin, job_end;
+  BPState s;
+  int32_t* qv[2];
+  int32_t* qeo[2];
+  TeamCtl* ctl;
+};
+
+int persistent_blocks_per_cu(int T);
+void launch_bp_persistent(hipStream_t st, int T, int grid_blocks, const Team* teams, int32_t n_teams,
+                          const GraphView* graphs, TileJob* jobs, int32_t n_jobs, TeamCtl* ctl, int32_t alpha,
+                          unsigned* gerr);
+
 // ---- plain per-source engine ----
 void launch_plain_init(hipStream_t st, int32_t* level, int32_t* parent, int32_t n);
 void launch_plain_set_source(hipStream_t st, int32_t* level, int32_t* parent, int32_t src, Queue q);
@@ -68,6 +126,8 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, i
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,
                             unsigned long long* out);
+
+void launch_checksum(hipStream_t st, const int32_t* a, int64_t count, unsigned long long* out);
 
 // ---- test hooks ----
 void launch_test_build_queue(hipStream_t st, const int32_t* deg, int32_t k, Queue q);

@@ -24,14 +24,20 @@
 //    lanes), and hub adjacency lists are cut into fixed pieces with one wave
 //    each.  Level codes are kept as 8 bit-sliced planes per (vertex, word).
 //    After the last level an emit pass writes the int32 level and parent
-//    tables in vertex order, resolving each parent as the lowest-id
-//    in-neighbour whose code is one less.
+//    tables in vertex order.
 //
-// Frontier compaction is wave-aggregated: a ballot gives each lane its rank,
-// a wave prefix sum gives its edge offset, and one 64-bit atomicAdd on a
-// packed (edges << 32 | count) counter reserves both ranges
-// (apps/xb/src/include/enode_queue/enode_queue.h:46-64 does one atomic per
-// warp for the slot range only).
+// Every phase is a __device__ function over a "team" of workgroups (Grid:
+// block index within the team, team size).  The host-driven engine launches
+// each phase as its own kernel (the team is the whole grid); the persistent
+// batch kernel gives every graph of a batch its own team of workgroups and runs
+// the level loop on the device, with team barriers between phases.
+//
+// Frontier compaction is staged in LDS: a ballot gives each lane its rank, one
+// LDS atomic per wave reserves staging slots, and the block flushes with a
+// block prefix sum and ONE 64-bit global atomic on a packed (edges << 32 |
+// count) counter that reserves both the queue slots and the edge range
+// (apps/xb/src/include/enode_queue/enode_queue.h:46-64 does one global atomic
+// per warp for the slot range only).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -44,18 +50,27 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 8192;
-constexpr int kPushR = 4;                 // edges per thread per chunk
-constexpr int kChunk = kBlock * kPushR;   // edges per block chunk
-constexpr int kPullU = 4;                 // independent neighbour loads per lane per step
+constexpr int kPushR = 4;                // edges per thread per chunk
+constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
+constexpr int kPullU = 4;                // independent neighbour loads per lane per step
 constexpr int kEmitU = 4;
+constexpr int kStageCap = 2048;
 
+// ---------------------------------------------------------------------------
+// Team geometry
+// ---------------------------------------------------------------------------
+struct Grid {
+  int bid;  // block index within the team
+  int nb;   // blocks in the team
+};
+
+__device__ __forceinline__ Grid self_grid() { return Grid{(int)blockIdx.x, (int)gridDim.x}; }
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
-
-__device__ __forceinline__ int64_t global_wave() {
-  return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-}
-__device__ __forceinline__ int64_t num_waves() { return ((int64_t)gridDim.x * blockDim.x) / kWave; }
+__device__ __forceinline__ int64_t gwave(Grid g) { return ((int64_t)g.bid * kBlock + threadIdx.x) / kWave; }
+__device__ __forceinline__ int64_t gnwaves(Grid g) { return (int64_t)g.nb * (kBlock / kWave); }
+__device__ __forceinline__ int64_t gthread(Grid g) { return (int64_t)g.bid * kBlock + threadIdx.x; }
+__device__ __forceinline__ int64_t gnthreads(Grid g) { return (int64_t)g.nb * kBlock; }
 
 __device__ __forceinline__ int wave_incl_scan(int x) {
   const int l = lane_id();
@@ -77,27 +92,6 @@ __device__ __forceinline__ uint64_t wave_excl_or(uint64_t x) {
   }
   uint64_t ex = __shfl_up(incl, 1, kWave);
   return l == 0 ? 0ull : ex;
-}
-
-// Wave-aggregated append of (v, deg) for lanes with pred set.  Must be called
-// by every lane of the wave (wave-uniform control flow).
-__device__ __forceinline__ void wave_append(bool pred, int32_t v, int32_t deg, Queue q) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
-  const int d = pred ? deg : 0;
-  const int incl = wave_incl_scan(d);
-  const int total = __shfl(incl, kWave - 1, kWave);
-  const int rank = __popcll(mask & lanemask_lt());
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  unsigned long long old = 0;
-  if (lane_id() == leader)
-    old = atomicAdd(q.ctr, ((unsigned long long)(uint32_t)total << 32) | (unsigned long long)__popcll(mask));
-  old = __shfl(old, leader, kWave);
-  if (pred) {
-    const uint32_t pos = (uint32_t)old + (uint32_t)rank;
-    q.v[pos] = v;
-    q.eo[pos] = (int32_t)(old >> 32) + incl - d;
-  }
 }
 
 // Wave-aggregated append of v to a plain list with a 32-bit counter.
@@ -125,40 +119,9 @@ __device__ __forceinline__ int owner_in(const int32_t* __restrict__ eo, int lo, 
   return lo;
 }
 
-// Block-level owner staging for an edge chunk [c0, c1) of a frontier queue.
-// On return s_eo[0..cnt) = eo[lo..lo+cnt) when cnt <= kChunk + 1 (else the
-// owners are searched in global memory between lo and lo+cnt).
-struct ChunkOwners {
-  int lo;
-  int cnt;
-};
-
-__device__ __forceinline__ ChunkOwners stage_owners(const int32_t* __restrict__ eo, int qn, int c0, int c1,
-                                                    int32_t* s_eo, int* s_lohi) {
-  if (threadIdx.x == 0) s_lohi[0] = owner_in(eo, 0, qn, c0);
-  if (threadIdx.x == kWave) s_lohi[1] = owner_in(eo, 0, qn, c1 - 1);
-  __syncthreads();
-  ChunkOwners co{s_lohi[0], s_lohi[1] - s_lohi[0] + 1};
-  if (co.cnt <= kChunk + 1)
-    for (int i = threadIdx.x; i < co.cnt; i += blockDim.x) s_eo[i] = eo[co.lo + i];
-  __syncthreads();
-  return co;
-}
-
-__device__ __forceinline__ int chunk_owner(const ChunkOwners& co, const int32_t* __restrict__ eo,
-                                           const int32_t* s_eo, int e) {
-  if (co.cnt <= kChunk + 1) return co.lo + owner_in(s_eo, 0, co.cnt, e);
-  return owner_in(eo, co.lo, co.lo + co.cnt, e);
-}
-
-// ---- block-level LDS staging of queue appends ----
-// Lanes stage (vertex, degree) entries in LDS (one LDS atomic per wave); the
-// block flushes the stage to the global queue with ONE global atomic per
-// flush, after a block prefix sum over the staged degrees.  Callers run in
-// block-uniform loops and call stage_sync_flush() once per iteration, which
-// flushes when another iteration's worth of entries might not fit.
-constexpr int kStageCap = 2048;
-
+// ---------------------------------------------------------------------------
+// LDS: one union per block, reused phase after phase
+// ---------------------------------------------------------------------------
 struct AppendStage {
   int32_t v[kStageCap];
   int32_t d[kStageCap];
@@ -167,7 +130,33 @@ struct AppendStage {
   unsigned long long base;
 };
 
+struct QueueLds {
+  AppendStage sg;
+  int32_t s_eo[kChunk + 1];
+  int s_lohi[2];
+};
+
+struct EmitLds {
+  int32_t tile[64][65];
+  uint64_t pl[64][8];
+};
+
+union Lds {
+  QueueLds q;
+  EmitLds e;
+};
+
+// One LDS union per workgroup, shared by every phase a kernel runs.
+__shared__ Lds g_lds;
+
+// ---- block-level LDS staging of queue appends ----
+// Lanes stage (vertex, degree) entries in LDS (one LDS atomic per wave); the
+// block flushes the stage to the global queue with ONE global atomic per
+// flush, after a block prefix sum over the staged degrees.  Callers run in
+// block-uniform loops and call stage_sync_flush() once per iteration, which
+// flushes when another iteration's worth of entries might not fit.
 __device__ __forceinline__ void stage_init(AppendStage& sg) {
+  __syncthreads();
   if (threadIdx.x == 0) sg.cnt = 0;
   __syncthreads();
 }
@@ -187,8 +176,8 @@ __device__ __forceinline__ void stage_put(AppendStage& sg, bool pred, int32_t v,
   }
 }
 
-// Block-uniform call.  with_eo: write the degree prefix too (frontier
-// queues); otherwise only ids (touched lists, 32-bit counter in ctr32).
+// Block-uniform call.  list == nullptr: frontier queue with the degree prefix;
+// otherwise a plain id list with a 32-bit counter ctr32.
 __device__ void stage_flush(AppendStage& sg, int cnt, Queue q, int32_t* list, uint32_t* ctr32) {
   const int per = (cnt + kBlock - 1) / kBlock;
   const int i0 = min((int)threadIdx.x * per, cnt);
@@ -237,6 +226,32 @@ __device__ __forceinline__ void stage_sync_flush(AppendStage& sg, bool force, Qu
   __syncthreads();
   if (cnt == 0 || (!force && cnt <= kStageCap - kBlock)) return;
   stage_flush(sg, cnt, q, list, ctr32);
+}
+
+// Block-level owner staging for an edge chunk [c0, c1) of a frontier queue.
+// On return s_eo[0..cnt) = eo[lo..lo+cnt) when cnt <= kChunk + 1 (else the
+// owners are searched in global memory between lo and lo+cnt).
+struct ChunkOwners {
+  int lo;
+  int cnt;
+};
+
+__device__ __forceinline__ ChunkOwners stage_owners(const int32_t* __restrict__ eo, int qn, int c0, int c1,
+                                                    QueueLds& L) {
+  if (threadIdx.x == 0) L.s_lohi[0] = owner_in(eo, 0, qn, c0);
+  if (threadIdx.x == kWave) L.s_lohi[1] = owner_in(eo, 0, qn, c1 - 1);
+  __syncthreads();
+  ChunkOwners co{L.s_lohi[0], L.s_lohi[1] - L.s_lohi[0] + 1};
+  if (co.cnt <= kChunk + 1)
+    for (int i = threadIdx.x; i < co.cnt; i += blockDim.x) L.s_eo[i] = eo[co.lo + i];
+  __syncthreads();
+  return co;
+}
+
+__device__ __forceinline__ int chunk_owner(const ChunkOwners& co, const int32_t* __restrict__ eo,
+                                           const int32_t* s_eo, int e) {
+  if (co.cnt <= kChunk + 1) return co.lo + owner_in(s_eo, 0, co.cnt, e);
+  return owner_in(eo, co.lo, co.lo + co.cnt, e);
 }
 
 inline int grid_for(int64_t items, int per_block) {
@@ -288,20 +303,19 @@ __global__ __launch_bounds__(kBlock) void k_plain_push(const int32_t* __restrict
                                                        const int32_t* __restrict__ col, Queue cur, int32_t qn,
                                                        int32_t E, int32_t* __restrict__ level,
                                                        int32_t* __restrict__ parent, int32_t d, Queue nxt) {
-  __shared__ int32_t s_eo[kChunk + 1];
-  __shared__ int s_lohi[2];
-  __shared__ AppendStage sg;
+  __shared__ QueueLds L;
+  AppendStage& sg = L.sg;
   stage_init(sg);
   for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
     const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
-    const ChunkOwners co = stage_owners(cur.eo, qn, (int)c0, c1, s_eo, s_lohi);
+    const ChunkOwners co = stage_owners(cur.eo, qn, (int)c0, c1, L);
 #pragma unroll 1
     for (int r = 0; r < kPushR; ++r) {
       const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
       bool fresh = false;
       int32_t u = 0, du = 0;
       if (e < c1) {
-        const int i = chunk_owner(co, cur.eo, s_eo, e);
+        const int i = chunk_owner(co, cur.eo, L.s_eo, e);
         const int32_t v = cur.v[i];
         u = col[off[v] + (e - cur.eo[i])];
         int32_t lu = level[u];
@@ -328,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_push(const int32_t* __restrict
 }
 
 // ============================================================================
-// Batched bit-parallel engine
+// Batched bit-parallel engine: phases
 // ============================================================================
 
 __device__ __forceinline__ uint64_t word_mask(int32_t cnt, int t) {
@@ -343,11 +357,19 @@ __device__ __forceinline__ void planes_or(uint64_t* __restrict__ p, uint32_t cod
     if ((code >> k) & 1u) p[k] |= bits;
 }
 
+__device__ void fill_u64(Grid g, uint64_t* __restrict__ p, int64_t count, uint64_t val) {
+  for (int64_t i = gthread(g); i < count; i += gnthreads(g)) p[i] = val;
+}
+
+__device__ void fill_u32(Grid g, uint32_t* __restrict__ p, int64_t count, uint32_t val) {
+  for (int64_t i = gthread(g); i < count; i += gnthreads(g)) p[i] = val;
+}
+
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_seed(BPState s, const int32_t* __restrict__ src, int32_t cnt) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+__device__ void seed_phase(Grid g, BPState s, const int32_t* __restrict__ src, int32_t cnt) {
+  for (int64_t i = gthread(g); i < cnt; i += gnthreads(g)) {
     const int32_t v = src[i];
-    const int t = i >> 6;
+    const int t = (int)(i >> 6);
     const uint64_t bit = 1ull << (i & 63);
     const int64_t w = (int64_t)v * T + t;
     atomicOr((unsigned long long*)&s.seen[w], (unsigned long long)bit);
@@ -358,24 +380,22 @@ __global__ __launch_bounds__(kBlock) void k_bp_seed(BPState s, const int32_t* __
 
 // Top-down push over the frontier queue, kPushR edges per thread.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_push(BPState s, int cur, const int32_t* __restrict__ off,
-                                                    const int32_t* __restrict__ col, Queue q, int32_t qn,
-                                                    int32_t E) {
-  __shared__ int32_t s_eo[kChunk + 1];
-  __shared__ int s_lohi[2];
-  __shared__ AppendStage sg;
+__device__ void push_phase(Grid g, BPState s, int cur, const int32_t* __restrict__ off,
+                           const int32_t* __restrict__ col, Queue q, int32_t qn, int32_t E) {
+  QueueLds& L = g_lds.q;
+  AppendStage& sg = L.sg;
   stage_init(sg);
   const uint64_t* __restrict__ front = s.front[cur];
-  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+  for (int64_t c0 = (int64_t)g.bid * kChunk; c0 < E; c0 += (int64_t)g.nb * kChunk) {
     const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
-    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, s_eo, s_lohi);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, L);
 #pragma unroll 1
     for (int r = 0; r < kPushR; ++r) {
       const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
       bool first = false;
       int32_t u = 0;
       if (e < c1) {
-        const int i = chunk_owner(co, q.eo, s_eo, e);
+        const int i = chunk_owner(co, q.eo, L.s_eo, e);
         const int32_t v = q.v[i];
         u = col[off[v] + (e - q.eo[i])];
         bool any = false;
@@ -401,13 +421,13 @@ __global__ __launch_bounds__(kBlock) void k_bp_push(BPState s, int cur, const in
 // Commit atomically gathered next words (push levels and pulled hubs): move
 // them into seen/front, record level codes, append to the next queue.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32_t code,
-                                                      const int32_t* __restrict__ out_off, Queue nxt) {
-  __shared__ AppendStage sg;
+__device__ void commit_phase(Grid g, BPState s, int cur, uint32_t code,
+                             const int32_t* __restrict__ out_off, Queue nxt) {
+  AppendStage& sg = g_lds.q.sg;
   stage_init(sg);
-  const uint32_t cnt = *s.tcount;
+  const uint32_t cnt = __hip_atomic_load(s.tcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t* __restrict__ fnext = s.front[cur ^ 1];
-  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < (int64_t)cnt; b0 += (int64_t)gridDim.x * kBlock) {
+  for (int64_t b0 = (int64_t)g.bid * kBlock; b0 < (int64_t)cnt; b0 += (int64_t)g.nb * kBlock) {
     const int64_t idx = b0 + threadIdx.x;
     bool any = false;
     int32_t u = 0, du = 0;
@@ -445,16 +465,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32
 // the first provider of every new bit -- the lowest-id parent -- is recorded
 // in the heavy-parent table s.hpar.
 template <int T, int P, bool REC>
-__global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
-                                                           const int32_t* __restrict__ in_off,
-                                                           const int32_t* __restrict__ in_col,
-                                                           const int32_t* __restrict__ out_off,
-                                                           const int32_t* __restrict__ perm, int32_t begin,
-                                                           int32_t end, Queue nxt) {
+__device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
+                                  const int32_t* __restrict__ in_off, const int32_t* __restrict__ in_col,
+                                  const int32_t* __restrict__ out_off, const int32_t* __restrict__ perm,
+                                  int32_t begin, int32_t end, Queue nxt) {
   constexpr int G = T * P;
   constexpr int VPW = kWave / G;
   constexpr int VPB = (kBlock / kWave) * VPW;
-  __shared__ AppendStage sg;
+  AppendStage& sg = g_lds.q.sg;
   stage_init(sg);
   const int l = lane_id();
   const int t = l % T;
@@ -464,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, u
   const uint64_t vmask = word_mask(cnt, t);
   const uint64_t* __restrict__ front = s.front[cur];
   uint64_t* __restrict__ fnext = s.front[cur ^ 1];
-  for (int64_t bb = begin + (int64_t)blockIdx.x * VPB; bb < end; bb += (int64_t)gridDim.x * VPB) {
+  for (int64_t bb = begin + (int64_t)g.bid * VPB; bb < end; bb += (int64_t)g.nb * VPB) {
     const int64_t idx = bb + wv * VPW + slot;
     const bool act = idx < end;
     int32_t v = 0, b = 0, e = 0;
@@ -492,14 +510,14 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, u
 #pragma unroll
       for (int i = 0; i < kPullU; ++i) fs[i] = (us[i] >= 0) ? front[(int64_t)us[i] * T + t] : 0ull;
       if (!REC) {
-        uint64_t g = 0;
+        uint64_t gg = 0;
 #pragma unroll
-        for (int i = 0; i < kPullU; ++i) g |= fs[i];
-        g &= need;
+        for (int i = 0; i < kPullU; ++i) gg |= fs[i];
+        gg &= need;
 #pragma unroll
-        for (int x = T; x < G; x <<= 1) g |= __shfl_xor(g, x, kWave);
-        got |= g;
-        need &= ~g;
+        for (int x = T; x < G; x <<= 1) gg |= __shfl_xor(gg, x, kWave);
+        got |= gg;
+        need &= ~gg;
       } else {
 #pragma unroll
         for (int i = 0; i < kPullU; ++i) {
@@ -541,20 +559,19 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, u
 }
 
 // Bottom-up pull over hub adjacency pieces: one wave per piece, results
-// OR-ed atomically into the next words and committed by k_bp_commit.  Each
+// OR-ed atomically into the next words and committed by commit_phase.  Each
 // piece records its first provider of every new bit with atomicMin into the
 // heavy-parent table, so the lowest id over all pieces wins.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int32_t cnt,
-                                                        const int32_t* __restrict__ in_col,
-                                                        const HubChunk* __restrict__ chunks, int32_t n_chunks) {
+__device__ void pull_hub_phase(Grid g, BPState s, int cur, int32_t cnt, const int32_t* __restrict__ in_col,
+                               const HubChunk* __restrict__ chunks, int32_t n_chunks) {
   constexpr int P = kWave / T;
   const int l = lane_id();
   const int t = l % T;
   const int p = l / T;
   const uint64_t vmask = word_mask(cnt, t);
   const uint64_t* __restrict__ front = s.front[cur];
-  for (int64_t c = global_wave(); c < n_chunks; c += num_waves()) {
+  for (int64_t c = gwave(g); c < n_chunks; c += gnwaves(g)) {
     const HubChunk hc = chunks[c];
     const int64_t w = (int64_t)hc.v * T + t;
     uint32_t* hp = s.hpar + ((int64_t)s.hidx[hc.v] * T + t) * 64;
@@ -603,11 +620,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int3
 
 // Clear the frontier words of the vertices of a finished level.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const int32_t* __restrict__ qv,
-                                                     int32_t qn) {
+__device__ void clear_phase(Grid g, BPState s, int cur, const int32_t* __restrict__ qv, int32_t qn) {
   uint64_t* __restrict__ f = s.front[cur];
   const int64_t total = (int64_t)qn * T;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = gthread(g); i < total; i += gnthreads(g)) {
     const int32_t v = qv[i / T];
     f[(int64_t)v * T + (i % T)] = 0ull;
   }
@@ -649,12 +665,11 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
 // resolved here, one wave per vertex, lane i testing in-neighbour j0+i, the
 // lowest lane offering a bit (exclusive prefix-OR) owning that bit's parent.
 template <int T, int K>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ heavy, int32_t n_heavy) {
+__device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t* __restrict__ in_off,
+                                 const int32_t* __restrict__ in_col, const int32_t* __restrict__ heavy,
+                                 int32_t n_heavy) {
   const int l = lane_id();
-  for (int64_t h = global_wave(); h < n_heavy; h += num_waves()) {
+  for (int64_t h = gwave(g); h < n_heavy; h += gnwaves(g)) {
     const int32_t v = heavy[h];
     const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
     uint64_t pk[K];
@@ -698,19 +713,17 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
 // wave takes 16 rows and each lane one vertex, so every store writes 256
 // contiguous bytes of one row of the level and parent tables.
 template <int T, int K>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
-                                                          const int32_t* __restrict__ src,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col, int32_t n,
-                                                          int32_t heavy_cut, int32_t* __restrict__ lvl_out,
-                                                          int32_t* __restrict__ par_out) {
-  __shared__ int32_t tile[64][65];
-  __shared__ uint64_t pl[64][K];
+__device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
+                                 const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
+                                 const int32_t* __restrict__ in_col, int32_t n, int32_t heavy_cut,
+                                 int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out) {
+  EmitLds& L = g_lds.e;
   const int l = lane_id();
   const int wv = threadIdx.x / kWave;
   const int vi = wv * 16 + (l >> 2);
   const int p = l & 3;
-  for (int64_t vb = (int64_t)blockIdx.x * 64; vb < n; vb += (int64_t)gridDim.x * 64) {
+  __syncthreads();
+  for (int64_t vb = (int64_t)g.bid * 64; vb < n; vb += (int64_t)g.nb * 64) {
     // ---- compute phase ----
     const int32_t v = (int32_t)(vb + vi);
     const bool act = v < n;
@@ -727,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
     }
     if (p == 0) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) pl[vi][k] = pk[k];
+      for (int k = 0; k < K; ++k) L.pl[vi][k] = pk[k];
     }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
@@ -735,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
     if (heavy) {
       const uint32_t* hp = s.hpar + ((int64_t)s.hidx[v] * T + word) * 64;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) tile[16 * p + i][vi] = (int32_t)hp[16 * p + i];
+      for (int i = 0; i < 16; ++i) L.tile[16 * p + i][vi] = (int32_t)hp[16 * p + i];
     }
     for (int32_t k0 = 0;; k0 += 4 * kEmitU) {
       const bool more = need != 0 && (b + k0 + p < e);
@@ -772,7 +785,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
         while (first) {
           const int bit = __builtin_ctzll(first);
           first &= first - 1;
-          tile[bit][vi] = us[i];
+          L.tile[bit][vi] = us[i];
         }
         need &= ~__shfl(incl, (l & ~3) | 3, kWave);
       }
@@ -783,7 +796,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
     if (v2 < n) {
       uint64_t q[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) q[k] = pl[l][k];
+      for (int k = 0; k < K; ++k) q[k] = L.pl[l][k];
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int bit = 16 * wv + i;
@@ -792,7 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
         uint32_t c = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
-        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : tile[bit][l]));
+        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[bit][l]));
         const int64_t o = (int64_t)r * n + v2;
         if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[o]);
         __builtin_nontemporal_store(par, &par_out[o]);
@@ -801,6 +814,303 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
     __syncthreads();
   }
 }
+
+// ============================================================================
+// Batched bit-parallel engine: standalone kernels (host-driven level loop)
+// ============================================================================
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_bp_seed(BPState s, const int32_t* __restrict__ src, int32_t cnt) {
+  seed_phase<T>(self_grid(), s, src, cnt);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_bp_push(BPState s, int cur, const int32_t* __restrict__ off,
+                                                    const int32_t* __restrict__ col, Queue q, int32_t qn,
+                                                    int32_t E) {
+  push_phase<T>(self_grid(), s, cur, off, col, q, qn, E);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32_t code,
+                                                      const int32_t* __restrict__ out_off, Queue nxt) {
+  commit_phase<T>(self_grid(), s, cur, code, out_off, nxt);
+}
+
+template <int T, int P, bool REC>
+__global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
+                                                           const int32_t* __restrict__ in_off,
+                                                           const int32_t* __restrict__ in_col,
+                                                           const int32_t* __restrict__ out_off,
+                                                           const int32_t* __restrict__ perm, int32_t begin,
+                                                           int32_t end, Queue nxt) {
+  pull_bucket_phase<T, P, REC>(self_grid(), s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end,
+                               nxt);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int32_t cnt,
+                                                        const int32_t* __restrict__ in_col,
+                                                        const HubChunk* __restrict__ chunks, int32_t n_chunks) {
+  pull_hub_phase<T>(self_grid(), s, cur, cnt, in_col, chunks, n_chunks);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const int32_t* __restrict__ qv,
+                                                     int32_t qn) {
+  clear_phase<T>(self_grid(), s, cur, qv, qn);
+}
+
+template <int T, int K>
+__global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word,
+                                                          const int32_t* __restrict__ in_off,
+                                                          const int32_t* __restrict__ in_col,
+                                                          const int32_t* __restrict__ heavy, int32_t n_heavy) {
+  emit_heavy_phase<T, K>(self_grid(), s, word, in_off, in_col, heavy, n_heavy);
+}
+
+template <int T, int K>
+__global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
+                                                          const int32_t* __restrict__ src,
+                                                          const int32_t* __restrict__ in_off,
+                                                          const int32_t* __restrict__ in_col, int32_t n,
+                                                          int32_t heavy_cut, int32_t* __restrict__ lvl_out,
+                                                          int32_t* __restrict__ par_out) {
+  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+}
+
+// ============================================================================
+// Persistent batch kernel: one team of workgroups per graph, device-side
+// level loop, team barriers between phases
+// ============================================================================
+
+// Team barrier.  Every thread fences its writes (agent scope, so they reach
+// the other XCDs' view), one thread per block arrives on the team counter and
+// the last arriver bumps the generation.  Waiting is bounded: after
+// kSpinLimit polls (about 4 s) the team sets the batch error flag and every block returns
+// at its next barrier, so the kernel always terminates.
+constexpr long kSpinLimit = 1L << 22;
+
+__device__ bool team_barrier(TeamCtl* ctl, int nblocks, unsigned* gerr) {
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (nblocks > 1) {
+      const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned arrived = atomicAdd(&ctl->bar_count, 1u);
+      if (arrived == (unsigned)nblocks - 1u) {
+        __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        long spins = 0;
+        while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > kSpinLimit) {
+            atomicOr(gerr, 1u);
+            break;
+          }
+          if ((spins & 1023) == 0 && __hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        }
+      }
+    }
+    if (__hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+    s_ok = ok;
+  }
+  __syncthreads();
+  __threadfence();
+  return s_ok != 0;
+}
+
+// Out-of-line entry points of the phases for the persistent kernel, so each
+// phase keeps its own register allocation instead of one inlined giant.
+template <int T>
+__device__ __noinline__ void nl_seed(Grid g, BPState s, const int32_t* src, int32_t cnt) {
+  seed_phase<T>(g, s, src, cnt);
+}
+template <int T>
+__device__ __noinline__ void nl_push(Grid g, BPState s, int cur, const int32_t* off, const int32_t* col, Queue q,
+                                     int32_t qn, int32_t E) {
+  push_phase<T>(g, s, cur, off, col, q, qn, E);
+}
+template <int T>
+__device__ __noinline__ void nl_commit(Grid g, BPState s, int cur, uint32_t code, const int32_t* out_off,
+                                       Queue nxt) {
+  commit_phase<T>(g, s, cur, code, out_off, nxt);
+}
+template <int T, int P, bool REC>
+__device__ __noinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
+                                            const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
+                                            const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
+  pull_bucket_phase<T, P, REC>(g, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
+}
+template <int T>
+__device__ __noinline__ void nl_pull_hub(Grid g, BPState s, int cur, int32_t cnt, const int32_t* in_col,
+                                         const HubChunk* chunks, int32_t n_chunks) {
+  pull_hub_phase<T>(g, s, cur, cnt, in_col, chunks, n_chunks);
+}
+template <int T>
+__device__ __noinline__ void nl_clear(Grid g, BPState s, int cur, const int32_t* qv, int32_t qn) {
+  clear_phase<T>(g, s, cur, qv, qn);
+}
+__device__ __noinline__ void nl_fill_u64(Grid g, uint64_t* p, int64_t count, uint64_t val) {
+  fill_u64(g, p, count, val);
+}
+__device__ __noinline__ void nl_fill_u32(Grid g, uint32_t* p, int64_t count, uint32_t val) {
+  fill_u32(g, p, count, val);
+}
+template <int T, int K>
+__device__ __noinline__ void nl_emit_heavy(Grid g, BPState s, int32_t word, const int32_t* in_off,
+                                           const int32_t* in_col, const int32_t* heavy, int32_t n_heavy) {
+  emit_heavy_phase<T, K>(g, s, word, in_off, in_col, heavy, n_heavy);
+}
+template <int T, int K>
+__device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
+                                           const int32_t* in_off, const int32_t* in_col, int32_t n,
+                                           int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
+  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+}
+
+template <int T, int K>
+__device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const TileJob& job, TeamCtl* ctl,
+                               unsigned* gerr) {
+  const int words = (job.cnt + 63) / 64;
+  if (gv.n_heavy > 0)
+    for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
+  if (!team_barrier(ctl, g.nb, gerr)) return false;
+  for (int t = 0; t < words; ++t)
+    nl_emit_light<T, K>(g, s, t, job.cnt, job.src, gv.in_off, gv.in_col, gv.n, gv.cut_medium, job.lvl_out,
+                        job.par_out);
+  return true;
+}
+
+// Runs one tile of up to 64*T sources of one graph on a team.  Returns false
+// if the batch was aborted.
+template <int T>
+__device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& job, int32_t alpha,
+                         unsigned* gerr) {
+  TeamCtl* ctl = tm.ctl;
+  BPState s = tm.s;
+  s.hidx = gv.hidx;
+  const int32_t n = gv.n;
+  const int64_t nw = (int64_t)n * T;
+  // Tile set-up: seen and planes cleared, heavy parents unresolved; fronts,
+  // next words and touched flags are clean between tiles by construction.
+  nl_fill_u64(g, s.seen, nw, 0ull);
+  nl_fill_u64(g, s.planes, nw * 8, 0ull);
+  if (gv.n_heavy > 0) nl_fill_u32(g, s.hpar, (int64_t)gv.n_heavy * T * 64, 0xFFFFFFFFu);
+  if (g.bid == 0 && threadIdx.x == 0) {
+    ctl->ctr[1] = 0ull;
+    ctl->tcount[0] = 0u;
+  }
+  if (!team_barrier(ctl, g.nb, gerr)) return false;
+  nl_seed<T>(g, s, job.src, job.cnt);
+  if (!team_barrier(ctl, g.nb, gerr)) return false;
+
+  int32_t qn = job.qn0, E = job.E0, d = 0;
+  const int64_t thresh = gv.m / (alpha > 0 ? alpha : 1);
+  while (qn > 0) {
+    const uint32_t code = (uint32_t)d + 2u;  // level d+1
+    const int cur = d & 1;
+    const Queue qc = (d == 0) ? Queue{const_cast<int32_t*>(job.q0v), const_cast<int32_t*>(job.q0eo), nullptr}
+                              : Queue{tm.qv[d & 1], tm.qeo[d & 1], &ctl->ctr[d % 3]};
+    if (code > 255u) {
+      // Deeper than the 8-bit level code: clear this frontier and hand the
+      // tile to the plain path on the host.
+      nl_clear<T>(g, s, cur, qc.v, qn);
+      if (g.bid == 0 && threadIdx.x == 0) job.status = 1;
+      return team_barrier(ctl, g.nb, gerr);
+    }
+    const Queue qx{tm.qv[(d + 1) & 1], tm.qeo[(d + 1) & 1], &ctl->ctr[(d + 1) % 3]};
+    s.tcount = &ctl->tcount[d & 1];
+    if (g.bid == 0 && threadIdx.x == 0) {
+      ctl->ctr[(d + 2) % 3] = 0ull;   // the counter of level d+2's queue
+      ctl->tcount[(d + 1) & 1] = 0u;  // next level's touched count
+    }
+    if ((int64_t)qn + E > thresh) {
+      int32_t begin = 0;
+      if (gv.bucket_end[0] > begin)
+        nl_pull_bucket<T, 1, false>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.perm, begin,
+                                       gv.bucket_end[0], qx);
+      begin = gv.bucket_end[0];
+      if (gv.bucket_end[1] > begin)
+        nl_pull_bucket<T, (T * 4 <= kWave ? 4 : kWave / T), false>(g, s, cur, code, job.cnt, gv.in_off,
+                                                                      gv.in_col, gv.off, gv.perm, begin,
+                                                                      gv.bucket_end[1], qx);
+      begin = gv.bucket_end[1];
+      if (gv.bucket_end[2] > begin)
+        nl_pull_bucket<T, kWave / T, true>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.perm,
+                                              begin, gv.bucket_end[2], qx);
+      if (gv.n_hub_chunks > 0) nl_pull_hub<T>(g, s, cur, job.cnt, gv.in_col, gv.hubs, gv.n_hub_chunks);
+    } else {
+      nl_push<T>(g, s, cur, gv.off, gv.col, qc, qn, E);
+    }
+    if (!team_barrier(ctl, g.nb, gerr)) return false;
+    nl_commit<T>(g, s, cur, code, gv.off, qx);
+    nl_clear<T>(g, s, cur, qc.v, qn);
+    if (!team_barrier(ctl, g.nb, gerr)) return false;
+    const unsigned long long c =
+        __hip_atomic_load(&ctl->ctr[(d + 1) % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    qn = (int32_t)(uint32_t)(c & 0xFFFFFFFFull);
+    E = (int32_t)(uint32_t)(c >> 32);
+    if (qn > 0) ++d;
+  }
+  int K = 0;
+  while ((1 << K) <= d + 1) ++K;
+  bool ok = true;
+  switch (K) {
+    case 1: ok = emit_all_words<T, 1>(g, s, gv, job, ctl, gerr); break;
+    case 2: ok = emit_all_words<T, 2>(g, s, gv, job, ctl, gerr); break;
+    case 3: ok = emit_all_words<T, 3>(g, s, gv, job, ctl, gerr); break;
+    case 4: ok = emit_all_words<T, 4>(g, s, gv, job, ctl, gerr); break;
+    case 5: ok = emit_all_words<T, 5>(g, s, gv, job, ctl, gerr); break;
+    case 6: ok = emit_all_words<T, 6>(g, s, gv, job, ctl, gerr); break;
+    case 7: ok = emit_all_words<T, 7>(g, s, gv, job, ctl, gerr); break;
+    default: ok = emit_all_words<T, 8>(g, s, gv, job, ctl, gerr); break;
+  }
+  if (!ok) return false;
+  if (g.bid == 0 && threadIdx.x == 0) {
+    job.status = 0;
+    job.levels = d;
+  }
+  return team_barrier(ctl, g.nb, gerr);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock, 4) void k_bp_persistent(const Team* __restrict__ teams, int32_t n_teams,
+                                                          const GraphView* __restrict__ graphs,
+                                                          TileJob* __restrict__ jobs, int32_t alpha,
+                                                          unsigned* __restrict__ gerr) {
+  int ti = 0;
+  while (ti + 1 < n_teams && (int)blockIdx.x >= teams[ti + 1].first_block) ++ti;
+  const Team tm = teams[ti];
+  const Grid g{(int)blockIdx.x - tm.first_block, tm.nblocks};
+  if (g.bid < 0 || g.bid >= tm.nblocks) return;  // block outside every team (the grid is the sum of the teams)
+  for (int32_t j = tm.job_begin; j < tm.job_end; ++j) {
+    TileJob& job = jobs[j];
+    if (!run_tile<T>(g, tm, graphs[job.graph], job, alpha, gerr)) return;
+  }
+}
+
+// Zeroes the team control blocks, the error flag and the job statuses inside
+// the captured graph (a kernel rather than memset nodes).
+__global__ void k_reset_ctl(TeamCtl* ctl, int32_t n_teams, TileJob* jobs, int32_t n_jobs, unsigned* gerr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_teams) {
+    TeamCtl z{};
+    ctl[i] = z;
+  }
+  if (i < n_jobs) {
+    jobs[i].status = -1;
+    jobs[i].levels = -1;
+  }
+  if (i == 0) *gerr = 0u;
+}
+
+// ============================================================================
+// Utilities and test hooks
+// ============================================================================
 
 // Sum over (source, vertex) with level >= 0 of the vertex's out-degree.
 // blockIdx.y strides over sources, the x dimension over vertices.
@@ -818,11 +1128,29 @@ __global__ __launch_bounds__(kBlock) void k_count_traversed(const int32_t* __res
   if (lane_id() == 0) atomicAdd(out, acc);
 }
 
-// ---- test hooks: the compaction + edge-mapping device code on its own ----
+// Order-independent checksum of an int32 array: sum over i of
+// splitmix64(i * 0x9E3779B97F4A7C15 ^ (uint32)a[i]), modulo 2^64.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_checksum(const int32_t* __restrict__ a, int64_t count,
+                                                     unsigned long long* out) {
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    acc += mix64((uint64_t)i * 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)a[i]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+  if (lane_id() == 0) atomicAdd(out, acc);
+}
 
 // Appends (i, deg[i]) for i < k through the block-staged compaction.
 __global__ __launch_bounds__(kBlock) void k_test_build_queue(const int32_t* __restrict__ deg, int32_t k, Queue q) {
-  __shared__ AppendStage sg;
+  __shared__ QueueLds L;
+  AppendStage& sg = L.sg;
   stage_init(sg);
   for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < k; b0 += (int64_t)gridDim.x * kBlock) {
     const int64_t i = b0 + threadIdx.x;
@@ -837,15 +1165,14 @@ __global__ __launch_bounds__(kBlock) void k_test_build_queue(const int32_t* __re
 // owner staging the push kernels use.
 __global__ __launch_bounds__(kBlock) void k_test_edge_map(Queue q, int32_t qn, int32_t E, int32_t* __restrict__ owner,
                                                           int32_t* __restrict__ offset) {
-  __shared__ int32_t s_eo[kChunk + 1];
-  __shared__ int s_lohi[2];
+  __shared__ QueueLds L;
   for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
     const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
-    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, s_eo, s_lohi);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, L);
     for (int r = 0; r < kPushR; ++r) {
       const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
       if (e < c1) {
-        const int i = chunk_owner(co, q.eo, s_eo, e);
+        const int i = chunk_owner(co, q.eo, L.s_eo, e);
         owner[e] = q.v[i];
         offset[e] = e - q.eo[i];
       }
@@ -876,26 +1203,62 @@ void launch_plain_push(hipStream_t st, const int32_t* off, const int32_t* col, Q
                      parent, d, nxt);
 }
 
-#define XB_DISPATCH_T(T, CALL)                   \
-  switch (T) {                                   \
-    case 1: {                                    \
-      constexpr int TT = 1;                      \
-      CALL;                                      \
-    } break;                                     \
-    case 2: {                                    \
-      constexpr int TT = 2;                      \
-      CALL;                                      \
-    } break;                                     \
-    case 4: {                                    \
-      constexpr int TT = 4;                      \
-      CALL;                                      \
-    } break;                                     \
-    case 8: {                                    \
-      constexpr int TT = 8;                      \
-      CALL;                                      \
-    } break;                                     \
-    default:                                     \
-      break;                                     \
+#define XB_DISPATCH_T(T, CALL) \
+  switch (T) {                 \
+    case 1: {                  \
+      constexpr int TT = 1;    \
+      CALL;                    \
+    } break;                   \
+    case 2: {                  \
+      constexpr int TT = 2;    \
+      CALL;                    \
+    } break;                   \
+    case 4: {                  \
+      constexpr int TT = 4;    \
+      CALL;                    \
+    } break;                   \
+    case 8: {                  \
+      constexpr int TT = 8;    \
+      CALL;                    \
+    } break;                   \
+    default:                   \
+      break;                   \
+  }
+
+#define XB_DISPATCH_K(K, CALL) \
+  switch (K) {                 \
+    case 1: {                  \
+      constexpr int KK = 1;    \
+      CALL;                    \
+    } break;                   \
+    case 2: {                  \
+      constexpr int KK = 2;    \
+      CALL;                    \
+    } break;                   \
+    case 3: {                  \
+      constexpr int KK = 3;    \
+      CALL;                    \
+    } break;                   \
+    case 4: {                  \
+      constexpr int KK = 4;    \
+      CALL;                    \
+    } break;                   \
+    case 5: {                  \
+      constexpr int KK = 5;    \
+      CALL;                    \
+    } break;                   \
+    case 6: {                  \
+      constexpr int KK = 6;    \
+      CALL;                    \
+    } break;                   \
+    case 7: {                  \
+      constexpr int KK = 7;    \
+      CALL;                    \
+    } break;                   \
+    default: {                 \
+      constexpr int KK = 8;    \
+      CALL;                    \
+    } break;                   \
   }
 
 void launch_bp_seed(hipStream_t st, int T, BPState s, const int32_t* src, int32_t cnt) {
@@ -954,42 +1317,6 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
                                       st, s, cur, q.v, qn));
 }
 
-#define XB_DISPATCH_K(K, CALL)   \
-  switch (K) {                   \
-    case 1: {                    \
-      constexpr int KK = 1;      \
-      CALL;                      \
-    } break;                     \
-    case 2: {                    \
-      constexpr int KK = 2;      \
-      CALL;                      \
-    } break;                     \
-    case 3: {                    \
-      constexpr int KK = 3;      \
-      CALL;                      \
-    } break;                     \
-    case 4: {                    \
-      constexpr int KK = 4;      \
-      CALL;                      \
-    } break;                     \
-    case 5: {                    \
-      constexpr int KK = 5;      \
-      CALL;                      \
-    } break;                     \
-    case 6: {                    \
-      constexpr int KK = 6;      \
-      CALL;                      \
-    } break;                     \
-    case 7: {                    \
-      constexpr int KK = 7;      \
-      CALL;                      \
-    } break;                     \
-    default: {                   \
-      constexpr int KK = 8;      \
-      CALL;                      \
-    } break;                     \
-  }
-
 template <int T, int K>
 static void emit_heavy_launch(hipStream_t st, BPState s, int32_t word, const int32_t* in_off, const int32_t* in_col,
                               const int32_t* heavy, int32_t n_heavy) {
@@ -1018,11 +1345,30 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, i
                                                                lvl_out, par_out))));
 }
 
+int persistent_blocks_per_cu(int T) {
+  int nb = 0;
+  XB_DISPATCH_T(T, (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bp_persistent<TT>, kBlock, 0));
+  return nb;
+}
+
+void launch_bp_persistent(hipStream_t st, int T, int grid_blocks, const Team* teams, int32_t n_teams,
+                          const GraphView* graphs, TileJob* jobs, int32_t n_jobs, TeamCtl* ctl, int32_t alpha,
+                          unsigned* gerr) {
+  const int nr = n_teams > n_jobs ? n_teams : n_jobs;
+  hipLaunchKernelGGL(k_reset_ctl, dim3((nr + 255) / 256), dim3(256), 0, st, ctl, n_teams, jobs, n_jobs, gerr);
+  XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_persistent<TT>, dim3(grid_blocks), dim3(kBlock), 0, st, teams, n_teams,
+                                      graphs, jobs, alpha, gerr));
+}
+
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,
                             unsigned long long* out) {
   const int gx = grid_for(n, kBlock) < 64 ? grid_for(n, kBlock) : 64;
   const int gy = S < 256 ? S : 256;
   hipLaunchKernelGGL(k_count_traversed, dim3(gx, gy), dim3(kBlock), 0, st, off, levels, S, n, out);
+}
+
+void launch_checksum(hipStream_t st, const int32_t* a, int64_t count, unsigned long long* out) {
+  hipLaunchKernelGGL(k_checksum, dim3(grid_for(count, kBlock * 8)), dim3(kBlock), 0, st, a, count, out);
 }
 
 void launch_test_build_queue(hipStream_t st, const int32_t* deg, int32_t k, Queue q) {

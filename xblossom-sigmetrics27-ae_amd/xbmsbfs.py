@@ -58,7 +58,12 @@ EXPORTED_SYMBOLS = (
     "xb_graph_destroy",
     "xb_msbfs_run",
     "xb_count_traversed",
+    "xb_device_checksum",
     "xb_test_edge_map",
+    "xb_batch_create",
+    "xb_batch_run",
+    "xb_batch_destroy",
+    "xb_msbfs_run_batched",
     "xb_device_alloc",
     "xb_device_free",
     "xb_memcpy_d2h",
@@ -90,7 +95,7 @@ class Opts(ctypes.Structure):
 
 # Kernel classes of xb_run_stats (XB_K_* in include/xb/xb_capi.h).
 KERNEL_CLASSES = ("memset", "seed", "push", "commit", "pull", "pull_hub", "clear", "emit_heavy", "emit",
-                  "plain_init", "plain_push")
+                  "plain_init", "plain_push", "persistent")
 K_CLASSES = 16
 
 
@@ -155,7 +160,15 @@ def load_engine():
     lib.xb_graph_destroy.argtypes = [VP]
     lib.xb_msbfs_run.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts), ctypes.POINTER(RunStats)]
     lib.xb_count_traversed.argtypes = [VP, _P32, I32, ctypes.POINTER(I64)]
+    lib.xb_device_checksum.argtypes = [_P32, I64, ctypes.POINTER(ctypes.c_uint64)]
     lib.xb_test_edge_map.argtypes = [_P32, I32, _P32, _P32, ctypes.POINTER(I64)]
+    PP32 = ctypes.POINTER(_P32)
+    lib.xb_batch_create.argtypes = [ctypes.POINTER(VP), I32, PP32, _P32, PP32, PP32, ctypes.POINTER(Opts),
+                                    ctypes.POINTER(VP)]
+    lib.xb_batch_run.argtypes = [VP, ctypes.POINTER(RunStats)]
+    lib.xb_batch_destroy.argtypes = [VP]
+    lib.xb_msbfs_run_batched.argtypes = [ctypes.POINTER(VP), I32, PP32, _P32, PP32, PP32, ctypes.POINTER(Opts),
+                                         ctypes.POINTER(RunStats)]
     lib.xb_device_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(VP)]
     lib.xb_device_free.argtypes = [VP]
     lib.xb_memcpy_d2h.argtypes = [VP, VP, ctypes.c_size_t]
@@ -257,6 +270,124 @@ class XBlossomGpuEngine10:
         out = ctypes.c_int64()
         _check(lib.xb_count_traversed(self.graph.handle, _ptr(levels_dev), S, ctypes.byref(out)))
         return out.value
+
+
+class DeviceBuffer:
+    """Device int32 array from the library's own allocator (no PyTorch needed).
+
+    In a process that also uses PyTorch-ROCm, import torch before the first
+    call into this module: both then share torch's HIP runtime."""
+
+    def __init__(self, shape, fill=None):
+        lib = load_engine()
+        self.shape = tuple(int(x) for x in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.count = int(np.prod(self.shape)) if self.shape else 1
+        self._p = ctypes.c_void_p()
+        _check(lib.xb_device_alloc(max(self.count, 1) * 4, ctypes.byref(self._p)))
+        if fill is not None:
+            self.upload(np.full(self.count, fill, dtype=np.int32))
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.int32).reshape(-1)
+        assert a.size == self.count
+        _check(load_engine().xb_memcpy_h2d(self._p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def rows(self, r0: int, r1: int) -> np.ndarray:
+        """Copies rows [r0, r1) of a 2-D buffer to the host."""
+        n = self.shape[1]
+        out = np.empty((r1 - r0, n), dtype=np.int32)
+        src = ctypes.c_void_p(self._p.value + r0 * n * 4)
+        _check(load_engine().xb_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), src, out.nbytes))
+        return out
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=np.int32)
+        _check(load_engine().xb_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self._p, out.nbytes))
+        return out.reshape(self.shape)
+
+    def checksum(self) -> int:
+        out = ctypes.c_uint64()
+        _check(load_engine().xb_device_checksum(_ptr(self.ptr), self.count, ctypes.byref(out)))
+        return out.value
+
+    def free(self):
+        if self._p:
+            load_engine().xb_device_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr_array(items):
+    arr = (_P32 * len(items))()
+    for i, a in enumerate(items):
+        arr[i] = _ptr(a) if a is not None else None
+    return arr
+
+
+class Batch:
+    """A planned multi-graph batch: one persistent, hipGraph-captured launch.
+
+    graphs: list of HGraph; sources: list of int32 arrays; levels/parents: lists
+    of device addresses (ints) of S_i x n_i int32 tables (levels may be None with
+    want_levels=False)."""
+
+    def __init__(self, graphs, sources, levels, parents, want_levels=True, **tunables):
+        lib = load_engine()
+        self._keep = [np.ascontiguousarray(s, dtype=np.int32) for s in sources]
+        G = len(graphs)
+        self.graphs = graphs
+        gh = (ctypes.c_void_p * G)(*[g.handle.value for g in graphs])
+        S = np.array([s.size for s in self._keep], dtype=np.int32)
+        self._S = S
+        flags = XB_OUT_DEVICE | (0 if want_levels else XB_NO_LEVELS)
+        o = default_opts(flags=flags, **tunables)
+        self._h = ctypes.c_void_p()
+        lv = _ptr_array(levels) if want_levels else None
+        _check(lib.xb_batch_create(gh, G, _ptr_array(self._keep), _ptr(S), lv, _ptr_array(parents),
+                                   ctypes.byref(o), ctypes.byref(self._h)))
+        self.last_stats = RunStats()
+
+    def run(self) -> RunStats:
+        st = RunStats()
+        _check(load_engine().xb_batch_run(self._h, ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def close(self):
+        if self._h:
+            _check(load_engine().xb_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_batched(graphs, sources, **tunables):
+    """Host-output convenience: returns [(levels, parents)] per graph."""
+    lib = load_engine()
+    srcs = [np.ascontiguousarray(s, dtype=np.int32) for s in sources]
+    G = len(graphs)
+    gh = (ctypes.c_void_p * G)(*[g.handle.value for g in graphs])
+    S = np.array([s.size for s in srcs], dtype=np.int32)
+    lv = [np.empty((s.size, g.nnodes), dtype=np.int32) for s, g in zip(srcs, graphs)]
+    pa = [np.empty((s.size, g.nnodes), dtype=np.int32) for s, g in zip(srcs, graphs)]
+    o = default_opts(**tunables)
+    st = RunStats()
+    _check(lib.xb_msbfs_run_batched(gh, G, _ptr_array(srcs), _ptr(S), _ptr_array(lv), _ptr_array(pa),
+                                    ctypes.byref(o), ctypes.byref(st)))
+    return list(zip(lv, pa)), st
 
 
 def test_edge_map(degrees):
@@ -367,6 +498,23 @@ def workload_amazon_like():
     """Chung-Lu n=403k, gamma 2.3, mean degree 8.4, max ~2,750, seed 3; 256 uniform sources."""
     g = gen_chung_lu(403_000, 2.3, 8.4, 2_800, seed=3, overdraw=1.005)
     return g, sample_sources(g, 256, seed=3)
+
+
+def workload_uneven_batch(count: int = 32):
+    """32 R-MAT graphs: scale uniform in [12, 22] (log-uniform n), edge factor uniform in
+    [8, 32] (seed 5); each gets 1-512 sources, log-uniform (seed 6), uniform over
+    non-isolated vertices."""
+    rg = np.random.default_rng(5)
+    rs = np.random.default_rng(6)
+    out = []
+    for i in range(count):
+        scale = int(rg.integers(12, 23))
+        ef = int(rg.integers(8, 33))
+        S = int(np.floor(np.exp(rs.uniform(0.0, np.log(513.0)))))
+        S = max(1, min(512, S))
+        g = gen_rmat(scale, ef, 0.57, 0.19, 0.19, seed=1000 + i)
+        out.append((g, sample_sources(g, S, seed=2000 + i)))
+    return out
 
 
 def workload_livejournal_like():
