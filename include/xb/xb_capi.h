@@ -99,7 +99,11 @@ typedef struct xb_msbfs_opts {
 /* Per-run statistics (filled when the pointer is non-null).  kernel_ms and
  * kernel_launches are filled only with XB_PROFILE; kernel_bytes always holds
  * the algorithmic bytes of each class (the bytes a launch must move at
- * minimum: CSR and state streams plus output, without redundant re-reads). */
+ * minimum: CSR and state streams plus output, without redundant re-reads).
+ * For a batch (xb_batch_run) kernel_ms[XB_K_PERSISTENT] is the launch's
+ * HIP-event time; with XB_PROFILE the other classes hold the phase clocks of
+ * the persistent kernel (each team's time between barriers, weighted by its
+ * share of the grid, so they add up to about the launch time). */
 typedef struct xb_run_stats {
   double ms_total;                     /* HIP-event time of the whole run    */
   double kernel_ms[XB_K_CLASSES];      /* summed HIP-event time per class    */

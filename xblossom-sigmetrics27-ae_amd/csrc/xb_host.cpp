@@ -1062,6 +1062,7 @@ struct xb_batch {
   std::vector<std::vector<int32_t>> job_sources;  // host copy of each tile's sources (plain re-runs)
   int32_t n_teams = 0;
   int grid = 0;
+  bool profile = false;  // XB_PROFILE: phase clocks of the persistent kernel go into the stats
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -1117,6 +1118,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   b->cut_medium = bo.cut_medium;
   b->cut_hub = bo.cut_hub;
   b->hub_chunk = bo.hub_chunk;
+  b->profile = (o.flags & XB_PROFILE) != 0;
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   const int T = b->T;
   const int32_t tile = 64 * T;
@@ -1342,6 +1344,21 @@ void run_batch(xb_batch* b, xb_run_stats* st) {
     // Section 8(d) bytes of the tile: L CSR streams, L state streams, outputs.
     const double L = j.levels + 1.0, W = (j.cnt + 63) / 64, n = g->n, m = (double)g->m;
     bytes += L * (4.0 * (n + 1) + 4.0 * m) + L * 24.0 * W * n + (j.lvl_out ? 8.0 : 4.0) * j.cnt * n;
+  }
+  if (st && b->profile) {
+    // Phase clocks: each team's critical-path time per phase, weighted by the
+    // team's share of the grid, so the classes add up to about the wall time.
+    std::vector<xb::TeamCtl> ctl((size_t)b->n_teams);
+    d2h(ctl.data(), b->d_ctl, sizeof(xb::TeamCtl) * ctl.size(), b->stream);
+    int dev = 0, khz = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz > 0)
+      for (int32_t t = 0; t < b->n_teams; ++t) {
+        const double share = (double)b->h_teams[t].nblocks / b->grid;
+        for (int c = 0; c < XB_K_CLASSES; ++c)
+          if (c != XB_K_PERSISTENT) st->kernel_ms[c] += (double)ctl[t].tphase[c] / khz * share;
+      }
   }
   if (st) {
     st->ms_total = ms;

@@ -74,10 +74,14 @@ struct TileJob {
 
 // Control block of one team (zeroed by a kernel at the start of every launch).
 struct TeamCtl {
-  unsigned bar_count;
-  unsigned bar_gen;
+  unsigned long long bar_count;  // monotonic arrivals: barrier k of the team opens at (k+1) * nblocks
   unsigned long long ctr[3];  // packed (edges << 32 | count) queue counters, level d uses ctr[d % 3]
   uint32_t tcount[2];         // touched-list counts, level d uses tcount[d & 1]
+  // Phase clocks (wall_clock64 ticks) kept by workgroup 0 of the team: the
+  // span between two team barriers is charged to the phase that ended at the
+  // second one, indexed by the XB_K_* class of that phase.
+  unsigned long long tlast;
+  unsigned long long tphase[16];
 };
 
 // A team: a contiguous range of workgroups running a contiguous range of jobs

@@ -43,6 +43,7 @@
 #include <cstdint>
 
 #include "xb_internal.hpp"
+#include "../../include/xb/xb_capi.h"
 
 namespace xb {
 namespace {
@@ -54,7 +55,7 @@ constexpr int kPushR = 4;                // edges per thread per chunk
 constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 constexpr int kPullU = 4;                // independent neighbour loads per lane per step
 constexpr int kEmitU = 4;
-constexpr int kStageCap = 2048;
+constexpr int kStageCap = 1536;
 
 // ---------------------------------------------------------------------------
 // Team geometry
@@ -120,7 +121,8 @@ __device__ __forceinline__ int owner_in(const int32_t* __restrict__ eo, int lo, 
 }
 
 // ---------------------------------------------------------------------------
-// LDS: one union per block, reused phase after phase
+// LDS: one union per block, reused phase after phase.  Both views stay under
+// 20 KB so that 8 workgroups of 256 threads fit in a CU's 160 KB.
 // ---------------------------------------------------------------------------
 struct AppendStage {
   int32_t v[kStageCap];
@@ -138,7 +140,6 @@ struct QueueLds {
 
 struct EmitLds {
   int32_t tile[64][65];
-  uint64_t pl[64][8];
 };
 
 union Lds {
@@ -148,6 +149,7 @@ union Lds {
 
 // One LDS union per workgroup, shared by every phase a kernel runs.
 __shared__ Lds g_lds;
+static_assert(sizeof(Lds) <= 20 * 1024 - 256, "LDS union must leave room for 8 workgroups per CU");
 
 // ---- block-level LDS staging of queue appends ----
 // Lanes stage (vertex, degree) entries in LDS (one LDS atomic per wave); the
@@ -738,10 +740,6 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
       b = in_off[v];
       e = in_off[v + 1];
     }
-    if (p == 0) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) L.pl[vi][k] = pk[k];
-    }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
     uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
@@ -794,9 +792,12 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     // ---- write phase ----
     const int32_t v2 = (int32_t)(vb + l);
     if (v2 < n) {
+      // The vertex's plane words, read again (the compute phase just brought
+      // them into the cache) rather than kept in LDS.
+      const uint64_t* pq = &s.planes[((int64_t)v2 * T + word) * 8];
       uint64_t q[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) q[k] = L.pl[l][k];
+      for (int k = 0; k < K; ++k) q[k] = pq[k];
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int bit = 16 * wv + i;
@@ -884,42 +885,60 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
 // level loop, team barriers between phases
 // ============================================================================
 
-// Team barrier.  Every thread fences its writes (agent scope, so they reach
-// the other XCDs' view), one thread per block arrives on the team counter and
-// the last arriver bumps the generation.  Waiting is bounded: after
-// kSpinLimit polls (about 4 s) the team sets the batch error flag and every block returns
-// at its next barrier, so the kernel always terminates.
+// Team barrier over the nblocks workgroups of a team (one monotonic arrival
+// counter per team; barrier k opens once (k+1) * nblocks arrivals are in).
+//
+// Visibility across XCDs (each XCD has its own L2, not coherent with the
+// others for ordinary device memory):
+//  * every wave first waits for its own stores (s_waitcnt 0), so after the
+//    workgroup barrier all of the workgroup's stores have reached its L2;
+//  * one thread then releases at agent scope (L2 write-back) and arrives;
+//  * it polls the counter with relaxed agent-scope loads, which read the
+//    coherent copy without touching the caches -- an acquire per poll would
+//    invalidate the XCD's L2 under every workgroup still working;
+//  * after the wait, one acquire at agent scope drops stale lines before the
+//    workgroup reads what other XCDs wrote.
+// Waiting is bounded: after kSpinLimit polls (about 4 s) the team sets the
+// batch error flag and every workgroup returns at its next barrier, so the
+// kernel always terminates.  Workgroup 0 of the team charges the time since
+// the previous barrier to `phase` (an XB_K_* class); the barrier opens only
+// when the slowest workgroup arrives, so the charge is the phase's critical
+// path on the team.
 constexpr long kSpinLimit = 1L << 22;
 
-__device__ bool team_barrier(TeamCtl* ctl, int nblocks, unsigned* gerr) {
-  __threadfence();
+__device__ bool team_barrier(TeamCtl* ctl, Grid g, unsigned* gerr, int phase) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   __shared__ int s_ok;
   if (threadIdx.x == 0) {
     int ok = 1;
-    if (nblocks > 1) {
-      const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned arrived = atomicAdd(&ctl->bar_count, 1u);
-      if (arrived == (unsigned)nblocks - 1u) {
-        __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        long spins = 0;
-        while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > kSpinLimit) {
-            atomicOr(gerr, 1u);
-            break;
-          }
-          if ((spins & 1023) == 0 && __hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    if (g.nb > 1) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_s_waitcnt(0);  // the write-back completes before the arrival is seen
+      const unsigned long long nb = (unsigned long long)g.nb;
+      const unsigned long long old =
+          __hip_atomic_fetch_add(&ctl->bar_count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long target = (old / nb + 1ull) * nb;
+      long spins = 0;
+      while (__hip_atomic_load(&ctl->bar_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kSpinLimit) {
+          atomicOr(gerr, 1u);
+          break;
         }
+        if ((spins & 1023) == 0 && __hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     if (__hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
     s_ok = ok;
+    if (g.bid == 0) {
+      const unsigned long long now = (unsigned long long)wall_clock64();
+      ctl->tphase[phase] += now - ctl->tlast;
+      ctl->tlast = now;
+    }
   }
   __syncthreads();
-  __threadfence();
   return s_ok != 0;
 }
 
@@ -978,7 +997,7 @@ __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const Til
   const int words = (job.cnt + 63) / 64;
   if (gv.n_heavy > 0)
     for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
-  if (!team_barrier(ctl, g.nb, gerr)) return false;
+  if (!team_barrier(ctl, g, gerr, XB_K_EMIT_HEAVY)) return false;
   for (int t = 0; t < words; ++t)
     nl_emit_light<T, K>(g, s, t, job.cnt, job.src, gv.in_off, gv.in_col, gv.n, gv.cut_medium, job.lvl_out,
                         job.par_out);
@@ -1004,9 +1023,9 @@ __device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& j
     ctl->ctr[1] = 0ull;
     ctl->tcount[0] = 0u;
   }
-  if (!team_barrier(ctl, g.nb, gerr)) return false;
+  if (!team_barrier(ctl, g, gerr, XB_K_MEMSET)) return false;
   nl_seed<T>(g, s, job.src, job.cnt);
-  if (!team_barrier(ctl, g.nb, gerr)) return false;
+  if (!team_barrier(ctl, g, gerr, XB_K_SEED)) return false;
 
   int32_t qn = job.qn0, E = job.E0, d = 0;
   const int64_t thresh = gv.m / (alpha > 0 ? alpha : 1);
@@ -1020,7 +1039,7 @@ __device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& j
       // tile to the plain path on the host.
       nl_clear<T>(g, s, cur, qc.v, qn);
       if (g.bid == 0 && threadIdx.x == 0) job.status = 1;
-      return team_barrier(ctl, g.nb, gerr);
+      return team_barrier(ctl, g, gerr, XB_K_CLEAR);
     }
     const Queue qx{tm.qv[(d + 1) & 1], tm.qeo[(d + 1) & 1], &ctl->ctr[(d + 1) % 3]};
     s.tcount = &ctl->tcount[d & 1];
@@ -1028,7 +1047,8 @@ __device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& j
       ctl->ctr[(d + 2) % 3] = 0ull;   // the counter of level d+2's queue
       ctl->tcount[(d + 1) & 1] = 0u;  // next level's touched count
     }
-    if ((int64_t)qn + E > thresh) {
+    const bool pull = (int64_t)qn + E > thresh;
+    if (pull) {
       int32_t begin = 0;
       if (gv.bucket_end[0] > begin)
         nl_pull_bucket<T, 1, false>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.perm, begin,
@@ -1046,10 +1066,10 @@ __device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& j
     } else {
       nl_push<T>(g, s, cur, gv.off, gv.col, qc, qn, E);
     }
-    if (!team_barrier(ctl, g.nb, gerr)) return false;
+    if (!team_barrier(ctl, g, gerr, pull ? XB_K_PULL : XB_K_PUSH)) return false;
     nl_commit<T>(g, s, cur, code, gv.off, qx);
     nl_clear<T>(g, s, cur, qc.v, qn);
-    if (!team_barrier(ctl, g.nb, gerr)) return false;
+    if (!team_barrier(ctl, g, gerr, XB_K_COMMIT)) return false;
     const unsigned long long c =
         __hip_atomic_load(&ctl->ctr[(d + 1) % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     qn = (int32_t)(uint32_t)(c & 0xFFFFFFFFull);
@@ -1074,11 +1094,20 @@ __device__ bool run_tile(Grid g, const Team& tm, const GraphView& gv, TileJob& j
     job.status = 0;
     job.levels = d;
   }
-  return team_barrier(ctl, g.nb, gerr);
+  return team_barrier(ctl, g, gerr, XB_K_EMIT);
 }
 
+// Workgroups of the persistent kernel per CU the register budget is sized for
+// (8 x 256 threads = 32 waves per CU, 64 VGPRs per lane; the LDS union is
+// sized to allow it).  Measured on the uneven 32-graph batch: 90 ms at 8 per
+// CU against 117 ms at 4 per CU (128 VGPRs), whose extra registers do not pay
+// for the halved latency hiding of the gathers.
+#ifndef XB_PERSISTENT_MIN_BLOCKS
+#define XB_PERSISTENT_MIN_BLOCKS 8
+#endif
+
 template <int T>
-__global__ __launch_bounds__(kBlock, 4) void k_bp_persistent(const Team* __restrict__ teams, int32_t n_teams,
+__global__ __launch_bounds__(kBlock, XB_PERSISTENT_MIN_BLOCKS) void k_bp_persistent(const Team* __restrict__ teams, int32_t n_teams,
                                                           const GraphView* __restrict__ graphs,
                                                           TileJob* __restrict__ jobs, int32_t alpha,
                                                           unsigned* __restrict__ gerr) {
@@ -1087,6 +1116,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_bp_persistent(const Team* __restr
   const Team tm = teams[ti];
   const Grid g{(int)blockIdx.x - tm.first_block, tm.nblocks};
   if (g.bid < 0 || g.bid >= tm.nblocks) return;  // block outside every team (the grid is the sum of the teams)
+  if (g.bid == 0 && threadIdx.x == 0) tm.ctl->tlast = (unsigned long long)wall_clock64();
   for (int32_t j = tm.job_begin; j < tm.job_end; ++j) {
     TileJob& job = jobs[j];
     if (!run_tile<T>(g, tm, graphs[job.graph], job, alpha, gerr)) return;

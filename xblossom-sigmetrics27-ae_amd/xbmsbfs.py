@@ -340,7 +340,7 @@ class Batch:
     of device addresses (ints) of S_i x n_i int32 tables (levels may be None with
     want_levels=False)."""
 
-    def __init__(self, graphs, sources, levels, parents, want_levels=True, **tunables):
+    def __init__(self, graphs, sources, levels, parents, want_levels=True, profile=False, **tunables):
         lib = load_engine()
         self._keep = [np.ascontiguousarray(s, dtype=np.int32) for s in sources]
         G = len(graphs)
@@ -348,7 +348,7 @@ class Batch:
         gh = (ctypes.c_void_p * G)(*[g.handle.value for g in graphs])
         S = np.array([s.size for s in self._keep], dtype=np.int32)
         self._S = S
-        flags = XB_OUT_DEVICE | (0 if want_levels else XB_NO_LEVELS)
+        flags = XB_OUT_DEVICE | (0 if want_levels else XB_NO_LEVELS) | (XB_PROFILE if profile else 0)
         o = default_opts(flags=flags, **tunables)
         self._h = ctypes.c_void_p()
         lv = _ptr_array(levels) if want_levels else None
