@@ -444,9 +444,11 @@ void ensure_queues(xb_graph* g) {
   w.tcount = dalloc<uint32_t>(1);
 }
 
+// Bit-parallel state for tiles of up to T words per vertex (a tile with fewer
+// words uses the leading n x words entries of each array).
 void ensure_bp(xb_graph* g, int T) {
   Workspace& w = g->ws;
-  if (w.T == T) return;
+  if (w.T >= T) return;
   w.free_bp();
   const size_t nw = (size_t)g->n * (size_t)T + 1;
   w.bp.seen = dalloc<uint64_t>(nw);
@@ -644,11 +646,13 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   xb::BPState& s = w.bp;
   prof.begin(XB_K_MEMSET);
   if (w.bp_dirty) {
-    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * nw, g->stream));
-    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * nw, g->stream));
-    HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * nw, g->stream));
+    // Whole capacity: later tiles may use more words than this one.
+    const size_t cap = (size_t)n * (size_t)w.T;
+    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * cap, g->stream));
+    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * cap, g->stream));
+    HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * cap, g->stream));
     HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), g->stream));
-    prof.add(XB_K_MEMSET, 24.0 * nw + 4.0 * n);
+    prof.add(XB_K_MEMSET, 24.0 * cap + 4.0 * n);
     w.bp_dirty = false;
   }
   HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, g->stream));
@@ -768,14 +772,24 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   return true;
 }
 
+// Words per vertex for a tile of cnt sources: the smallest power of two with
+// 64 * words >= cnt, at most Tmax.  State traffic scales with the words, so a
+// short tile does not pay for the empty ones.
+int tile_words(int32_t cnt, int Tmax) {
+  int t = 1;
+  while (t * 64 < cnt && t < Tmax) t *= 2;
+  return t;
+}
+
 void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl, int32_t* par,
                  bool dev_out, Profiler& prof, xb_run_stats* st) {
-  ensure_queues(g);
-  ensure_bp(g, o.T);
-  ensure_layout(g, o);
-  ensure_hpar(g, o.T);
-  const int32_t n = g->n;
   const int32_t tile = 64 * o.T;
+  const int Tneed = tile_words(std::min(S, tile), o.T);  // the widest tile of this run
+  ensure_queues(g);
+  ensure_bp(g, Tneed);
+  ensure_layout(g, o);
+  ensure_hpar(g, Tneed);
+  const int32_t n = g->n;
   ensure_src(g, tile);
   int32_t* stage = nullptr;
   if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)tile * n);
@@ -783,7 +797,9 @@ void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S
     const int32_t cnt = std::min(tile, S - base);
     int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : (lvl ? stage : nullptr);
     int32_t* pr = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
-    bool ok = run_bp_tile(g, o, sources + base, cnt, lr, pr, prof, st);
+    BPOpts ot = o;
+    ot.T = tile_words(cnt, o.T);
+    bool ok = run_bp_tile(g, ot, sources + base, cnt, lr, pr, prof, st);
     if (!ok) {
       // Deeper than 254 levels: these sources take the plain path.
       if (st) st->fallback_sources += cnt;
@@ -1059,6 +1075,7 @@ struct xb_batch {
   unsigned* d_err = nullptr;
   std::vector<xb::TileJob> h_jobs;
   std::vector<xb::Team> h_teams;
+  std::vector<int> team_words;  // words per vertex the team's workspace holds
   std::vector<std::vector<int32_t>> job_sources;  // host copy of each tile's sources (plain re-runs)
   int32_t n_teams = 0;
   int grid = 0;
@@ -1089,7 +1106,7 @@ void batch_clear_state(xb_batch* b) {
   for (size_t i = 0; i < b->h_teams.size(); ++i) {
     const xb::Team& tm = b->h_teams[i];
     const xb_graph* g = b->graphs[b->h_jobs[tm.job_begin].graph];
-    const size_t nw = (size_t)g->n * b->T + 1;
+    const size_t nw = (size_t)g->n * b->team_words[i] + 1;
     HIPCHK(hipMemsetAsync(tm.s.front[0], 0, sizeof(uint64_t) * nw, b->stream));
     HIPCHK(hipMemsetAsync(tm.s.front[1], 0, sizeof(uint64_t) * nw, b->stream));
     HIPCHK(hipMemsetAsync(tm.s.nextw, 0, sizeof(uint64_t) * nw, b->stream));
@@ -1150,7 +1167,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   // Tiles: sources and first frontiers, packed into one device buffer.
   std::vector<int32_t> packed;
   struct TileHost {
-    int32_t graph, base, cnt;
+    int32_t graph, base, cnt, words;
     size_t src_off, q0_off;
     int32_t qn0, E0;
   };
@@ -1158,7 +1175,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   for (int32_t i = 0; i < G; ++i) {
     for (int32_t base = 0; base < S[i]; base += tile) {
       const int32_t cnt = std::min(tile, S[i] - base);
-      TileHost th{i, base, cnt, packed.size(), 0, 0, 0};
+      TileHost th{i, base, cnt, tile_words(cnt, T), packed.size(), 0, 0, 0};
       packed.insert(packed.end(), sources[i] + base, sources[i] + base + cnt);
       std::vector<int32_t> uniq(sources[i] + base, sources[i] + base + cnt);
       std::sort(uniq.begin(), uniq.end());
@@ -1182,51 +1199,75 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   if (!packed.empty())
     h2d(d_packed, packed.data(), sizeof(int32_t) * packed.size(), b->stream);
 
-  // Teams: one per graph with tiles, workgroups in proportion to estimated cost.
+  // Teams: one per graph with tiles.  A team's wall time is its cost over its
+  // workgroups, so workgroups go in proportion to an estimate of the bytes
+  // its tiles move (the launch ends with the slowest team):
+  //   ~3 pull levels  x (CSR + one frontier word per in-edge and word),
+  //   ~8 levels       x 24 B of state per vertex and word,
+  //   the output rows, and the emit's plane and in-edge reads.
   hipDeviceProp_t prop;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipGetDeviceProperties(&prop, dev));
-  const int per_cu = std::max(1, xb::persistent_blocks_per_cu(T));
+  const int per_cu = std::max(1, xb::persistent_blocks_per_cu());
   const int budget = per_cu * prop.multiProcessorCount;
+  const double out_b = no_levels ? 4.0 : 8.0;
   std::vector<int32_t> team_graph;
   std::vector<double> cost;
-  for (int32_t i = 0; i < G; ++i)
-    if (S[i] > 0) {
-      team_graph.push_back(i);
-      const double tiles_i = (double)((S[i] + tile - 1) / tile);
-      cost.push_back(tiles_i * ((double)graphs[i]->m + 8.0 * graphs[i]->n));
-    }
+  for (int32_t i = 0; i < G; ++i) {
+    if (S[i] == 0) continue;
+    const double n = graphs[i]->n, m = (double)graphs[i]->m;
+    double c = 0;
+    for (const TileHost& th : tiles)
+      if (th.graph == i) {
+        const double w = th.words;
+        c += 3.0 * (4.0 * m + 8.0 * w * m) + 8.0 * 24.0 * w * n + out_b * th.cnt * n + w * (64.0 * n + 16.0 * m);
+      }
+    team_graph.push_back(i);
+    cost.push_back(c);
+  }
   const int nt = (int)team_graph.size();
   if (nt == 0) fail(XB_ERR_INVALID_ARG, "batch has no sources");
   if (nt > budget) fail(XB_ERR_UNSUPPORTED, "more graphs in one batch than co-resident workgroups");
-  double total_cost = 0;
-  for (double c : cost) total_cost += c;
+  // Workgroups a team can keep busy: one per 4096 vertices or edges.
+  auto useful = [&](int t) {
+    const xb_graph* g = graphs[team_graph[t]];
+    return (int)std::max<int64_t>(1, (g->m + (int64_t)g->n) / 4096);
+  };
+  // A floor of min(useful, 8) keeps small graphs off a single workgroup's
+  // latency; the rest is shared in proportion to cost.
   std::vector<int> blocks((size_t)nt);
   int used = 0;
   for (int t = 0; t < nt; ++t) {
-    const xb_graph* g = graphs[team_graph[t]];
-    const int useful = (int)std::max<int64_t>(1, (g->m + (int64_t)g->n) / 4096);
-    blocks[t] = std::max(1, std::min(useful, (int)(budget * cost[t] / total_cost)));
+    blocks[t] = std::min(useful(t), std::max(1, std::min(8, budget / nt)));
     used += blocks[t];
   }
-  while (used > budget) {
-    int big = (int)(std::max_element(blocks.begin(), blocks.end()) - blocks.begin());
-    --blocks[big];
-    --used;
-  }
-  // Hand leftover workgroups to the costliest teams that can use them.
-  for (bool grew = true; grew && used < budget;) {
-    grew = false;
-    for (int t = 0; t < nt && used < budget; ++t) {
-      const xb_graph* g = graphs[team_graph[t]];
-      const int useful = (int)std::max<int64_t>(1, (g->m + (int64_t)g->n) / 4096);
-      if (blocks[t] < useful && blocks[t] < (int)(budget * cost[t] / total_cost) + 1) {
-        ++blocks[t];
-        ++used;
-        grew = true;
-      }
+  for (int round = 0; round < 4 && used < budget; ++round) {
+    double open_cost = 0;
+    for (int t = 0; t < nt; ++t)
+      if (blocks[t] < useful(t)) open_cost += cost[t];
+    if (open_cost <= 0) break;
+    const int spare = budget - used;
+    for (int t = 0; t < nt; ++t) {
+      if (blocks[t] >= useful(t)) continue;
+      const int want = (int)(spare * cost[t] / open_cost);
+      const int add = std::min(want, useful(t) - blocks[t]);
+      blocks[t] += add;
+      used += add;
     }
+  }
+  // Rounding leftovers: one at a time to the team with the highest cost per workgroup.
+  while (used < budget) {
+    int best = -1;
+    double best_load = -1;
+    for (int t = 0; t < nt; ++t)
+      if (blocks[t] < useful(t) && cost[t] / blocks[t] > best_load) {
+        best_load = cost[t] / blocks[t];
+        best = t;
+      }
+    if (best < 0) break;
+    ++blocks[best];
+    ++used;
   }
 
   b->d_ctl = b->alloc<xb::TeamCtl>((size_t)nt);
@@ -1240,6 +1281,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     tm.nblocks = blocks[t];
     first += blocks[t];
     tm.job_begin = (int32_t)b->h_jobs.size();
+    int tw = 1;  // widest tile of the team
     for (const TileHost& th : tiles) {
       if (th.graph != gi) continue;
       xb::TileJob j{};
@@ -1254,10 +1296,13 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
       j.par_out = parents_dev[gi] + (int64_t)th.base * g->n;
       j.status = -1;
       j.levels = -1;
+      j.words = th.words;
+      j.pad_ = 0;
+      tw = std::max(tw, th.words);
       b->h_jobs.push_back(j);
     }
     tm.job_end = (int32_t)b->h_jobs.size();
-    const size_t nw = (size_t)g->n * T + 1;
+    const size_t nw = (size_t)g->n * tw + 1;
     tm.s.seen = b->alloc<uint64_t>(nw);
     tm.s.front[0] = b->alloc<uint64_t>(nw);
     tm.s.front[1] = b->alloc<uint64_t>(nw);
@@ -1265,7 +1310,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     tm.s.planes = b->alloc<uint64_t>(nw * 8);
     tm.s.tflag = b->alloc<uint32_t>((size_t)g->n + 1);
     tm.s.tlist = b->alloc<int32_t>((size_t)g->n + 1);
-    tm.s.hpar = b->alloc<uint32_t>((size_t)std::max(g->n_heavy, 1) * T * 64);
+    tm.s.hpar = b->alloc<uint32_t>((size_t)std::max(g->n_heavy, 1) * tw * 64);
     tm.s.tcount = nullptr;  // set per level from the control block
     for (int k = 0; k < 2; ++k) {
       tm.qv[k] = b->alloc<int32_t>((size_t)g->n + 1);
@@ -1273,6 +1318,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     }
     tm.ctl = b->d_ctl + t;
     b->h_teams.push_back(tm);
+    b->team_words.push_back(tw);
   }
   // Job order follows the teams; remap the host-side source copies to match.
   {
@@ -1294,7 +1340,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
 
   // Capture the launch (control reset + persistent kernel) once.
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  xb::launch_bp_persistent(b->stream, T, b->grid, b->d_teams, b->n_teams, b->d_views, b->d_jobs,
+  xb::launch_bp_persistent(b->stream, b->grid, b->d_teams, b->n_teams, b->d_views, b->d_jobs,
                            (int32_t)b->h_jobs.size(), b->d_ctl, b->alpha, b->d_err);
   HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
   HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));

@@ -70,11 +70,26 @@ struct TileJob {
   int32_t* par_out;
   int32_t status;          // out: 0 done, 1 deeper than the level code (re-run on the plain path)
   int32_t levels;          // out: deepest level reached
+  int32_t words;           // 64-bit words per vertex of this tile (1, 2, 4 or 8; 64 * words >= cnt)
+  int32_t pad_;
+};
+
+// Team barrier counters live in 256-byte slots of their own: arrivals from
+// thousands of workgroups serialize per address, so every XCD gets its own
+// arrival counter and only the last arriver of each XCD touches the team's.
+constexpr int kMaxXcd = 16;
+struct alignas(256) XcdSlot {
+  unsigned long long arrivals;  // monotonic: the team's workgroups on this XCD arrive once per barrier
+  unsigned long long members;   // the team's workgroups on this XCD (counted at launch)
+  unsigned long long pad_[30];
 };
 
 // Control block of one team (zeroed by a kernel at the start of every launch).
-struct TeamCtl {
-  unsigned long long bar_count;  // monotonic arrivals: barrier k of the team opens at (k+1) * nblocks
+struct alignas(256) TeamCtl {
+  unsigned long long done;       // XCD arrivals: barrier k of the team opens at (k+1) * (XCDs in the team)
+  unsigned long long joined;     // workgroups registered at launch
+  unsigned long long pad0_[30];
+  XcdSlot xcd[kMaxXcd];
   unsigned long long ctr[3];  // packed (edges << 32 | count) queue counters, level d uses ctr[d % 3]
   uint32_t tcount[2];         // touched-list counts, level d uses tcount[d & 1]
   // Phase clocks (wall_clock64 ticks) kept by workgroup 0 of the team: the
@@ -83,6 +98,7 @@ struct TeamCtl {
   unsigned long long tlast;
   unsigned long long tphase[16];
 };
+static_assert(sizeof(TeamCtl) % 8 == 0, "TeamCtl is cleared as 64-bit words");
 
 // A team: a contiguous range of workgroups running a contiguous range of jobs
 // (the tiles of one graph) with its own workspace.
@@ -95,8 +111,8 @@ struct Team {
   TeamCtl* ctl;
 };
 
-int persistent_blocks_per_cu(int T);
-void launch_bp_persistent(hipStream_t st, int T, int grid_blocks, const Team* teams, int32_t n_teams,
+int persistent_blocks_per_cu();
+void launch_bp_persistent(hipStream_t st, int grid_blocks, const Team* teams, int32_t n_teams,
                           const GraphView* graphs, TileJob* jobs, int32_t n_jobs, TeamCtl* ctl, int32_t alpha,
                           unsigned* gerr);
 
