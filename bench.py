@@ -192,12 +192,129 @@ def bench_uneven(args) -> int:  # noqa: C901
     return 0
 
 
+def bench_serving(args) -> int:
+    """Serving: directed web-crawl-shaped R-MAT (n = 20M, ~300M edges, a=.6
+    b=.15 c=.15, seed 7) resident on the device; queries arrive by a Poisson
+    process (rate --rate, seed 7), each with 1-256 sources drawn log-uniformly
+    (seed 8) among vertices with out-edges.  Latency runs from a query's
+    scheduled arrival to the end of its last tile; aggregate GTEPS is the
+    traversed edges of all queries over the span from the first arrival to
+    the last completion."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    import xbmsbfs as xb
+
+    t0 = time.perf_counter()
+    g = xb.gen_rmat_capped(25, 20_000_000, 386_000_000, 0.6, 0.15, 0.15, seed=7)
+    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
+    hg = xb.HGraph(g.row_offsets, g.col_indices, directed=True)
+    rng_a = np.random.default_rng(7)
+    rng_s = np.random.default_rng(8)
+    cand = np.flatnonzero(g.degrees() > 0).astype(np.int32)
+    N = args.requests
+    arrivals = np.cumsum(rng_a.exponential(1.0 / args.rate, size=N))
+    reqs = []
+    for _ in range(N + args.warmup):
+        S = max(1, min(256, int(np.floor(np.exp(rng_s.uniform(0.0, np.log(257.0)))))))
+        reqs.append(rng_s.choice(cand, size=S, replace=True).astype(np.int32))
+    warm, reqs = reqs[: args.warmup], reqs[args.warmup:]
+    srv = xb.Server(hg, slots=args.slots)
+    for s_ in warm:
+        rid = srv.submit(s_)
+        srv.wait(rid)
+        srv.release(rid)
+    base = srv.counters()
+    infos = [None] * N
+    todo = queue.Queue()
+
+    def collect():
+        while True:
+            item = todo.get()
+            if item is None:
+                return
+            i, rid = item
+            inf = srv.wait(rid)
+            infos[i] = (inf.t_submit_us, inf.t_admit_us, inf.t_start_us, inf.t_done_us, inf.edges, inf.S, inf.tiles)
+            srv.release(rid)
+
+    workers = [threading.Thread(target=collect) for _ in range(16)]
+    for w in workers:
+        w.start()
+    sub_us = np.zeros(N)
+    start = time.perf_counter()
+    # Server clock of the first submit anchors the arrival schedule.
+    for i in range(N):
+        delay = start + arrivals[i] - arrivals[0] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        rid = srv.submit(reqs[i])
+        todo.put((i, rid))
+    for _ in workers:
+        todo.put(None)
+    for w in workers:
+        w.join()
+    c = srv.counters()
+    srv.close()
+    t_sub = np.array([x[0] for x in infos])
+    sched = t_sub[0] + (arrivals - arrivals[0]) * 1e6  # scheduled arrivals on the server clock
+    done = np.array([x[3] for x in infos])
+    start_t = np.array([x[2] for x in infos])
+    admit = np.array([x[1] for x in infos])
+    lat_ms = (done - np.minimum(sched, t_sub)) / 1e3
+    svc_ms = (done - start_t) / 1e3
+    wait_ms = (admit - t_sub) / 1e3
+    edges = int(sum(x[4] for x in infos))
+    span_s = (done.max() - sched.min()) / 1e6
+    gteps = edges / span_s / 1e9
+    out_bytes = sum(8 * x[5] * g.n for x in infos)
+    line = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": N,
+        "warmup": args.warmup,
+        "ms_per_step": round(span_s * 1e3 / N, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "serving: directed R-MAT n=20M (a=.6 b=.15 c=.15, seed 7), Poisson arrivals "
+                               f"{args.rate:g}/s (seed 7), 1-256 sources per query (seed 8)",
+                   "n": g.n, "m": g.m, "queries": N, "slots": args.slots,
+                   "sources": int(sum(x[5] for x in infos))},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+        "service_p50_ms": round(float(np.percentile(svc_ms, 50)), 2),
+        "service_p99_ms": round(float(np.percentile(svc_ms, 99)), 2),
+        "admission_wait_p50_ms": round(float(np.percentile(wait_ms, 50)), 2),
+        "queries_per_s": round(N / span_s, 1),
+        "offered_per_s": args.rate,
+        "output_TBps": round(out_bytes / span_s / 1e12, 3),
+        "tiles": int(c.tiles - base.tiles),
+        "rows_per_tile": round((c.tile_rows - base.tile_rows) / max(1, c.tiles - base.tiles), 1),
+        "peak_table_GB": round(c.peak_used_bytes / 1e9, 1),
+        "budget_GB": round(c.budget_bytes / 1e9, 1),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven"])
+    ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven", "serving"])
+    ap.add_argument("--requests", type=int, default=400, help="serving: timed queries")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (queries/s)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent tiles")
     ap.add_argument("--engine", default="hostloop", choices=["hostloop", "persistent"],
                     help="hostloop: one launch per phase, host-driven level loop; persistent: the batch API "
                          "(one persistent kernel, level loop on the device, replayed hipGraph)")
@@ -223,6 +340,8 @@ def main() -> int:
 
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     t0 = time.perf_counter()
     g, src, desc = make_workload(args.workload)

@@ -45,6 +45,7 @@ extern "C" {
 #define XB_ERR_BUDGET (-7)        /* request exceeds the HBM admission budget */
 #define XB_ERR_UNSUPPORTED (-8)   /* option combination not supported         */
 #define XB_ERR_NO_DEVICE (-9)     /* no HIP device visible                    */
+#define XB_ERR_TIMEOUT (-10)      /* xb_server_wait: not done within timeout  */
 
 /* ---- graph flags ------------------------------------------------------ */
 #define XB_GRAPH_SYMMETRIC 0u /* undirected: the CSR holds both directions */
@@ -206,6 +207,74 @@ int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes);
 
 /* Copies the last error message of the calling thread into buf. */
 int32_t xb_last_error(char* buf, size_t len);
+
+/* ---- serving: continuous batching over one device-resident graph -------
+ * No counterpart in the reference, whose drivers run one dataset and one
+ * source list per process (apps/xb/src/run_xb_pp_by_dataset.cu); these
+ * entry points serve multi-source BFS queries against a resident graph.
+ *
+ * Requests (source sets) are queued on submit and admitted in arrival order
+ * as their level/parent tables (S x n int32 each, 8*S*n bytes) fit in the
+ * server's device budget.  Worker slots (one HIP stream each) pack the rows
+ * of admitted requests into bit-parallel tiles of up to 64*words_per_tile
+ * sources -- rows of several requests share a tile, a request larger than a
+ * tile spans several -- so small queries share traversals and tiles on
+ * different slots overlap on the device.  A row's result does not depend on
+ * the tile it ran in: every table equals the one xb_msbfs_run writes for
+ * the same sources.  The server owns the graph's layout while it lives: do
+ * not run other work on the same graph handle meanwhile. */
+typedef struct xb_server xb_server;
+
+#define XB_SERVE_COUNT_EDGES 1u /* count traversed edges per request (GTEPS) */
+
+typedef struct xb_server_opts {
+  int32_t slots;          /* worker slots (streams), default 4                 */
+  int32_t max_queued;     /* submitted and unreleased requests, default 65536  */
+  int64_t budget_bytes;   /* device bytes for request tables; 0 = 3/4 of free  */
+  uint32_t flags;         /* XB_SERVE_COUNT_EDGES (default on)                 */
+  int32_t reserved[7];
+} xb_server_opts;
+
+typedef struct xb_request_info {
+  int32_t status;         /* XB_OK once done, else the failure's status code   */
+  int32_t S;              /* sources (rows)                                    */
+  int64_t n;              /* vertices (columns)                                */
+  int32_t* levels;        /* device S x n, valid until xb_server_release       */
+  int32_t* parents;       /* device S x n                                      */
+  int64_t edges;          /* traversed edges (XB_SERVE_COUNT_EDGES), else -1   */
+  double t_submit_us;     /* times since the server was created                */
+  double t_admit_us;      /* tables allocated                                  */
+  double t_start_us;      /* first tile carrying it started                    */
+  double t_done_us;       /* last tile carrying it finished                    */
+  int32_t tiles;          /* tiles that carried its rows                       */
+  int32_t reserved[7];
+} xb_request_info;
+
+typedef struct xb_server_counters {
+  int64_t submitted, completed, failed, released;
+  int64_t tiles, tile_rows;             /* tiles run and the rows they carried  */
+  int64_t budget_bytes, used_bytes, peak_used_bytes;
+  int32_t queued;                       /* waiting for admission or a tile      */
+  int32_t running_tiles;
+  int32_t reserved[8];
+} xb_server_counters;
+
+int32_t xb_server_opts_default(xb_server_opts* o);
+/* engine_opts may be null (defaults); its mode must not be XB_MODE_PLAIN.  */
+int32_t xb_server_create(xb_graph* g, const xb_server_opts* opts, const xb_msbfs_opts* engine_opts,
+                         xb_server** out);
+/* Copies the sources.  XB_ERR_BUDGET if 8*S*n exceeds the whole budget (it
+ * could never be admitted), XB_ERR_SOURCE_RANGE for an id outside [0, n),
+ * XB_ERR_INVALID_ARG for S < 1 or a full queue. */
+int32_t xb_server_submit(xb_server* s, const int32_t* sources, int32_t S, uint64_t* id);
+/* Waits up to timeout_ms (< 0: no limit) for the request; XB_ERR_TIMEOUT if
+ * it is not done by then.  Returns the request's own status once done. */
+int32_t xb_server_wait(xb_server* s, uint64_t id, int32_t timeout_ms, xb_request_info* info);
+/* Frees the request's tables (waits for it first if it is still running). */
+int32_t xb_server_release(xb_server* s, uint64_t id);
+int32_t xb_server_stats(xb_server* s, xb_server_counters* st);
+/* Finishes the tiles in flight, fails what is still queued, frees everything. */
+int32_t xb_server_destroy(xb_server* s);
 
 #ifdef __cplusplus
 }

@@ -100,3 +100,19 @@ def test_text_loader_errors(tmp_path):
     ro = tmp_path / "ro.txt"
     ro.write_text("0 1\n3\n")  # row_offsets[n] != number of column indices
     assert lib.xb_graph_load_text(str(ro).encode(), str(good).encode(), 0, ctypes.byref(h)) == xb.XB_ERR_BAD_CSR
+
+
+def test_server_defaults_and_null_arguments():
+    lib = xb.load_engine()
+    so = xb.ServerOpts()
+    assert lib.xb_server_opts_default(ctypes.byref(so)) == xb.XB_OK
+    assert so.slots >= 1 and so.max_queued >= 1 and so.flags & xb.XB_SERVE_COUNT_EDGES
+    h = ctypes.c_void_p()
+    assert lib.xb_server_create(None, ctypes.byref(so), None, ctypes.byref(h)) == xb.XB_ERR_INVALID_ARG
+    rid = ctypes.c_uint64()
+    src = np.zeros(1, dtype=np.int32)
+    assert lib.xb_server_submit(None, xb._ptr(src), 1, ctypes.byref(rid)) == xb.XB_ERR_INVALID_ARG
+    assert lib.xb_server_wait(None, 1, 0, None) == xb.XB_ERR_INVALID_ARG
+    assert lib.xb_server_release(None, 1) == xb.XB_ERR_INVALID_ARG
+    assert lib.xb_server_stats(None, None) == xb.XB_ERR_INVALID_ARG
+    assert lib.xb_server_destroy(None) == xb.XB_OK

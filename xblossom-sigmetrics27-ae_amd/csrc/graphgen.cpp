@@ -135,13 +135,25 @@ static int build_csr(int32_t n, const int32_t* src, const int32_t* dst, int64_t 
 
 // R-MAT: 2^scale vertices, ndraws edge draws.  symmetric != 0 adds the
 // reverse of every draw.  Quadrant probabilities a, b, c, d = 1-a-b-c.
+int xbgen_rmat_capped(int32_t scale, int32_t n_cap, int64_t ndraws, double a, double b, double c,
+                      uint64_t seed, int32_t symmetric, int32_t nthreads, xbgen_csr* out);
+
 int xbgen_rmat(int32_t scale, int64_t ndraws, double a, double b, double c, uint64_t seed,
                int32_t symmetric, int32_t nthreads, xbgen_csr* out) {
+  if (scale < 1 || scale > 30) return -1;
+  return xbgen_rmat_capped(scale, (int32_t)1 << scale, ndraws, a, b, c, seed, symmetric, nthreads, out);
+}
+
+// R-MAT on 2^scale ids restricted to the first n_cap: a draw with an endpoint
+// at or above n_cap is dropped (so n need not be a power of two).
+int xbgen_rmat_capped(int32_t scale, int32_t n_cap, int64_t ndraws, double a, double b, double c,
+                      uint64_t seed, int32_t symmetric, int32_t nthreads, xbgen_csr* out) {
   if (!out || scale < 1 || scale > 30 || ndraws < 0 || a + b + c > 1.0 + 1e-12) return -1;
+  if (n_cap < 1 || (int64_t)n_cap > ((int64_t)1 << scale)) return -1;
   nthreads = resolve_threads(nthreads);
-  const int32_t n = (int32_t)1 << scale;
+  const int32_t n = n_cap;
   const double ab = a + b, abc = a + b + c;
-  int64_t k = symmetric ? 2 * ndraws : ndraws;
+  const int64_t k = symmetric ? 2 * ndraws : ndraws;
   std::vector<int32_t> src((size_t)std::max<int64_t>(k, 1)), dst((size_t)std::max<int64_t>(k, 1));
   parallel_for(ndraws, nthreads, [&](int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; ++i) {
@@ -158,6 +170,8 @@ int xbgen_rmat(int32_t scale, int64_t ndraws, double a, double b, double c, uint
           v |= 1u << bit;
         }
       }
+      // Out-of-range draws become self-loops of vertex 0, which the CSR build drops.
+      if ((int64_t)u >= n || (int64_t)v >= n) u = v = 0;
       src[i] = (int32_t)u;
       dst[i] = (int32_t)v;
       if (symmetric) {

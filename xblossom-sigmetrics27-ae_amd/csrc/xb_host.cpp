@@ -18,6 +18,12 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -120,7 +126,12 @@ struct Pinned {
   }
 };
 
+// Device state of one engine instance.  The stream and the pinned counter slot
+// belong to the owner (the graph's default workspace uses the graph's; each
+// serving slot has its own), so instances on different streams run at once.
 struct Workspace {
+  hipStream_t st = nullptr;
+  unsigned long long* hctr = nullptr;  // pinned host slot for per-level counter reads
   int T = 0;
   int32_t n = 0;
   xb::BPState bp{};
@@ -160,7 +171,11 @@ struct Workspace {
     dfree(side);
     dfree(stage);
     if (q0_host) (void)hipHostFree(q0_host);
+    hipStream_t keep_st = st;
+    unsigned long long* keep_hctr = hctr;
     *this = Workspace{};
+    st = keep_st;
+    hctr = keep_hctr;
   }
 };
 
@@ -278,6 +293,8 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
   g->flags = flags;
   g->h_off.assign(off, off + n + 1);
   HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  g->ws.st = g->stream;
+  g->ws.hctr = g->pinned.p;
   g->d_off = dalloc<int32_t>((size_t)n + 1);
   g->d_col = dalloc<int32_t>((size_t)m);
   h2d(g->d_off, off, sizeof(int32_t) * ((size_t)n + 1), g->stream);
@@ -430,8 +447,7 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
 // ---------------------------------------------------------------------------
 // Workspace
 // ---------------------------------------------------------------------------
-void ensure_queues(xb_graph* g) {
-  Workspace& w = g->ws;
+void ensure_queues(const xb_graph* g, Workspace& w) {
   if (w.ctrs) return;
   const size_t n = (size_t)g->n + 1;
   w.n = g->n;
@@ -446,8 +462,7 @@ void ensure_queues(xb_graph* g) {
 
 // Bit-parallel state for tiles of up to T words per vertex (a tile with fewer
 // words uses the leading n x words entries of each array).
-void ensure_bp(xb_graph* g, int T) {
-  Workspace& w = g->ws;
+void ensure_bp(const xb_graph* g, Workspace& w, int T) {
   if (w.T >= T) return;
   w.free_bp();
   const size_t nw = (size_t)g->n * (size_t)T + 1;
@@ -463,8 +478,7 @@ void ensure_bp(xb_graph* g, int T) {
   w.bp_dirty = true;
 }
 
-void ensure_src(xb_graph* g, int64_t cnt) {
-  Workspace& w = g->ws;
+void ensure_src(Workspace& w, int64_t cnt) {
   if (w.src_cap >= cnt) return;
   dfree(w.d_src);
   w.d_src = nullptr;
@@ -473,8 +487,7 @@ void ensure_src(xb_graph* g, int64_t cnt) {
 }
 
 // Heavy-parent table: n_heavy x T x 64 entries.
-void ensure_hpar(xb_graph* g, int T) {
-  Workspace& w = g->ws;
+void ensure_hpar(const xb_graph* g, Workspace& w, int T) {
   const int64_t need = (int64_t)std::max(g->n_heavy, 1) * T * 64;
   if (w.side_cap < need) {
     dfree(w.side);
@@ -486,8 +499,7 @@ void ensure_hpar(xb_graph* g, int T) {
   w.bp.hidx = g->d_hidx;
 }
 
-int32_t* ensure_stage(xb_graph* g, int64_t ints) {
-  Workspace& w = g->ws;
+int32_t* ensure_stage(Workspace& w, int64_t ints) {
   if (w.stage_cap < ints) {
     dfree(w.stage);
     w.stage = nullptr;
@@ -497,8 +509,7 @@ int32_t* ensure_stage(xb_graph* g, int64_t ints) {
   return w.stage;
 }
 
-int32_t* ensure_q0_host(xb_graph* g, int64_t ints) {
-  Workspace& w = g->ws;
+int32_t* ensure_q0_host(Workspace& w, int64_t ints) {
   if (w.q0_cap < ints) {
     if (w.q0_host) (void)hipHostFree(w.q0_host);
     w.q0_host = nullptr;
@@ -576,10 +587,10 @@ struct Profiler {
 };
 
 // Reads the packed (edges << 32 | count) counter of a queue.
-void read_ctr(xb_graph* g, const unsigned long long* dctr, int32_t* qn, int32_t* E) {
-  HIPCHK(hipMemcpyAsync(g->pinned.p, dctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  const unsigned long long v = *g->pinned.p;
+void read_ctr(Workspace& w, const unsigned long long* dctr, int32_t* qn, int32_t* E) {
+  HIPCHK(hipMemcpyAsync(w.hctr, dctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  const unsigned long long v = *w.hctr;
   *qn = (int32_t)(uint32_t)(v & 0xFFFFFFFFull);
   *E = (int32_t)(uint32_t)(v >> 32);
 }
@@ -587,31 +598,30 @@ void read_ctr(xb_graph* g, const unsigned long long* dctr, int32_t* qn, int32_t*
 // ---------------------------------------------------------------------------
 // Plain per-source engine
 // ---------------------------------------------------------------------------
-void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows, int32_t* par_rows,
-               bool dev_out, Profiler& prof, xb_run_stats* st) {
-  ensure_queues(g);
-  Workspace& w = g->ws;
+void run_plain(const xb_graph* g, Workspace& w, const int32_t* sources, int32_t S, int32_t* lvl_rows,
+               int32_t* par_rows, bool dev_out, Profiler& prof, xb_run_stats* st) {
+  ensure_queues(g, w);
   const int32_t n = g->n;
   int32_t* stage = nullptr;
-  if (!dev_out || !lvl_rows) stage = ensure_stage(g, 2 * (int64_t)n);
+  if (!dev_out || !lvl_rows) stage = ensure_stage(w, 2 * (int64_t)n);
   for (int32_t s = 0; s < S; ++s) {
     const int32_t src = sources[s];
     int32_t* lv = (dev_out && lvl_rows) ? lvl_rows + (int64_t)s * n : stage;
     int32_t* pa = dev_out ? par_rows + (int64_t)s * n : stage + n;
     prof.begin(XB_K_PLAIN_INIT);
-    xb::launch_plain_init(g->stream, lv, pa, n);
-    xb::launch_plain_set_source(g->stream, lv, pa, src, w.q[0]);
+    xb::launch_plain_init(w.st, lv, pa, n);
+    xb::launch_plain_set_source(w.st, lv, pa, src, w.q[0]);
     prof.end();
     prof.add(XB_K_PLAIN_INIT, 8.0 * n);
     int32_t qn = 1, E = g->outdeg(src);
     int cur = 0, d = 0;
     while (qn > 0) {
-      HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), g->stream));
+      HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), w.st));
       prof.begin(XB_K_PLAIN_PUSH);
-      xb::launch_plain_push(g->stream, g->d_off, g->d_col, w.q[cur], qn, E, lv, pa, d, w.q[cur ^ 1]);
+      xb::launch_plain_push(w.st, g->d_off, g->d_col, w.q[cur], qn, E, lv, pa, d, w.q[cur ^ 1]);
       prof.end();
       const int32_t E_prev = E;
-      read_ctr(g, w.q[cur ^ 1].ctr, &qn, &E);
+      read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
       // col + level gather per frontier edge; level + parent + queue write per new vertex.
       prof.add(XB_K_PLAIN_PUSH, 8.0 * E_prev + 16.0 * qn);
       cur ^= 1;
@@ -624,10 +634,10 @@ void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows
     if (!dev_out) {
       if (lvl_rows)
         HIPCHK(hipMemcpyAsync(lvl_rows + (int64_t)s * n, stage, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
-                              g->stream));
+                              w.st));
       HIPCHK(hipMemcpyAsync(par_rows + (int64_t)s * n, stage + n, sizeof(int32_t) * (size_t)n,
-                            hipMemcpyDeviceToHost, g->stream));
-      HIPCHK(hipStreamSynchronize(g->stream));
+                            hipMemcpyDeviceToHost, w.st));
+      HIPCHK(hipStreamSynchronize(w.st));
     }
   }
 }
@@ -637,9 +647,8 @@ void run_plain(xb_graph* g, const int32_t* sources, int32_t S, int32_t* lvl_rows
 // ---------------------------------------------------------------------------
 
 // Returns false when the tile got deeper than the 8-bit level code allows.
-bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, int32_t* lvl_rows,
-                 int32_t* par_rows, Profiler& prof, xb_run_stats* st) {
-  Workspace& w = g->ws;
+bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t* src, int32_t cnt,
+                 int32_t* lvl_rows, int32_t* par_rows, const xb::RowMap& rows, Profiler& prof, xb_run_stats* st) {
   const int T = o.T;
   const int32_t n = g->n;
   const size_t nw = (size_t)n * (size_t)T;
@@ -648,17 +657,17 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   if (w.bp_dirty) {
     // Whole capacity: later tiles may use more words than this one.
     const size_t cap = (size_t)n * (size_t)w.T;
-    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * cap, g->stream));
-    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * cap, g->stream));
-    HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * cap, g->stream));
-    HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), g->stream));
+    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * cap, w.st));
+    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * cap, w.st));
+    HIPCHK(hipMemsetAsync(s.nextw, 0, sizeof(uint64_t) * cap, w.st));
+    HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), w.st));
     prof.add(XB_K_MEMSET, 24.0 * cap + 4.0 * n);
     w.bp_dirty = false;
   }
-  HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, g->stream));
-  HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, g->stream));
+  HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, w.st));
+  HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, w.st));
   if (g->n_heavy > 0)
-    HIPCHK(hipMemsetAsync(s.hpar, 0xFF, sizeof(uint32_t) * (size_t)g->n_heavy * T * 64, g->stream));
+    HIPCHK(hipMemsetAsync(s.hpar, 0xFF, sizeof(uint32_t) * (size_t)g->n_heavy * T * 64, w.st));
   prof.end();
   prof.add(XB_K_MEMSET, 72.0 * nw + 256.0 * T * g->n_heavy);
   // First frontier: the distinct source vertices with their degree prefix.
@@ -666,8 +675,8 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   std::sort(uniq.begin(), uniq.end());
   uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
   const int32_t qn0 = (int32_t)uniq.size();
-  HIPCHK(hipStreamSynchronize(g->stream));  // earlier uploads from the pinned buffer are done
-  int32_t* q0 = ensure_q0_host(g, 2 * (int64_t)qn0 + cnt);
+  HIPCHK(hipStreamSynchronize(w.st));  // earlier uploads from the pinned buffer are done
+  int32_t* q0 = ensure_q0_host(w, 2 * (int64_t)qn0 + cnt);
   int64_t E0 = 0;
   for (int32_t i = 0; i < qn0; ++i) {
     q0[i] = uniq[i];
@@ -675,11 +684,11 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
     E0 += g->outdeg(uniq[i]);
   }
   std::memcpy(q0 + 2 * qn0, src, sizeof(int32_t) * (size_t)cnt);
-  HIPCHK(hipMemcpyAsync(w.q[0].v, q0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
-  HIPCHK(hipMemcpyAsync(w.q[0].eo, q0 + qn0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, g->stream));
-  HIPCHK(hipMemcpyAsync(w.d_src, q0 + 2 * qn0, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(w.q[0].v, q0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.q[0].eo, q0 + qn0, sizeof(int32_t) * (size_t)qn0, hipMemcpyHostToDevice, w.st));
+  HIPCHK(hipMemcpyAsync(w.d_src, q0 + 2 * qn0, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice, w.st));
   prof.begin(XB_K_SEED);
-  xb::launch_bp_seed(g->stream, T, s, w.d_src, cnt);
+  xb::launch_bp_seed(w.st, T, s, w.d_src, cnt);
   prof.end();
   prof.add(XB_K_SEED, 28.0 * cnt);
 
@@ -696,15 +705,15 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
     }
     xb::Queue& qc = w.q[cur];
     xb::Queue& qx = w.q[cur ^ 1];
-    HIPCHK(hipMemsetAsync(qx.ctr, 0, sizeof(unsigned long long), g->stream));
-    HIPCHK(hipMemsetAsync(w.tcount, 0, sizeof(uint32_t), g->stream));
+    HIPCHK(hipMemsetAsync(qx.ctr, 0, sizeof(unsigned long long), w.st));
+    HIPCHK(hipMemsetAsync(w.tcount, 0, sizeof(uint32_t), w.st));
     if ((int64_t)qn + E > thresh) {
       int32_t begin = 0;
       for (int c = 0; c < 3; ++c) {
         const int32_t end = g->bucket_end[c];
         if (end > begin) {
           prof.begin(XB_K_PULL);
-          xb::launch_bp_pull_bucket(g->stream, T, c, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off,
+          xb::launch_bp_pull_bucket(w.st, T, c, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off,
                                     g->d_perm, begin, end, qx);
           prof.end();
           // offsets + in-edges + (front, seen r/w) per vertex and word
@@ -714,30 +723,30 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
       }
       if (g->n_hub_chunks > 0) {
         prof.begin(XB_K_PULL_HUB);
-        xb::launch_bp_pull_hub(g->stream, T, s, cur, cnt, g->d_in_col, g->d_hub_chunks, g->n_hub_chunks);
+        xb::launch_bp_pull_hub(w.st, T, s, cur, cnt, g->d_in_col, g->d_hub_chunks, g->n_hub_chunks);
         prof.end();
         prof.add(XB_K_PULL_HUB, 4.0 * g->hub_edges + 24.0 * T * g->n_hubs);
         prof.begin(XB_K_COMMIT);
-        xb::launch_bp_commit(g->stream, T, s, cur, code, g->d_off, qx, g->n_hubs);
+        xb::launch_bp_commit(w.st, T, s, cur, code, g->d_off, qx, g->n_hubs);
         prof.end();
       }
       ++pulls;
     } else {
       prof.begin(XB_K_PUSH);
-      xb::launch_bp_push(g->stream, T, s, cur, g->d_off, g->d_col, qc, qn, E);
+      xb::launch_bp_push(w.st, T, s, cur, g->d_off, g->d_col, qc, qn, E);
       prof.end();
       // queue + frontier words of the queue, then col + seen words per edge
       prof.add(XB_K_PUSH, (8.0 + 8.0 * T) * qn + (4.0 + 8.0 * T) * E);
       prof.begin(XB_K_COMMIT);
-      xb::launch_bp_commit(g->stream, T, s, cur, code, g->d_off, qx, (int32_t)std::min<int64_t>(E, n));
+      xb::launch_bp_commit(w.st, T, s, cur, code, g->d_off, qx, (int32_t)std::min<int64_t>(E, n));
       prof.end();
       ++pushes;
     }
     prof.begin(XB_K_CLEAR);
-    xb::launch_bp_clear(g->stream, T, s, cur, qc, qn);
+    xb::launch_bp_clear(w.st, T, s, cur, qc, qn);
     prof.end();
     prof.add(XB_K_CLEAR, (4.0 + 8.0 * T) * qn);
-    read_ctr(g, qx.ctr, &qn, &E);
+    read_ctr(w, qx.ctr, &qn, &E);
     cur ^= 1;
     if (qn > 0) ++d;
   }
@@ -752,12 +761,12 @@ bool run_bp_tile(xb_graph* g, const BPOpts& o, const int32_t* src, int32_t cnt, 
   for (int t = 0; t < words; ++t) {
     if (g->n_heavy > 0) {
       prof.begin(XB_K_EMIT_HEAVY);
-      xb::launch_bp_emit_heavy(g->stream, T, s, t, K, g->d_in_off, g->d_in_col, g->d_heavy, g->n_heavy);
+      xb::launch_bp_emit_heavy(w.st, T, s, t, K, g->d_in_off, g->d_in_col, g->d_heavy, g->n_heavy);
       prof.end();
     }
     prof.begin(XB_K_EMIT);
-    xb::launch_bp_emit(g->stream, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.cut_medium, lvl_rows,
-                       par_rows);
+    xb::launch_bp_emit(w.st, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.cut_medium, lvl_rows,
+                       par_rows, rows);
     prof.end();
     const int rows_w = std::min(64, cnt - 64 * t);
     // level + parent int32 per (source, vertex) of this word
@@ -781,39 +790,39 @@ int tile_words(int32_t cnt, int Tmax) {
   return t;
 }
 
-void run_batched(xb_graph* g, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl, int32_t* par,
+void run_batched(xb_graph* g, Workspace& w, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl, int32_t* par,
                  bool dev_out, Profiler& prof, xb_run_stats* st) {
   const int32_t tile = 64 * o.T;
   const int Tneed = tile_words(std::min(S, tile), o.T);  // the widest tile of this run
-  ensure_queues(g);
-  ensure_bp(g, Tneed);
+  ensure_queues(g, w);
+  ensure_bp(g, w, Tneed);
   ensure_layout(g, o);
-  ensure_hpar(g, Tneed);
+  ensure_hpar(g, w, Tneed);
   const int32_t n = g->n;
-  ensure_src(g, tile);
+  ensure_src(w, tile);
   int32_t* stage = nullptr;
-  if (!dev_out) stage = ensure_stage(g, 2 * (int64_t)tile * n);
+  if (!dev_out) stage = ensure_stage(w, 2 * (int64_t)tile * n);
   for (int32_t base = 0; base < S; base += tile) {
     const int32_t cnt = std::min(tile, S - base);
     int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : (lvl ? stage : nullptr);
     int32_t* pr = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
     BPOpts ot = o;
     ot.T = tile_words(cnt, o.T);
-    bool ok = run_bp_tile(g, ot, sources + base, cnt, lr, pr, prof, st);
+    bool ok = run_bp_tile(g, w, ot, sources + base, cnt, lr, pr, xb::RowMap{}, prof, st);
     if (!ok) {
       // Deeper than 254 levels: these sources take the plain path.
       if (st) st->fallback_sources += cnt;
-      run_plain(g, sources + base, cnt, lvl ? lvl + (int64_t)base * n : nullptr, par + (int64_t)base * n,
+      run_plain(g, w, sources + base, cnt, lvl ? lvl + (int64_t)base * n : nullptr, par + (int64_t)base * n,
                 dev_out, prof, st);
       continue;
     }
     if (!dev_out) {
       if (lvl)
         HIPCHK(hipMemcpyAsync(lvl + (int64_t)base * n, stage, sizeof(int32_t) * (size_t)cnt * n,
-                              hipMemcpyDeviceToHost, g->stream));
+                              hipMemcpyDeviceToHost, w.st));
       HIPCHK(hipMemcpyAsync(par + (int64_t)base * n, stage + (int64_t)tile * n, sizeof(int32_t) * (size_t)cnt * n,
-                            hipMemcpyDeviceToHost, g->stream));
-      HIPCHK(hipStreamSynchronize(g->stream));
+                            hipMemcpyDeviceToHost, w.st));
+      HIPCHK(hipStreamSynchronize(w.st));
     }
   }
 }
@@ -917,7 +926,6 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     if (S == 0) return;
     if (!sources || !parents) fail(XB_ERR_INVALID_ARG, "null sources or parents");
     const xb_msbfs_opts o = parse_opts(opts);
-    const int T = o.words_per_tile;
     if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
     check_sources(g, sources, S);
     const bool dev_out = (o.flags & XB_OUT_DEVICE) != 0;
@@ -930,10 +938,10 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, g->stream));
     if (o.mode == XB_MODE_PLAIN) {
-      run_plain(g, sources, S, levels, parents, dev_out, prof, stats);
+      run_plain(g, g->ws, sources, S, levels, parents, dev_out, prof, stats);
     } else {
       const BPOpts bo = bp_opts(o);
-      run_batched(g, bo, sources, S, levels, parents, dev_out, prof, stats);
+      run_batched(g, g->ws, bo, sources, S, levels, parents, dev_out, prof, stats);
     }
     HIPCHK(hipEventRecord(e1, g->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -1381,7 +1389,7 @@ void run_batch(xb_batch* b, xb_run_stats* st) {
       // Deeper than the level code: the plain path writes this tile's rows.
       Profiler prof;
       prof.st = g->stream;
-      run_plain(g, b->job_sources[k].data(), j.cnt, j.lvl_out, j.par_out, true, prof, st);
+      run_plain(g, g->ws, b->job_sources[k].data(), j.cnt, j.lvl_out, j.par_out, true, prof, st);
       if (st) st->fallback_sources += j.cnt;
       continue;
     }
@@ -1482,6 +1490,445 @@ int32_t xb_msbfs_run_batched(xb_graph* const* graphs, int32_t G, const int32_t* 
       HIPCHK(hipMemcpy(parents[i], dp[i], sizeof(int32_t) * cells, hipMemcpyDeviceToHost));
       if (!no_levels) HIPCHK(hipMemcpy(levels[i], dl[i], sizeof(int32_t) * cells, hipMemcpyDeviceToHost));
     }
+  });
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving: continuous batching over one device-resident graph
+// ===========================================================================
+namespace {
+
+using ServeClock = std::chrono::steady_clock;
+
+// First-fit allocator over one device allocation (byte offsets, 256-aligned),
+// with neighbour coalescing on release.
+struct Arena {
+  char* base = nullptr;
+  int64_t size = 0, used = 0, peak = 0;
+  std::map<int64_t, int64_t> free_;  // offset -> length
+
+  static int64_t round(int64_t b) { return (b + 255) & ~(int64_t)255; }
+  void init(int64_t bytes) {
+    base = dalloc<char>((size_t)bytes);
+    size = bytes;
+    free_.clear();
+    free_[0] = bytes;
+  }
+  int64_t alloc(int64_t bytes) {
+    bytes = round(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes) {
+        const int64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        used += bytes;
+        peak = std::max(peak, used);
+        return off;
+      }
+    return -1;
+  }
+  void release(int64_t off, int64_t bytes) {
+    bytes = round(bytes);
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct ServeReq {
+  uint64_t id = 0;
+  std::vector<int32_t> src;
+  int32_t S = 0;
+  int64_t off = -1, bytes = 0;  // arena block: levels (S x n) then parents (S x n)
+  int32_t next_row = 0;         // rows handed to tiles so far
+  int32_t rows_done = 0;
+  int32_t tiles = 0;
+  int64_t edges = 0;
+  int32_t status = 1;  // 1 while pending, then XB_OK or the failure's code
+  std::string err;
+  double t_submit = 0, t_admit = -1, t_start = -1, t_done = -1;
+};
+
+// Request rows [row0, row0 + rows) carried by consecutive rows of one tile.
+struct TileSeg {
+  ServeReq* req;
+  int32_t row0, rows;
+};
+
+struct ServeSlot {
+  Workspace ws;
+  unsigned long long* hctr = nullptr;  // pinned, per-level counter reads
+  int64_t* h_rows = nullptr;           // pinned: level row offsets, then parent row offsets
+  int64_t* d_rows = nullptr;
+  unsigned long long* h_edges = nullptr;  // pinned, one per segment
+  unsigned long long* d_edges = nullptr;
+  std::thread th;
+};
+
+}  // namespace
+
+struct xb_server {
+  xb_graph* g = nullptr;
+  BPOpts o{};
+  int dev = 0;
+  int32_t tile = 0;
+  bool count_edges = true;
+  int32_t max_queued = 0;
+  Arena arena;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ServeReq*> pending;  // arrival order; requests with rows not yet in a tile
+  std::unordered_map<uint64_t, std::unique_ptr<ServeReq>> reqs;
+  std::vector<std::unique_ptr<ServeSlot>> slots;
+  uint64_t next_id = 1;
+  bool stop = false;
+  ServeClock::time_point t0;
+  int64_t submitted = 0, completed = 0, failed = 0, released = 0, tiles = 0, tile_rows = 0;
+  int32_t running = 0;
+
+  double now_us() const {
+    return std::chrono::duration<double, std::micro>(ServeClock::now() - t0).count();
+  }
+  ~xb_server() {
+    for (auto& sl : slots) {
+      sl->ws.free_all();
+      dfree(sl->d_rows);
+      dfree(sl->d_edges);
+      if (sl->h_rows) (void)hipHostFree(sl->h_rows);
+      if (sl->h_edges) (void)hipHostFree(sl->h_edges);
+      if (sl->hctr) (void)hipHostFree(sl->hctr);
+      if (sl->ws.st) (void)hipStreamDestroy(sl->ws.st);
+    }
+    dfree(arena.base);
+  }
+};
+
+namespace {
+
+// Under the lock: admits requests in arrival order while their tables fit the
+// budget (the first that does not fit holds the rest back, so a large request
+// is not starved) and packs their rows into one tile of up to `tile` sources.
+bool form_tile(xb_server* sv, std::vector<TileSeg>& segs, std::vector<int32_t>& src) {
+  segs.clear();
+  src.clear();
+  int32_t cap = sv->tile;
+  for (auto it = sv->pending.begin(); it != sv->pending.end() && cap > 0;) {
+    ServeReq* r = *it;
+    if (r->off < 0) {
+      const int64_t off = sv->arena.alloc(r->bytes);
+      if (off < 0) break;
+      r->off = off;
+      r->t_admit = sv->now_us();
+    }
+    const int32_t k = std::min(cap, r->S - r->next_row);
+    segs.push_back(TileSeg{r, r->next_row, k});
+    src.insert(src.end(), r->src.begin() + r->next_row, r->src.begin() + r->next_row + k);
+    if (r->t_start < 0) r->t_start = sv->now_us();
+    r->tiles++;
+    r->next_row += k;
+    cap -= k;
+    if (r->next_row == r->S)
+      it = sv->pending.erase(it);
+    else
+      ++it;
+  }
+  return !segs.empty();
+}
+
+// One tile on a slot's stream: the bit-parallel engine with the tile's rows
+// mapped into the requests' tables (rows deeper than the level code take the
+// plain engine, row by row), then the traversed-edge count per segment.
+void run_serve_tile(xb_server* sv, ServeSlot* sl, const std::vector<TileSeg>& segs,
+                    const std::vector<int32_t>& src, Profiler& prof, std::vector<int64_t>& edges) {
+  const xb_graph* g = sv->g;
+  const int64_t n = g->n;
+  const int32_t cnt = (int32_t)src.size();
+  int32_t* base = reinterpret_cast<int32_t*>(sv->arena.base);
+  int32_t r = 0;
+  for (const TileSeg& sg : segs) {
+    const int64_t lvl0 = sg.req->off / (int64_t)sizeof(int32_t);
+    const int64_t par0 = lvl0 + (int64_t)sg.req->S * n;
+    for (int32_t i = 0; i < sg.rows; ++i, ++r) {
+      sl->h_rows[r] = lvl0 + (int64_t)(sg.row0 + i) * n;
+      sl->h_rows[sv->tile + r] = par0 + (int64_t)(sg.row0 + i) * n;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(sl->d_rows, sl->h_rows, sizeof(int64_t) * 2 * (size_t)sv->tile, hipMemcpyHostToDevice,
+                        sl->ws.st));
+  BPOpts ot = sv->o;
+  ot.T = tile_words(cnt, sv->o.T);
+  const xb::RowMap map{sl->d_rows, sl->d_rows + sv->tile};
+  if (!run_bp_tile(g, sl->ws, ot, src.data(), cnt, base, base, map, prof, nullptr)) {
+    for (int32_t i = 0; i < cnt; ++i)
+      run_plain(g, sl->ws, &src[(size_t)i], 1, base + sl->h_rows[i], base + sl->h_rows[sv->tile + i], true, prof,
+                nullptr);
+  }
+  if (sv->count_edges) {
+    HIPCHK(hipMemsetAsync(sl->d_edges, 0, sizeof(unsigned long long) * segs.size(), sl->ws.st));
+    int32_t r0 = 0;
+    for (size_t k = 0; k < segs.size(); ++k) {
+      xb::launch_count_traversed(sl->ws.st, g->d_off, base + sl->h_rows[r0], segs[k].rows, (int32_t)n,
+                                 sl->d_edges + k);
+      r0 += segs[k].rows;
+    }
+    HIPCHK(hipMemcpyAsync(sl->h_edges, sl->d_edges, sizeof(unsigned long long) * segs.size(), hipMemcpyDeviceToHost,
+                          sl->ws.st));
+  }
+  HIPCHK(hipStreamSynchronize(sl->ws.st));
+  for (size_t k = 0; k < segs.size(); ++k) edges[k] = sv->count_edges ? (int64_t)sl->h_edges[k] : -1;
+}
+
+void serve_worker(xb_server* sv, ServeSlot* sl) {
+  (void)hipSetDevice(sv->dev);
+  Profiler prof;
+  prof.st = sl->ws.st;
+  std::vector<TileSeg> segs;
+  std::vector<int32_t> src;
+  std::vector<int64_t> edges;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      for (;;) {
+        if (sv->stop) return;
+        if (form_tile(sv, segs, src)) break;
+        sv->cv_work.wait(lk);
+      }
+      sv->running++;
+    }
+    int32_t code = XB_OK;
+    std::string msg;
+    edges.assign(segs.size(), -1);
+    try {
+      run_serve_tile(sv, sl, segs, src, prof, edges);
+    } catch (const XbError& e) {
+      code = e.code;
+      msg = e.msg;
+    } catch (const std::exception& e) {
+      code = XB_ERR_DEVICE;
+      msg = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      const double t = sv->now_us();
+      sv->running--;
+      sv->tiles++;
+      sv->tile_rows += (int64_t)src.size();
+      for (size_t k = 0; k < segs.size(); ++k) {
+        ServeReq* r = segs[k].req;
+        r->rows_done += segs[k].rows;
+        if (code != XB_OK && r->err.empty()) {
+          r->err = msg;
+          r->status = code;
+        }
+        if (edges[k] >= 0) r->edges += edges[k];
+        if (r->rows_done == r->S) {
+          if (r->status == 1) {
+            r->status = XB_OK;
+            sv->completed++;
+          } else {
+            sv->failed++;
+          }
+          r->t_done = t;
+        }
+      }
+    }
+    sv->cv_done.notify_all();
+  }
+}
+
+ServeReq* find_req(xb_server* sv, uint64_t id) {
+  auto it = sv->reqs.find(id);
+  if (it == sv->reqs.end()) fail(XB_ERR_INVALID_ARG, "unknown request id " + std::to_string(id));
+  return it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t xb_server_opts_default(xb_server_opts* o) {
+  return guarded([&] {
+    if (!o) fail(XB_ERR_INVALID_ARG, "null opts");
+    std::memset(o, 0, sizeof(*o));
+    o->slots = 4;
+    o->max_queued = 65536;
+    o->budget_bytes = 0;
+    o->flags = XB_SERVE_COUNT_EDGES;
+  });
+}
+
+int32_t xb_server_create(xb_graph* g, const xb_server_opts* opts, const xb_msbfs_opts* engine_opts,
+                         xb_server** out) {
+  return guarded([&] {
+    if (!g || !out) fail(XB_ERR_INVALID_ARG, "null graph or out handle");
+    xb_server_opts so;
+    xb_server_opts_default(&so);
+    if (opts) {
+      if (opts->slots < 0 || opts->max_queued < 0 || opts->budget_bytes < 0)
+        fail(XB_ERR_INVALID_ARG, "negative server option");
+      if (opts->slots) so.slots = opts->slots;
+      if (opts->max_queued) so.max_queued = opts->max_queued;
+      so.budget_bytes = opts->budget_bytes;
+      so.flags = opts->flags;
+    }
+    if (so.slots > 64) fail(XB_ERR_INVALID_ARG, "at most 64 slots");
+    const xb_msbfs_opts eo = parse_opts(engine_opts);
+    if (eo.mode == XB_MODE_PLAIN) fail(XB_ERR_UNSUPPORTED, "the server runs the bit-parallel engine");
+    std::unique_ptr<xb_server> sv(new xb_server());
+    sv->g = g;
+    sv->o = bp_opts(eo);
+    sv->tile = 64 * sv->o.T;
+    sv->count_edges = (so.flags & XB_SERVE_COUNT_EDGES) != 0;
+    sv->max_queued = so.max_queued;
+    HIPCHK(hipGetDevice(&sv->dev));
+    ensure_layout(g, sv->o);
+    // Slots: every buffer a tile needs is allocated here (freeing device
+    // memory synchronizes the device, which would stall the other slots).
+    for (int i = 0; i < so.slots; ++i) {
+      std::unique_ptr<ServeSlot> sl(new ServeSlot());
+      HIPCHK(hipStreamCreateWithFlags(&sl->ws.st, hipStreamNonBlocking));
+      HIPCHK(hipHostMalloc((void**)&sl->hctr, 64, hipHostMallocDefault));
+      sl->ws.hctr = sl->hctr;
+      ensure_queues(g, sl->ws);
+      ensure_bp(g, sl->ws, sv->o.T);
+      ensure_hpar(g, sl->ws, sv->o.T);
+      ensure_src(sl->ws, sv->tile);
+      ensure_q0_host(sl->ws, 3 * (int64_t)sv->tile);
+      sl->d_rows = dalloc<int64_t>(2 * (size_t)sv->tile);
+      sl->d_edges = dalloc<unsigned long long>((size_t)sv->tile);
+      HIPCHK(hipHostMalloc((void**)&sl->h_rows, sizeof(int64_t) * 2 * (size_t)sv->tile, hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&sl->h_edges, sizeof(unsigned long long) * (size_t)sv->tile,
+                           hipHostMallocDefault));
+      sv->slots.push_back(std::move(sl));
+    }
+    int64_t budget = so.budget_bytes;
+    if (budget == 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      budget = (int64_t)(fr / 4 * 3);
+    }
+    if (budget < 256) fail(XB_ERR_BUDGET, "serving budget below 256 bytes");
+    sv->arena.init(budget);
+    sv->t0 = ServeClock::now();
+    for (auto& sl : sv->slots) sl->th = std::thread(serve_worker, sv.get(), sl.get());
+    *out = sv.release();
+  });
+}
+
+int32_t xb_server_submit(xb_server* sv, const int32_t* sources, int32_t S, uint64_t* id) {
+  return guarded([&] {
+    if (!sv || !sources || !id) fail(XB_ERR_INVALID_ARG, "null argument");
+    if (S < 1) fail(XB_ERR_INVALID_ARG, "a request needs at least one source");
+    check_sources(sv->g, sources, S);
+    const int64_t bytes = 8 * (int64_t)S * sv->g->n;
+    if (Arena::round(bytes) > sv->arena.size)
+      fail(XB_ERR_BUDGET, "request tables need " + std::to_string(bytes) + " bytes, the serving budget is " +
+                              std::to_string(sv->arena.size));
+    std::unique_ptr<ServeReq> r(new ServeReq());
+    r->src.assign(sources, sources + S);
+    r->S = S;
+    r->bytes = bytes;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if ((int64_t)sv->reqs.size() >= sv->max_queued) fail(XB_ERR_INVALID_ARG, "serving queue is full");
+      r->id = sv->next_id++;
+      r->t_submit = sv->now_us();
+      *id = r->id;
+      sv->pending.push_back(r.get());
+      sv->reqs.emplace(r->id, std::move(r));
+      sv->submitted++;
+    }
+    sv->cv_work.notify_one();
+  });
+}
+
+int32_t xb_server_wait(xb_server* sv, uint64_t id, int32_t timeout_ms, xb_request_info* info) {
+  return guarded([&] {
+    if (!sv) fail(XB_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    ServeReq* r = find_req(sv, id);
+    auto done = [&] { return r->status != 1; };
+    if (timeout_ms < 0)
+      sv->cv_done.wait(lk, done);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+      fail(XB_ERR_TIMEOUT, "request " + std::to_string(id) + " not done within " + std::to_string(timeout_ms) + " ms");
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->status = r->status;
+      info->S = r->S;
+      info->n = sv->g->n;
+      int32_t* base = reinterpret_cast<int32_t*>(sv->arena.base);
+      info->levels = r->off >= 0 ? base + r->off / (int64_t)sizeof(int32_t) : nullptr;
+      info->parents = r->off >= 0 ? info->levels + (int64_t)r->S * sv->g->n : nullptr;
+      info->edges = sv->count_edges ? r->edges : -1;
+      info->t_submit_us = r->t_submit;
+      info->t_admit_us = r->t_admit;
+      info->t_start_us = r->t_start;
+      info->t_done_us = r->t_done;
+      info->tiles = r->tiles;
+    }
+    if (r->status != XB_OK) fail(r->status, "request " + std::to_string(id) + " failed: " + r->err);
+  });
+}
+
+int32_t xb_server_release(xb_server* sv, uint64_t id) {
+  return guarded([&] {
+    if (!sv) fail(XB_ERR_INVALID_ARG, "null server");
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      ServeReq* r = find_req(sv, id);
+      sv->cv_done.wait(lk, [&] { return r->status != 1; });
+      if (r->off >= 0) sv->arena.release(r->off, r->bytes);
+      sv->reqs.erase(id);
+      sv->released++;
+    }
+    sv->cv_work.notify_all();
+  });
+}
+
+int32_t xb_server_stats(xb_server* sv, xb_server_counters* st) {
+  return guarded([&] {
+    if (!sv || !st) fail(XB_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::memset(st, 0, sizeof(*st));
+    st->submitted = sv->submitted;
+    st->completed = sv->completed;
+    st->failed = sv->failed;
+    st->released = sv->released;
+    st->tiles = sv->tiles;
+    st->tile_rows = sv->tile_rows;
+    st->budget_bytes = sv->arena.size;
+    st->used_bytes = sv->arena.used;
+    st->peak_used_bytes = sv->arena.peak;
+    st->queued = (int32_t)sv->pending.size();
+    st->running_tiles = sv->running;
+  });
+}
+
+int32_t xb_server_destroy(xb_server* sv) {
+  return guarded([&] {
+    if (!sv) return;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    for (auto& sl : sv->slots)
+      if (sl->th.joinable()) sl->th.join();
+    delete sv;
   });
 }
 

@@ -59,6 +59,14 @@ struct GraphView {
 };
 
 // One tile (up to 64*T sources) of one graph.
+// Where the emit writes row r of a tile (element offsets from the level and
+// parent base pointers).  Null tables: row r at r * n, the S x n layout of one
+// run; a serving tile maps its rows into the tables of several requests.
+struct RowMap {
+  const int64_t* lvl_row = nullptr;
+  const int64_t* par_row = nullptr;
+};
+
 struct TileJob {
   int32_t graph;           // index into the GraphView array
   int32_t cnt;             // sources in the tile
@@ -141,7 +149,7 @@ void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_
 // src: the tile's source ids (level-1 parents are the sources themselves).
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
-                    int32_t* par_out);
+                    int32_t* par_out, RowMap map);
 
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

@@ -718,7 +718,7 @@ template <int T, int K>
 __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
                                  const int32_t* __restrict__ in_col, int32_t n, int32_t heavy_cut,
-                                 int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out) {
+                                 int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
   EmitLds& L = g_lds.e;
   const int l = lane_id();
   const int wv = threadIdx.x / kWave;
@@ -807,9 +807,10 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
 #pragma unroll
         for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
         const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[bit][l]));
-        const int64_t o = (int64_t)r * n + v2;
-        if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[o]);
-        __builtin_nontemporal_store(par, &par_out[o]);
+        const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v2;
+        const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v2;
+        if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[lo]);
+        __builtin_nontemporal_store(par, &par_out[po]);
       }
     }
     __syncthreads();
@@ -876,8 +877,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t wor
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col, int32_t n,
                                                           int32_t heavy_cut, int32_t* __restrict__ lvl_out,
-                                                          int32_t* __restrict__ par_out) {
-  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+                                                          int32_t* __restrict__ par_out, RowMap map) {
+  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out, map);
 }
 
 // ============================================================================
@@ -1048,7 +1049,7 @@ template <int T, int K>
 __device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
                                            const int32_t* in_off, const int32_t* in_col, int32_t n,
                                            int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
-  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out);
+  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out, RowMap{});
 }
 
 template <int T, int K>
@@ -1425,9 +1426,9 @@ static void emit_heavy_launch(hipStream_t st, BPState s, int32_t word, const int
 template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
-                              int32_t* lvl_out, int32_t* par_out) {
+                              int32_t* lvl_out, int32_t* par_out, RowMap map) {
   hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64)), dim3(kBlock), 0, st, s, word, rows, src, in_off,
-                     in_col, n, heavy_cut, lvl_out, par_out);
+                     in_col, n, heavy_cut, lvl_out, par_out, map);
 }
 
 void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
@@ -1438,9 +1439,9 @@ void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_
 
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
-                    int32_t* par_out) {
+                    int32_t* par_out, RowMap map) {
   XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, n, heavy_cut,
-                                                               lvl_out, par_out))));
+                                                               lvl_out, par_out, map))));
 }
 
 int persistent_blocks_per_cu() {

@@ -37,6 +37,7 @@ XB_ERR_SOURCE_RANGE = -6
 XB_ERR_BUDGET = -7
 XB_ERR_UNSUPPORTED = -8
 XB_ERR_NO_DEVICE = -9
+XB_ERR_TIMEOUT = -10
 
 XB_GRAPH_SYMMETRIC = 0
 XB_GRAPH_DIRECTED = 1
@@ -69,7 +70,16 @@ EXPORTED_SYMBOLS = (
     "xb_memcpy_d2h",
     "xb_memcpy_h2d",
     "xb_last_error",
+    "xb_server_opts_default",
+    "xb_server_create",
+    "xb_server_submit",
+    "xb_server_wait",
+    "xb_server_release",
+    "xb_server_stats",
+    "xb_server_destroy",
 )
+
+XB_SERVE_COUNT_EDGES = 1
 
 
 class XbError(RuntimeError):
@@ -129,6 +139,30 @@ class RunStats(ctypes.Structure):
         return d
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("max_queued", ctypes.c_int32), ("budget_bytes", ctypes.c_int64),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("S", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("levels", ctypes.c_void_p), ("parents", ctypes.c_void_p), ("edges", ctypes.c_int64),
+                ("t_submit_us", ctypes.c_double), ("t_admit_us", ctypes.c_double),
+                ("t_start_us", ctypes.c_double), ("t_done_us", ctypes.c_double),
+                ("tiles", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("released", ctypes.c_int64), ("tiles", ctypes.c_int64), ("tile_rows", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64), ("used_bytes", ctypes.c_int64),
+                ("peak_used_bytes", ctypes.c_int64), ("queued", ctypes.c_int32),
+                ("running_tiles", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 _lib = None
 _gen = None
 
@@ -174,6 +208,13 @@ def load_engine():
     lib.xb_memcpy_d2h.argtypes = [VP, VP, ctypes.c_size_t]
     lib.xb_memcpy_h2d.argtypes = [VP, VP, ctypes.c_size_t]
     lib.xb_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.xb_server_opts_default.argtypes = [ctypes.POINTER(ServerOpts)]
+    lib.xb_server_create.argtypes = [VP, ctypes.POINTER(ServerOpts), ctypes.POINTER(Opts), ctypes.POINTER(VP)]
+    lib.xb_server_submit.argtypes = [VP, _P32, I32, ctypes.POINTER(ctypes.c_uint64)]
+    lib.xb_server_wait.argtypes = [VP, ctypes.c_uint64, I32, ctypes.POINTER(RequestInfo)]
+    lib.xb_server_release.argtypes = [VP, ctypes.c_uint64]
+    lib.xb_server_stats.argtypes = [VP, ctypes.POINTER(ServerCounters)]
+    lib.xb_server_destroy.argtypes = [VP]
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name).restype = ctypes.c_int32
     _lib = lib
@@ -374,6 +415,68 @@ class Batch:
             pass
 
 
+class Server:
+    """Continuous-batching query server over one device-resident graph.
+
+    submit(sources) -> request id; wait(id) -> RequestInfo with device
+    pointers to the request's S x n level and parent tables (valid until
+    release(id)).  Rows of several requests share bit-parallel tiles, tiles
+    on different slots run concurrently, and a request is admitted once its
+    tables fit in the device budget."""
+
+    def __init__(self, graph: HGraph, slots: int = 4, budget_bytes: int = 0, count_edges: bool = True,
+                 max_queued: int = 0, **tunables):
+        lib = load_engine()
+        so = ServerOpts()
+        _check(lib.xb_server_opts_default(ctypes.byref(so)))
+        so.slots = slots
+        so.budget_bytes = budget_bytes
+        so.flags = XB_SERVE_COUNT_EDGES if count_edges else 0
+        if max_queued:
+            so.max_queued = max_queued
+        eo = default_opts(**tunables)
+        self.graph = graph
+        self._h = ctypes.c_void_p()
+        _check(lib.xb_server_create(graph.handle, ctypes.byref(so), ctypes.byref(eo), ctypes.byref(self._h)))
+
+    def submit(self, sources) -> int:
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        rid = ctypes.c_uint64()
+        _check(load_engine().xb_server_submit(self._h, _ptr(src), src.size, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> RequestInfo:
+        info = RequestInfo()
+        _check(load_engine().xb_server_wait(self._h, rid, timeout_ms, ctypes.byref(info)))
+        return info
+
+    def release(self, rid: int):
+        _check(load_engine().xb_server_release(self._h, rid))
+
+    def counters(self) -> ServerCounters:
+        c = ServerCounters()
+        _check(load_engine().xb_server_stats(self._h, ctypes.byref(c)))
+        return c
+
+    def close(self):
+        if self._h:
+            _check(load_engine().xb_server_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_checksum(ptr: int, count: int) -> int:
+    """Order-independent checksum of count int32 values at a device address."""
+    out = ctypes.c_uint64()
+    _check(load_engine().xb_device_checksum(_ptr(int(ptr)), count, ctypes.byref(out)))
+    return out.value
+
+
 def run_batched(graphs, sources, **tunables):
     """Host-output convenience: returns [(levels, parents)] per graph."""
     lib = load_engine()
@@ -418,6 +521,7 @@ def load_gen():
     g = ctypes.CDLL(GEN_LIB)
     D, I32, I64, U64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
     g.xbgen_rmat.argtypes = [I32, I64, D, D, D, U64, I32, I32, ctypes.POINTER(_Csr)]
+    g.xbgen_rmat_capped.argtypes = [I32, I32, I64, D, D, D, U64, I32, I32, ctypes.POINTER(_Csr)]
     g.xbgen_chung_lu.argtypes = [I32, D, D, D, D, U64, I32, I32, ctypes.POINTER(_Csr)]
     g.xbgen_free.argtypes = [ctypes.POINTER(_Csr)]
     g.xbgen_sample_sources.argtypes = [I32, _P32, I32, U64, I32, _P32]
@@ -466,6 +570,17 @@ def gen_rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: i
                       ctypes.byref(out))
     if rc != 0:
         raise XbError(XB_ERR_INVALID_ARG, f"xbgen_rmat failed ({rc})")
+    return _take(out)
+
+
+def gen_rmat_capped(scale: int, n: int, draws: int, a: float, b: float, c: float, seed: int,
+                    symmetric: bool = False, threads: int = 0) -> Csr:
+    """R-MAT over 2**scale ids keeping the draws with both endpoints below n."""
+    g = load_gen()
+    out = _Csr()
+    rc = g.xbgen_rmat_capped(scale, n, draws, a, b, c, seed, 1 if symmetric else 0, threads, ctypes.byref(out))
+    if rc != 0:
+        raise XbError(XB_ERR_INVALID_ARG, f"xbgen_rmat_capped failed ({rc})")
     return _take(out)
 
 
