@@ -314,10 +314,14 @@ def main() -> int:
     ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven", "serving"])
     ap.add_argument("--requests", type=int, default=400, help="serving: timed queries")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (queries/s)")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent tiles")
-    ap.add_argument("--engine", default="hostloop", choices=["hostloop", "persistent"],
+    ap.add_argument("--slots", type=int, default=2,
+                    help="serving: concurrent tiles (2 measured best on the 20M-vertex graph: the table budget caps rows in flight)")
+    ap.add_argument("--engine", default="auto", choices=["auto", "hostloop", "persistent"],
                     help="hostloop: one launch per phase, host-driven level loop; persistent: the batch API "
-                         "(one persistent kernel, level loop on the device, replayed hipGraph)")
+                         "(one persistent kernel, level loop on the device, replayed hipGraph); auto: time "
+                         "both with the tuned options and keep the faster")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (words per tile, alpha, pull cut-offs) before the timed region")
     ap.add_argument("--words-per-tile", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -353,13 +357,40 @@ def main() -> int:
     tun = {}
     if args.words_per_tile:
         tun["words_per_tile"] = args.words_per_tile
+    tune_info = None
+    if not args.no_autotune:
+        t1 = time.perf_counter()
+        best, rep = xb.autotune(hg, src, **tun)
+        tune_info = {"seconds": round(time.perf_counter() - t1, 2), "trials": rep.trials, "passes": rep.passes,
+                     "default_ms_per_source": round(rep.default_ms_per_source, 5),
+                     "best_ms_per_source": round(rep.best_ms_per_source, 5), "chosen": best}
+        tun = best
+        log(f"autotune: {tune_info}")
     lv = torch.empty((S, n), dtype=torch.int32, device="cuda")
     pa = torch.empty((S, n), dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
 
     batch = None
-    if args.engine == "persistent":
-        batch = xb.Batch([hg], [src], [lv.data_ptr()], [pa.data_ptr()], **tun)
+    engine = args.engine
+    if engine in ("persistent", "auto"):
+        batch = xb.Batch([hg], [src], [lv.data_ptr()], [pa.data_ptr()], profile=True, **tun)
+    engine_ms = {}
+    if engine == "auto":
+        for name in ("hostloop", "persistent"):
+            best_ms = 1e30
+            for _ in range(3):
+                if name == "persistent":
+                    best_ms = min(best_ms, batch.run().ms_total)
+                else:
+                    eng.MultiSourceBFS(src, lv.data_ptr(), pa.data_ptr(), mode=xb.XB_MODE_BATCHED, device_out=True,
+                                       **tun)
+                    best_ms = min(best_ms, eng.last_stats.ms_total)
+            engine_ms[name] = round(best_ms, 3)
+        engine = min(engine_ms, key=engine_ms.get)
+        log(f"engine choice: {engine_ms} -> {engine}")
+    if engine != "persistent" and batch is not None:
+        batch.close()
+        batch = None
 
     def run_batched(profile: bool):
         if batch is not None:
@@ -474,8 +505,10 @@ def main() -> int:
             "graph_checksum": f"{g.checksum():016x}",
             "edges_traversed_per_step": int(edges),
             "max_level": last["max_level"] if last else None,
-            "words_per_tile": args.words_per_tile or 8,
-            "engine": args.engine,
+            "words_per_tile": tun.get("words_per_tile", args.words_per_tile or 8),
+            "engine": engine,
+            "engine_ms": engine_ms or None,
+            "autotune": tune_info,
         },
         "plain_per_source": plain,
         "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,

@@ -208,6 +208,32 @@ int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes);
 /* Copies the last error message of the calling thread into buf. */
 int32_t xb_last_error(char* buf, size_t len);
 
+/* ---- autotuner ---------------------------------------------------------
+ * No counterpart in the reference, whose thresholds are hand-set constants
+ * (the |E|/20 push/pull switch of apps/ligra/ligra/ligra.h:236-277 and the
+ * XB engine's fixed launch shapes).  Coordinate descent over words per
+ * tile, the push/pull switch alpha and the pull bucket cut-offs: each trial
+ * runs one tile of the caller's sample sources on the device and is scored
+ * in milliseconds per source (best of two runs); a value replaces the
+ * current one only if it is at least 2% faster.  The sample's results are
+ * discarded.  *best receives the tuned options (mode XB_MODE_BATCHED). */
+typedef struct xb_tune_trial {
+  int32_t words_per_tile, alpha, pull_cut_small, pull_cut_medium, pull_cut_hub, hub_chunk;
+  double ms_per_source;
+} xb_tune_trial;
+
+#define XB_TUNE_MAX_TRIALS 128
+typedef struct xb_tune_report {
+  int32_t trials;                         /* entries of trial[] filled      */
+  int32_t passes;
+  double default_ms_per_source;           /* the base options' score        */
+  double best_ms_per_source;
+  xb_tune_trial trial[XB_TUNE_MAX_TRIALS];
+} xb_tune_report;
+
+int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, const xb_msbfs_opts* base,
+                    xb_msbfs_opts* best, xb_tune_report* report);
+
 /* ---- serving: continuous batching over one device-resident graph -------
  * No counterpart in the reference, whose drivers run one dataset and one
  * source list per process (apps/xb/src/run_xb_pp_by_dataset.cu); these

@@ -139,6 +139,18 @@ class XBlossomGpuEngine10 {
                          &stats_));
   }
 
+  // Tunes words per tile, alpha and the pull cut-offs on a sample of sources
+  // (xb_autotune); later searches use the tuned options.
+  xb_tune_report Autotune(const std::vector<int32_t>& sample) {
+    xb_tune_report rep{};
+    xb_msbfs_opts best{};
+    XbCheck(xb_autotune(graph_, sample.data(), (int32_t)sample.size(), &opts_, &best, &rep));
+    best.flags = opts_.flags;
+    best.mode = opts_.mode;
+    opts_ = best;
+    return rep;
+  }
+
   // Device tables (caller-allocated, S x n int32 each).
   void MultiSourceBFSDevice(const int32_t* sources, int32_t S, int32_t* levels_dev, int32_t* parents_dev) {
     xb_msbfs_opts o = opts_;

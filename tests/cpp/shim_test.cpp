@@ -50,6 +50,8 @@ int main(int argc, char** argv) {
 
   zblossom::XBlossomGpuEngine10 engine(graph, 80);
   std::vector<int32_t> sources = {0, n - 1, (R / 2) * C + C / 2, 5, n - 7};
+  const xb_tune_report rep = engine.Autotune(sources);
+  EXPECT(rep.trials >= 20 && rep.best_ms_per_source <= rep.default_ms_per_source);
   for (int mode : {XB_MODE_BATCHED, XB_MODE_PLAIN}) {
     engine.Options().mode = mode;
     std::vector<int32_t> lv, pa;

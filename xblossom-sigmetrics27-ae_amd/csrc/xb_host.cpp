@@ -1496,6 +1496,124 @@ int32_t xb_msbfs_run_batched(xb_graph* const* graphs, int32_t G, const int32_t* 
 }  // extern "C"
 
 // ===========================================================================
+// Autotuner: coordinate descent over the engine's tunables
+// ===========================================================================
+namespace {
+
+struct TuneRun {
+  xb_graph* g;
+  const int32_t* sample;
+  int32_t S;
+  int32_t* dl = nullptr;
+  int32_t* dp = nullptr;
+  xb_tune_report* rep;
+
+  // Milliseconds per source of one tile of the sample (best of two runs,
+  // after one untimed run that also rebuilds the layout for new cut-offs).
+  double measure(const xb_msbfs_opts& o) {
+    const BPOpts bo = bp_opts(o);
+    const int32_t cnt = std::min(S, 64 * bo.T);
+    Profiler prof;
+    prof.st = g->stream;
+    run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr);
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    double best = 1e300;
+    for (int rep_i = 0; rep_i < 2; ++rep_i) {
+      HIPCHK(hipEventRecord(a, g->stream));
+      run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr);
+      HIPCHK(hipEventRecord(b, g->stream));
+      HIPCHK(hipEventSynchronize(b));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, (double)ms / cnt);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rep && rep->trials < XB_TUNE_MAX_TRIALS) {
+      xb_tune_trial& t = rep->trial[rep->trials++];
+      t.words_per_tile = o.words_per_tile;
+      t.alpha = o.alpha;
+      t.pull_cut_small = o.pull_cut_small;
+      t.pull_cut_medium = o.pull_cut_medium;
+      t.pull_cut_hub = o.pull_cut_hub;
+      t.hub_chunk = o.hub_chunk;
+      t.ms_per_source = best;
+    }
+    return best;
+  }
+};
+
+bool valid_tunables(const xb_msbfs_opts& o) {
+  return o.alpha >= 1 && o.pull_cut_small >= 1 && o.pull_cut_medium >= o.pull_cut_small &&
+         o.pull_cut_hub >= o.pull_cut_medium && o.hub_chunk >= 1;
+}
+
+}  // namespace
+
+extern "C" int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, const xb_msbfs_opts* base,
+                               xb_msbfs_opts* best, xb_tune_report* report) {
+  return guarded([&] {
+    if (!g || !sample || S < 1 || !best) fail(XB_ERR_INVALID_ARG, "bad argument");
+    check_sources(g, sample, S);
+    xb_msbfs_opts cur = parse_opts(base);
+    cur.mode = XB_MODE_BATCHED;
+    cur.flags = XB_OUT_DEVICE;
+    if (report) std::memset(report, 0, sizeof(*report));
+    TuneRun tr{g, sample, S, nullptr, nullptr, report};
+    const int32_t rows = std::min(S, 512);
+    struct Free {
+      TuneRun& t;
+      ~Free() {
+        dfree(t.dl);
+        dfree(t.dp);
+      }
+    } freer{tr};
+    tr.dl = dalloc<int32_t>((size_t)rows * (size_t)g->n);
+    tr.dp = dalloc<int32_t>((size_t)rows * (size_t)g->n);
+    double t_cur = tr.measure(cur);
+    if (report) report->default_ms_per_source = t_cur;
+    // Candidate values per tunable, tried in this order.
+    struct Axis {
+      int32_t xb_msbfs_opts::*field;
+      std::vector<int32_t> values;
+    };
+    const std::vector<Axis> axes = {
+        {&xb_msbfs_opts::words_per_tile, {1, 2, 4, 8}},
+        {&xb_msbfs_opts::alpha, {6, 10, 14, 20, 28, 40, 64}},
+        {&xb_msbfs_opts::pull_cut_small, {4, 8, 16}},
+        {&xb_msbfs_opts::pull_cut_medium, {32, 64, 128, 256}},
+        {&xb_msbfs_opts::pull_cut_hub, {512, 1024, 2048, 4096}},
+        {&xb_msbfs_opts::hub_chunk, {256, 512, 1024, 2048}},
+    };
+    int passes = 0;
+    for (bool changed = true; changed && passes < 2; ++passes) {
+      changed = false;
+      for (const Axis& ax : axes)
+        for (int32_t v : ax.values) {
+          if (cur.*(ax.field) == v) continue;
+          xb_msbfs_opts trial = cur;
+          trial.*(ax.field) = v;
+          if (!valid_tunables(trial)) continue;
+          const double t = tr.measure(trial);
+          if (t < 0.98 * t_cur) {
+            cur = trial;
+            t_cur = t;
+            changed = true;
+          }
+        }
+    }
+    if (report) {
+      report->passes = passes;
+      report->best_ms_per_source = t_cur;
+    }
+    *best = cur;
+    best->flags = base ? base->flags : 0u;
+  });
+}
+
+// ===========================================================================
 // Serving: continuous batching over one device-resident graph
 // ===========================================================================
 namespace {

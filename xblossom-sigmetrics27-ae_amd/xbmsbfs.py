@@ -70,6 +70,7 @@ EXPORTED_SYMBOLS = (
     "xb_memcpy_d2h",
     "xb_memcpy_h2d",
     "xb_last_error",
+    "xb_autotune",
     "xb_server_opts_default",
     "xb_server_create",
     "xb_server_submit",
@@ -137,6 +138,25 @@ class RunStats(ctypes.Structure):
              if k not in ("reserved", "kernel_ms", "kernel_bytes", "kernel_launches")}
         d["kernels"] = self.kernels()
         return d
+
+
+TUNABLES = ("words_per_tile", "alpha", "pull_cut_small", "pull_cut_medium", "pull_cut_hub", "hub_chunk")
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in TUNABLES] + [("ms_per_source", ctypes.c_double)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("trials", ctypes.c_int32), ("passes", ctypes.c_int32),
+                ("default_ms_per_source", ctypes.c_double), ("best_ms_per_source", ctypes.c_double),
+                ("trial", TuneTrial * 128)]
+
+    def as_dict(self) -> dict:
+        return {"passes": self.passes, "default_ms_per_source": self.default_ms_per_source,
+                "best_ms_per_source": self.best_ms_per_source,
+                "trials": [dict({k: getattr(t, k) for k in TUNABLES}, ms_per_source=t.ms_per_source)
+                           for t in self.trial[: self.trials]]}
 
 
 class ServerOpts(ctypes.Structure):
@@ -208,6 +228,8 @@ def load_engine():
     lib.xb_memcpy_d2h.argtypes = [VP, VP, ctypes.c_size_t]
     lib.xb_memcpy_h2d.argtypes = [VP, VP, ctypes.c_size_t]
     lib.xb_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.xb_autotune.argtypes = [VP, _P32, I32, ctypes.POINTER(Opts), ctypes.POINTER(Opts),
+                                ctypes.POINTER(TuneReport)]
     lib.xb_server_opts_default.argtypes = [ctypes.POINTER(ServerOpts)]
     lib.xb_server_create.argtypes = [VP, ctypes.POINTER(ServerOpts), ctypes.POINTER(Opts), ctypes.POINTER(VP)]
     lib.xb_server_submit.argtypes = [VP, _P32, I32, ctypes.POINTER(ctypes.c_uint64)]
@@ -278,6 +300,14 @@ class XBlossomGpuEngine10:
         self.graph = graph
         self.path_table_buffer_ratio = path_table_buffer_ratio
         self.last_stats = RunStats()
+        self.tunables = {}  # set by Autotune; explicit keyword tunables override them
+
+    def Autotune(self, sample, **base) -> TuneReport:
+        """Tunes words per tile, alpha and the pull cut-offs on one tile of
+        ``sample`` sources; later MultiSourceBFS calls use the result."""
+        best, rep = autotune(self.graph, sample, **base)
+        self.tunables = best
+        return rep
 
     def MultiSourceBFS(self, sources, levels=None, parents=None, *, mode: int = XB_MODE_AUTO,
                        device_out: bool = False, profile: bool = False, want_levels: bool = True,
@@ -299,7 +329,7 @@ class XBlossomGpuEngine10:
                 parents = np.empty((S, n), dtype=np.int32)
             if levels is None and want_levels:
                 levels = np.empty((S, n), dtype=np.int32)
-        o = default_opts(mode=mode, flags=flags, **tunables)
+        o = default_opts(mode=mode, flags=flags, **{**self.tunables, **tunables})
         st = RunStats()
         _check(lib.xb_msbfs_run(self.graph.handle, _ptr(src), S, _ptr(levels), _ptr(parents), ctypes.byref(o),
                                 ctypes.byref(st)))
@@ -468,6 +498,17 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def autotune(graph: HGraph, sample, **base):
+    """Runs xb_autotune; returns (tuned tunables as a dict, TuneReport)."""
+    src = np.ascontiguousarray(sample, dtype=np.int32)
+    o = default_opts(**base)
+    best = Opts()
+    rep = TuneReport()
+    _check(load_engine().xb_autotune(graph.handle, _ptr(src), src.size, ctypes.byref(o), ctypes.byref(best),
+                                     ctypes.byref(rep)))
+    return {k: int(getattr(best, k)) for k in TUNABLES}, rep
 
 
 def device_checksum(ptr: int, count: int) -> int:
