@@ -474,6 +474,7 @@ void ensure_bp(const xb_graph* g, Workspace& w, int T) {
   w.bp.tflag = dalloc<uint32_t>((size_t)g->n + 1);
   w.bp.tlist = dalloc<int32_t>((size_t)g->n + 1);
   w.bp.tcount = w.tcount;
+  w.bp.n = g->n;
   w.T = T;
   w.bp_dirty = true;
 }
@@ -1316,6 +1317,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     tm.s.front[1] = b->alloc<uint64_t>(nw);
     tm.s.nextw = b->alloc<uint64_t>(nw);
     tm.s.planes = b->alloc<uint64_t>(nw * 8);
+    tm.s.n = g->n;
     tm.s.tflag = b->alloc<uint32_t>((size_t)g->n + 1);
     tm.s.tlist = b->alloc<int32_t>((size_t)g->n + 1);
     tm.s.hpar = b->alloc<uint32_t>((size_t)std::max(g->n_heavy, 1) * tw * 64);

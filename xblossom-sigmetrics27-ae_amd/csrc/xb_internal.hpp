@@ -8,11 +8,17 @@
 
 namespace xb {
 
-// Bit-parallel tile state.  All arrays are vertex-major with T 64-bit source
-// words per vertex ([v*T + t]); planes hold 8 bit-sliced planes of the level
-// code (code = level + 1, 0 = unreached) per (vertex, word):
-// planes[(v*T + t)*8 + k] bit b = bit k of code(source 64t+b, v).
+// Bit-parallel tile state.  seen, front and nextw are vertex-major with T
+// 64-bit source words per vertex ([v*T + t]).  planes hold 8 bit-sliced
+// planes of the level code (code = level + 1, 0 = unreached), bit b of plane
+// k = bit k of code(source 64t+b, v), stored word-major in two halves:
+//   plane k < 4:   planes[(t*n + v)*4 + k]
+//   plane k >= 4:  planes[T*n*4 + (t*n + v)*4 + k - 4]
+// so that the emit of word t with up to 4 planes (levels <= 14) reads one
+// contiguous n x 32 B block, which stays in the 256 MB Infinity Cache for
+// graphs of up to ~8M vertices while its in-neighbour reads hop at random.
 struct BPState {
+  int64_t n;            // vertices (plane addressing)
   uint64_t* seen;
   uint64_t* front[2];
   uint64_t* nextw;

@@ -352,11 +352,42 @@ __device__ __forceinline__ uint64_t word_mask(int32_t cnt, int t) {
   return nb >= 64 ? ~0ull : (nb <= 0 ? 0ull : ((1ull << nb) - 1ull));
 }
 
-// OR `bits` into the planes whose bit is set in `code`.
-__device__ __forceinline__ void planes_or(uint64_t* __restrict__ p, uint32_t code, uint64_t bits) {
+// Level-code planes of (v, word t) in the word-major split layout.
+template <int T>
+__device__ __forceinline__ uint64_t* plane_lo(const BPState& s, int64_t v, int t) {
+  return s.planes + ((int64_t)t * s.n + v) * 4;
+}
+template <int T>
+__device__ __forceinline__ uint64_t* plane_hi(const BPState& s, int64_t v, int t) {
+  return s.planes + (int64_t)T * s.n * 4 + ((int64_t)t * s.n + v) * 4;
+}
+
+// OR `bits` into the planes of (v, t) whose bit is set in `code`.
+template <int T>
+__device__ __forceinline__ void planes_or(const BPState& s, int64_t v, int t, uint32_t code, uint64_t bits) {
+  uint64_t* lo = plane_lo<T>(s, v, t);
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    if ((code >> k) & 1u) p[k] |= bits;
+  for (int k = 0; k < 4; ++k)
+    if ((code >> k) & 1u) lo[k] |= bits;
+  if (code >> 4) {
+    uint64_t* hi = plane_hi<T>(s, v, t);
+#pragma unroll
+    for (int k = 4; k < 8; ++k)
+      if ((code >> k) & 1u) hi[k - 4] |= bits;
+  }
+}
+
+// The first K planes of (v, t).
+template <int T, int K>
+__device__ __forceinline__ void load_planes(const BPState& s, int64_t v, int t, uint64_t (&pk)[K]) {
+  const uint64_t* lo = plane_lo<T>(s, v, t);
+#pragma unroll
+  for (int k = 0; k < (K < 4 ? K : 4); ++k) pk[k] = lo[k];
+  if (K > 4) {
+    const uint64_t* hi = plane_hi<T>(s, v, t);
+#pragma unroll
+    for (int k = 4; k < K; ++k) pk[k] = hi[k - 4];
+  }
 }
 
 __device__ void fill_u64(Grid g, uint64_t* __restrict__ p, int64_t count, uint64_t val) {
@@ -376,7 +407,7 @@ __device__ void seed_phase(Grid g, BPState s, const int32_t* __restrict__ src, i
     const int64_t w = (int64_t)v * T + t;
     atomicOr((unsigned long long*)&s.seen[w], (unsigned long long)bit);
     atomicOr((unsigned long long*)&s.front[0][w], (unsigned long long)bit);
-    atomicOr((unsigned long long*)&s.planes[w * 8 + 0], (unsigned long long)bit);  // code 1 = level 0
+    atomicOr((unsigned long long*)plane_lo<T>(s, v, t), (unsigned long long)bit);  // code 1 = level 0
   }
 }
 
@@ -446,7 +477,7 @@ __device__ void commit_phase(Grid g, BPState s, int cur, uint32_t code,
         if (!nb) continue;
         s.seen[w] = old | nb;
         fnext[w] = nb;
-        planes_or(&s.planes[w * 8], code, nb);
+        planes_or<T>(s, u, t, code, nb);
         any = true;
       }
       s.tflag[u] = 0u;
@@ -548,7 +579,7 @@ __device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int
       const int64_t w = (int64_t)v * T + t;
       s.seen[w] = old | got;
       fnext[w] = got;
-      planes_or(&s.planes[w * 8], code, got);
+      planes_or<T>(s, v, t, code, got);
     }
     const uint64_t bal = __ballot(act && p == 0 && got != 0);
     const uint64_t grp = (bal >> (slot * G)) & ((T == 64) ? ~0ull : ((1ull << T) - 1ull));
@@ -673,10 +704,8 @@ __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t*
   const int l = lane_id();
   for (int64_t h = gwave(g); h < n_heavy; h += gnwaves(g)) {
     const int32_t v = heavy[h];
-    const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
     uint64_t pk[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) pk[k] = pv[k];
+    load_planes<T, K>(s, v, word, pk);
     const CodeMasks<K> cm = code_masks<K>(pk);
     uint32_t* hp = s.hpar + ((int64_t)h * T + word) * 64;
     const uint64_t unset = __ballot(hp[l] == 0xFFFFFFFFu);
@@ -688,7 +717,8 @@ __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t*
       int32_t u = 0;
       if (j < e) {
         u = in_col[j];
-        const uint64_t* pu = &s.planes[((int64_t)u * T + word) * 8];
+        uint64_t pu[K];
+        load_planes<T, K>(s, u, word, pu);
         eq = need;
 #pragma unroll
         for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
@@ -734,9 +764,7 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     for (int k = 0; k < K; ++k) pk[k] = 0;
     int32_t b = 0, e = 0;
     if (act) {
-      const uint64_t* pv = &s.planes[((int64_t)v * T + word) * 8];
-#pragma unroll
-      for (int k = 0; k < K; ++k) pk[k] = pv[k];
+      load_planes<T, K>(s, v, word, pk);
       b = in_off[v];
       e = in_off[v + 1];
     }
@@ -762,7 +790,8 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
       for (int i = 0; i < kEmitU; ++i) {
         uint64_t eq = 0;
         if (us[i] >= 0) {
-          const uint64_t* pu = &s.planes[((int64_t)us[i] * T + word) * 8];
+          uint64_t pu[K];
+          load_planes<T, K>(s, us[i], word, pu);
           eq = ~0ull;
 #pragma unroll
           for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
@@ -794,10 +823,8 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     if (v2 < n) {
       // The vertex's plane words, read again (the compute phase just brought
       // them into the cache) rather than kept in LDS.
-      const uint64_t* pq = &s.planes[((int64_t)v2 * T + word) * 8];
       uint64_t q[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) q[k] = pq[k];
+      load_planes<T, K>(s, v2, word, q);
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int bit = 16 * wv + i;
