@@ -185,35 +185,50 @@ struct xb_graph {
   int32_t n = 0;
   int64_t m = 0;
   uint32_t flags = 0;
-  std::vector<int32_t> h_off;     // out-edge offsets (host copy)
-  std::vector<int32_t> h_in_off;  // in-edge offsets (empty when symmetric)
-  int32_t* d_off = nullptr;
+  std::vector<int32_t> h_off;  // out-edge offsets, original ids (host copy)
+  int32_t* d_off = nullptr;    // original-id CSR: plain engine, traversed-edge counts
   int32_t* d_col = nullptr;
-  int32_t* d_in_off = nullptr;  // == d_off when symmetric
-  int32_t* d_in_col = nullptr;  // == d_col when symmetric
+  int32_t* d_oin_off = nullptr;  // original-order in-offsets (the emit); == d_off when symmetric
+  int32_t* d_oin_col = nullptr;  // original-order in-lists with engine-id entries (the emit)
+  // Bit-parallel engine graph on engine ids (descending out-degree order).
+  std::vector<int32_t> h_new_of;    // original id -> engine id
+  std::vector<int32_t> h_ein_off;   // engine in-edge offsets (layout of the pull and emit)
+  int32_t* d_new_of = nullptr;
+  int32_t* d_orig_of = nullptr;
+  int32_t* d_eoff = nullptr;        // engine out-CSR (push); == the in-CSR when symmetric
+  int32_t* d_ecol = nullptr;
+  int32_t* d_ein_off = nullptr;     // engine in-CSR, each list in ascending ORIGINAL id
+  int32_t* d_ein_col = nullptr;     // in-neighbours as engine ids
+  int32_t* d_ein_col_o = nullptr;   // the same in-neighbours as original ids
   hipStream_t stream = nullptr;
   Pinned pinned;
   // In-degree layout for the pull and emit, rebuilt when a cut changes.
   int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1;
-  int32_t* d_perm = nullptr;          // in-degree > 0, bucket-major, ascending id per bucket
+  int32_t* d_perm = nullptr;          // engine ids with in-degree > 0, bucket-major, ascending per bucket
   int32_t bucket_end[3] = {0, 0, 0};  // perm ranges of the small / medium / large buckets
   int64_t bucket_edges[3] = {0, 0, 0};
   xb::HubChunk* d_hub_chunks = nullptr;
   int32_t n_hub_chunks = 0;
   int32_t n_hubs = 0;
   int64_t hub_edges = 0;
-  int32_t* d_heavy = nullptr;  // in-degree > cut_medium, ascending id (the emit heavy list)
-  int32_t* d_hidx = nullptr;   // vertex -> index in d_heavy, else -1
+  int32_t* d_heavy = nullptr;  // engine ids with in-degree > cut_medium, ascending (the emit heavy list)
+  int32_t* d_hidx = nullptr;   // engine id -> index in d_heavy, else -1
   int32_t n_heavy = 0;
   Workspace ws;
 
-  const std::vector<int32_t>& in_off() const { return h_in_off.empty() ? h_off : h_in_off; }
   int32_t outdeg(int32_t v) const { return h_off[v + 1] - h_off[v]; }
 
   ~xb_graph() {
     ws.free_all();
-    if (d_in_col != d_col) dfree(d_in_col);
-    if (d_in_off != d_off) dfree(d_in_off);
+    if (d_eoff != d_ein_off) dfree(d_eoff);
+    if (d_ecol != d_ein_col) dfree(d_ecol);
+    dfree(d_ein_off);
+    dfree(d_ein_col);
+    dfree(d_ein_col_o);
+    dfree(d_new_of);
+    dfree(d_orig_of);
+    if (d_oin_off != d_off) dfree(d_oin_off);
+    dfree(d_oin_col);
     dfree(d_off);
     dfree(d_col);
     dfree(d_perm);
@@ -280,6 +295,80 @@ void validate_csr(const int32_t* off, int64_t n, const int32_t* col, int64_t m, 
   }
 }
 
+// Engine ids: vertices in descending out-degree order, ties by ascending
+// original id (a counting sort on the degree).  The pull and the emit gather
+// per-vertex state of in-neighbours, and an in-neighbour u is gathered once
+// per out-edge of u, so this order packs the state that most gathers hit into
+// the lowest ids, where it stays cached.  In-lists keep ascending ORIGINAL id
+// order and carry the original ids beside the engine ids, so "first provider
+// in list order" remains the lowest original id.
+void build_engine_graph(xb_graph* g, const int32_t* off, const int32_t* col, const int32_t* in_off,
+                        const int32_t* in_col, bool directed) {
+  const int32_t n = g->n;
+  const int64_t m = g->m;
+  int32_t maxd = 0;
+  for (int32_t v = 0; v < n; ++v) maxd = std::max(maxd, off[v + 1] - off[v]);
+  std::vector<int64_t> start((size_t)maxd + 2, 0);
+  for (int32_t v = 0; v < n; ++v) start[(size_t)(maxd - (off[v + 1] - off[v])) + 1]++;
+  for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+  std::vector<int32_t> orig_of((size_t)n);
+  for (int32_t v = 0; v < n; ++v) orig_of[(size_t)start[(size_t)(maxd - (off[v + 1] - off[v]))]++] = v;
+  g->h_new_of.assign((size_t)n, 0);
+  for (int32_t i = 0; i < n; ++i) g->h_new_of[(size_t)orig_of[i]] = i;
+  const std::vector<int32_t>& new_of = g->h_new_of;
+  // Engine CSR from an original-id CSR: row i = the list of orig_of[i], in its
+  // original order, as engine ids (and as original ids when `orig` is given).
+  auto relabel = [&](const int32_t* ro, const int32_t* co, std::vector<int32_t>& eoff, std::vector<int32_t>& ecol,
+                     std::vector<int32_t>* orig) {
+    eoff.assign((size_t)n + 1, 0);
+    for (int32_t i = 0; i < n; ++i) eoff[i + 1] = eoff[i] + (ro[orig_of[i] + 1] - ro[orig_of[i]]);
+    ecol.assign((size_t)std::max<int64_t>(m, 1), 0);
+    if (orig) orig->assign((size_t)std::max<int64_t>(m, 1), 0);
+    parallel_rows(n, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; ++i) {
+        const int32_t o = orig_of[(size_t)i];
+        int32_t p = eoff[(size_t)i];
+        for (int32_t j = ro[o]; j < ro[o + 1]; ++j, ++p) {
+          ecol[(size_t)p] = new_of[(size_t)co[j]];
+          if (orig) (*orig)[(size_t)p] = co[j];
+        }
+      }
+    });
+  };
+  std::vector<int32_t> ein_col, ein_col_o;
+  relabel(in_off, in_col, g->h_ein_off, ein_col, &ein_col_o);
+  g->d_new_of = dalloc<int32_t>((size_t)n);
+  g->d_orig_of = dalloc<int32_t>((size_t)n);
+  h2d(g->d_orig_of, orig_of.data(), sizeof(int32_t) * (size_t)n, g->stream);
+  g->d_ein_off = dalloc<int32_t>((size_t)n + 1);
+  g->d_ein_col = dalloc<int32_t>((size_t)m);
+  g->d_ein_col_o = dalloc<int32_t>((size_t)m);
+  h2d(g->d_new_of, new_of.data(), sizeof(int32_t) * (size_t)n, g->stream);
+  h2d(g->d_ein_off, g->h_ein_off.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
+  h2d(g->d_ein_col, ein_col.data(), sizeof(int32_t) * (size_t)m, g->stream);
+  h2d(g->d_ein_col_o, ein_col_o.data(), sizeof(int32_t) * (size_t)m, g->stream);
+  {
+    // The emit's in-lists: original vertex and list order, engine-id entries.
+    std::vector<int32_t> oin((size_t)std::max<int64_t>(m, 1));
+    parallel_rows(m, [&](int64_t b, int64_t e) {
+      for (int64_t j = b; j < e; ++j) oin[(size_t)j] = new_of[(size_t)in_col[j]];
+    });
+    g->d_oin_col = dalloc<int32_t>((size_t)m);
+    h2d(g->d_oin_col, oin.data(), sizeof(int32_t) * (size_t)m, g->stream);
+  }
+  if (directed) {
+    std::vector<int32_t> eoff, ecol;
+    relabel(off, col, eoff, ecol, nullptr);
+    g->d_eoff = dalloc<int32_t>((size_t)n + 1);
+    g->d_ecol = dalloc<int32_t>((size_t)m);
+    h2d(g->d_eoff, eoff.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
+    h2d(g->d_ecol, ecol.data(), sizeof(int32_t) * (size_t)m, g->stream);
+  } else {
+    g->d_eoff = g->d_ein_off;
+    g->d_ecol = g->d_ein_col;
+  }
+}
+
 xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, int64_t m, uint32_t flags) {
   if (!off || n_plus_1 < 1 || (m > 0 && !col)) fail(XB_ERR_INVALID_ARG, "null or empty CSR arrays");
   if (flags & ~XB_GRAPH_DIRECTED) fail(XB_ERR_INVALID_ARG, "unknown graph flag");
@@ -299,25 +388,27 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
   g->d_col = dalloc<int32_t>((size_t)m);
   h2d(g->d_off, off, sizeof(int32_t) * ((size_t)n + 1), g->stream);
   h2d(g->d_col, colv.data(), sizeof(int32_t) * (size_t)m, g->stream);
+  std::vector<int32_t> ioff, icol;
+  const int32_t* in_off = off;
+  const int32_t* in_col = colv.data();
   if (flags & XB_GRAPH_DIRECTED) {
     // Transpose by counting sort on the destination; sources are visited in
     // ascending order, so every in-list comes out ascending.
-    std::vector<int32_t>& ioff = g->h_in_off;
     ioff.assign((size_t)n + 1, 0);
     for (int64_t j = 0; j < m; ++j) ioff[(size_t)colv[j] + 1]++;
     for (int64_t v = 0; v < n; ++v) ioff[v + 1] += ioff[v];
     std::vector<int32_t> pos(ioff.begin(), ioff.end() - 1);
-    std::vector<int32_t> icol((size_t)std::max<int64_t>(m, 1));
+    icol.assign((size_t)std::max<int64_t>(m, 1), 0);
     for (int64_t u = 0; u < n; ++u)
       for (int32_t j = off[u]; j < off[u + 1]; ++j) icol[pos[colv[j]]++] = (int32_t)u;
-    g->d_in_off = dalloc<int32_t>((size_t)n + 1);
-    g->d_in_col = dalloc<int32_t>((size_t)m);
-    h2d(g->d_in_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
-    h2d(g->d_in_col, icol.data(), sizeof(int32_t) * (size_t)m, g->stream);
+    in_off = ioff.data();
+    in_col = icol.data();
+    g->d_oin_off = dalloc<int32_t>((size_t)n + 1);
+    h2d(g->d_oin_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
   } else {
-    g->d_in_off = g->d_off;
-    g->d_in_col = g->d_col;
+    g->d_oin_off = g->d_off;
   }
+  build_engine_graph(g.get(), off, colv.data(), in_off, in_col, (flags & XB_GRAPH_DIRECTED) != 0);
   return g.release();
 }
 
@@ -376,7 +467,7 @@ struct BPOpts {
 // In-degree layout: bucket order for the pull, hub pieces, heavy emit list
 // ---------------------------------------------------------------------------
 void ensure_layout(xb_graph* g, const BPOpts& o) {
-  const auto& io = g->in_off();
+  const auto& io = g->h_ein_off;
   const int32_t n = g->n;
   if (o.cut_small != g->cut_small || o.cut_medium != g->cut_medium || o.cut_hub != g->cut_hub ||
       o.hub_chunk != g->hub_chunk) {
@@ -498,6 +589,9 @@ void ensure_hpar(const xb_graph* g, Workspace& w, int T) {
   }
   w.bp.hpar = (uint32_t*)w.side;
   w.bp.hidx = g->d_hidx;
+  w.bp.new_of = g->d_new_of;
+  w.bp.orig_of = g->d_orig_of;
+  w.bp.in_col_o = g->d_ein_col_o;
 }
 
 int32_t* ensure_stage(Workspace& w, int64_t ints) {
@@ -680,7 +774,7 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   int32_t* q0 = ensure_q0_host(w, 2 * (int64_t)qn0 + cnt);
   int64_t E0 = 0;
   for (int32_t i = 0; i < qn0; ++i) {
-    q0[i] = uniq[i];
+    q0[i] = g->h_new_of[(size_t)uniq[i]];  // the frontier holds engine ids
     q0[qn0 + i] = (int32_t)E0;
     E0 += g->outdeg(uniq[i]);
   }
@@ -714,7 +808,7 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
         const int32_t end = g->bucket_end[c];
         if (end > begin) {
           prof.begin(XB_K_PULL);
-          xb::launch_bp_pull_bucket(w.st, T, c, s, cur, code, cnt, g->d_in_off, g->d_in_col, g->d_off,
+          xb::launch_bp_pull_bucket(w.st, T, c, s, cur, code, cnt, g->d_ein_off, g->d_ein_col, g->d_eoff,
                                     g->d_perm, begin, end, qx);
           prof.end();
           // offsets + in-edges + (front, seen r/w) per vertex and word
@@ -724,22 +818,22 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
       }
       if (g->n_hub_chunks > 0) {
         prof.begin(XB_K_PULL_HUB);
-        xb::launch_bp_pull_hub(w.st, T, s, cur, cnt, g->d_in_col, g->d_hub_chunks, g->n_hub_chunks);
+        xb::launch_bp_pull_hub(w.st, T, s, cur, cnt, g->d_ein_col, g->d_hub_chunks, g->n_hub_chunks);
         prof.end();
         prof.add(XB_K_PULL_HUB, 4.0 * g->hub_edges + 24.0 * T * g->n_hubs);
         prof.begin(XB_K_COMMIT);
-        xb::launch_bp_commit(w.st, T, s, cur, code, g->d_off, qx, g->n_hubs);
+        xb::launch_bp_commit(w.st, T, s, cur, code, g->d_eoff, qx, g->n_hubs);
         prof.end();
       }
       ++pulls;
     } else {
       prof.begin(XB_K_PUSH);
-      xb::launch_bp_push(w.st, T, s, cur, g->d_off, g->d_col, qc, qn, E);
+      xb::launch_bp_push(w.st, T, s, cur, g->d_eoff, g->d_ecol, qc, qn, E);
       prof.end();
       // queue + frontier words of the queue, then col + seen words per edge
       prof.add(XB_K_PUSH, (8.0 + 8.0 * T) * qn + (4.0 + 8.0 * T) * E);
       prof.begin(XB_K_COMMIT);
-      xb::launch_bp_commit(w.st, T, s, cur, code, g->d_off, qx, (int32_t)std::min<int64_t>(E, n));
+      xb::launch_bp_commit(w.st, T, s, cur, code, g->d_eoff, qx, (int32_t)std::min<int64_t>(E, n));
       prof.end();
       ++pushes;
     }
@@ -762,11 +856,11 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   for (int t = 0; t < words; ++t) {
     if (g->n_heavy > 0) {
       prof.begin(XB_K_EMIT_HEAVY);
-      xb::launch_bp_emit_heavy(w.st, T, s, t, K, g->d_in_off, g->d_in_col, g->d_heavy, g->n_heavy);
+      xb::launch_bp_emit_heavy(w.st, T, s, t, K, g->d_ein_off, g->d_ein_col, g->d_heavy, g->n_heavy);
       prof.end();
     }
     prof.begin(XB_K_EMIT);
-    xb::launch_bp_emit(w.st, T, s, t, K, cnt, w.d_src, g->d_in_off, g->d_in_col, n, o.cut_medium, lvl_rows,
+    xb::launch_bp_emit(w.st, T, s, t, K, cnt, w.d_src, g->d_oin_off, g->d_oin_col, n, o.cut_medium, lvl_rows,
                        par_rows, rows);
     prof.end();
     const int rows_w = std::min(64, cnt - 64 * t);
@@ -1156,10 +1250,15 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     b->graphs.push_back(g);
     ensure_layout(g, bo);
     xb::GraphView& v = views[i];
-    v.off = g->d_off;
-    v.col = g->d_col;
-    v.in_off = g->d_in_off;
-    v.in_col = g->d_in_col;
+    v.off = g->d_eoff;
+    v.col = g->d_ecol;
+    v.in_off = g->d_ein_off;
+    v.in_col = g->d_ein_col;
+    v.in_col_o = g->d_ein_col_o;
+    v.new_of = g->d_new_of;
+    v.orig_of = g->d_orig_of;
+    v.oin_off = g->d_oin_off;
+    v.oin_col = g->d_oin_col;
     v.perm = g->d_perm;
     for (int c = 0; c < 3; ++c) v.bucket_end[c] = g->bucket_end[c];
     v.n_hub_chunks = g->n_hub_chunks;
@@ -1198,6 +1297,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
         E += graphs[i]->outdeg(uniq[k]);
       }
       th.E0 = (int32_t)E;
+      for (int32_t& u : uniq) u = graphs[i]->h_new_of[(size_t)u];  // the frontier holds engine ids
       packed.insert(packed.end(), uniq.begin(), uniq.end());
       packed.insert(packed.end(), eo.begin(), eo.end());
       tiles.push_back(th);

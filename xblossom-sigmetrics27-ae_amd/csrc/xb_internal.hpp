@@ -8,8 +8,20 @@
 
 namespace xb {
 
+// The traversal works on engine ids: vertices relabelled by descending
+// out-degree (ties by original id), so the frontier words of the hubs that
+// most in-edges point at sit together at the low ids and stay in cache.
+// Engine in-edge lists keep ascending ORIGINAL id order, and parents are
+// recorded as original ids, so "first provider in list order" is still the
+// lowest original id.  All tile state, the level-code planes included, is
+// on engine ids.  The emit walks vertices in output (original) order over an
+// original-order in-CSR whose entries are engine ids, so its in-list reads
+// are sequential streams and its neighbour-plane reads hit the clustered
+// hubs; a parent found there is translated back with orig_of.
+//
 // Bit-parallel tile state.  seen, front and nextw are vertex-major with T
-// 64-bit source words per vertex ([v*T + t]).  planes hold 8 bit-sliced
+// 64-bit source words per vertex ([v*T + t], engine ids).  planes hold 8
+// bit-sliced
 // planes of the level code (code = level + 1, 0 = unreached), bit b of plane
 // k = bit k of code(source 64t+b, v), stored word-major in two halves:
 //   plane k < 4:   planes[(t*n + v)*4 + k]
@@ -26,8 +38,11 @@ struct BPState {
   uint32_t* tflag;
   int32_t* tlist;       // vertices whose next words were OR-ed atomically
   uint32_t* tcount;     // device count of tlist
-  uint32_t* hpar;       // heavy-vertex parents [(h*T + t)*64 + bit], 0xFFFFFFFF = unresolved
+  uint32_t* hpar;       // heavy-vertex parents (original ids) [(h*T + t)*64 + bit], 0xFFFFFFFF = unresolved
   const int32_t* hidx;  // vertex -> heavy index (in-degree above the medium pull cut), else -1
+  const int32_t* new_of;    // original id -> engine id (degree order)
+  const int32_t* orig_of;   // engine id -> original id
+  const int32_t* in_col_o;  // original id of each in-edge's source, parallel to the engine's in_col
 };
 
 // Frontier queue: vertex ids plus the exclusive prefix of their out-degrees
@@ -47,10 +62,15 @@ struct HubChunk {
 // ---- persistent batch kernel ----
 // Read-only view of one device graph and its in-degree layout.
 struct GraphView {
-  const int32_t* off;
+  const int32_t* off;  // engine-id out-CSR (push)
   const int32_t* col;
-  const int32_t* in_off;
+  const int32_t* in_off;  // engine-id in-CSR, lists in ascending original id (pull, emit)
   const int32_t* in_col;
+  const int32_t* in_col_o;  // original ids parallel to in_col
+  const int32_t* new_of;    // original id -> engine id
+  const int32_t* orig_of;   // engine id -> original id
+  const int32_t* oin_off;   // in-CSR in original vertex and list order, entries as engine ids (emit)
+  const int32_t* oin_col;
   const int32_t* perm;
   int32_t bucket_end[3];
   int32_t n_hub_chunks;

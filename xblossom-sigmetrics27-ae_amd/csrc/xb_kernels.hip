@@ -140,6 +140,7 @@ struct QueueLds {
 
 struct EmitLds {
   int32_t tile[64][65];
+  uint64_t pl[64][4];  // the block's vertices' first 4 code planes, compute -> write phase
 };
 
 union Lds {
@@ -401,7 +402,8 @@ __device__ void fill_u32(Grid g, uint32_t* __restrict__ p, int64_t count, uint32
 template <int T>
 __device__ void seed_phase(Grid g, BPState s, const int32_t* __restrict__ src, int32_t cnt) {
   for (int64_t i = gthread(g); i < cnt; i += gnthreads(g)) {
-    const int32_t v = src[i];
+    const int32_t vo = src[i];
+    const int32_t v = s.new_of[vo];
     const int t = (int)(i >> 6);
     const uint64_t bit = 1ull << (i & 63);
     const int64_t w = (int64_t)v * T + t;
@@ -564,10 +566,13 @@ __device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int
           uint64_t excl = __shfl_up(incl, T, kWave);
           if (p == 0) excl = 0;
           uint64_t first = c & ~excl;
-          while (first) {
-            const int bit = __builtin_ctzll(first);
-            first &= first - 1;
-            hp[bit] = (uint32_t)us[i];
+          if (first) {
+            const uint32_t uo = (uint32_t)s.in_col_o[b + k + i * P + p];
+            while (first) {
+              const int bit = __builtin_ctzll(first);
+              first &= first - 1;
+              hp[bit] = uo;
+            }
           }
           const uint64_t all = __shfl(incl, slot * G + (P - 1) * T + t, kWave);
           got |= all;
@@ -633,10 +638,13 @@ __device__ void pull_hub_phase(Grid g, BPState s, int cur, int32_t cnt, const in
         uint64_t excl = __shfl_up(incl, T, kWave);
         if (p == 0) excl = 0;
         uint64_t first = cbits & ~excl;
-        while (first) {
-          const int bit = __builtin_ctzll(first);
-          first &= first - 1;
-          atomicMin(&hp[bit], (uint32_t)us[i]);
+        if (first) {
+          const uint32_t uo = (uint32_t)s.in_col_o[k + i * P + p];
+          while (first) {
+            const int bit = __builtin_ctzll(first);
+            first &= first - 1;
+            atomicMin(&hp[bit], uo);
+          }
         }
         const uint64_t all = __shfl(incl, (P - 1) * T + t, kWave);
         got |= all;
@@ -714,9 +722,8 @@ __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t*
     for (int32_t j0 = b; j0 < e && need; j0 += kWave) {
       const int32_t j = j0 + l;
       uint64_t eq = 0;
-      int32_t u = 0;
       if (j < e) {
-        u = in_col[j];
+        const int32_t u = in_col[j];
         uint64_t pu[K];
         load_planes<T, K>(s, u, word, pu);
         eq = need;
@@ -725,10 +732,13 @@ __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t*
       }
       const uint64_t ex = wave_excl_or(eq);
       uint64_t first = eq & ~ex;
-      while (first) {
-        const int bit = __builtin_ctzll(first);
-        first &= first - 1;
-        hp[bit] = (uint32_t)u;
+      if (first) {
+        const uint32_t uo = (uint32_t)s.in_col_o[j];
+        while (first) {
+          const int bit = __builtin_ctzll(first);
+          first &= first - 1;
+          hp[bit] = uo;
+        }
       }
       const uint64_t all = __shfl(ex | eq, kWave - 1, kWave);
       need &= ~all;
@@ -762,17 +772,22 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     uint64_t pk[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) pk[k] = 0;
-    int32_t b = 0, e = 0;
+    int32_t b = 0, e = 0, vn = 0;
     if (act) {
-      load_planes<T, K>(s, v, word, pk);
+      vn = s.new_of[v];
+      load_planes<T, K>(s, vn, word, pk);
       b = in_off[v];
       e = in_off[v + 1];
+    }
+    if (p == 0) {
+#pragma unroll
+      for (int k = 0; k < (K < 4 ? K : 4); ++k) L.pl[vi][k] = pk[k];
     }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
     uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
     if (heavy) {
-      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[v] * T + word) * 64;
+      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[vn] * T + word) * 64;
 #pragma unroll
       for (int i = 0; i < 16; ++i) L.tile[16 * p + i][vi] = (int32_t)hp[16 * p + i];
     }
@@ -809,10 +824,13 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
         uint64_t excl = __shfl_up(incl, 1, kWave);
         if (p == 0) excl = 0;
         uint64_t first = c & ~excl;
-        while (first) {
-          const int bit = __builtin_ctzll(first);
-          first &= first - 1;
-          L.tile[bit][vi] = us[i];
+        if (first) {
+          const int32_t uo = s.orig_of[us[i]];
+          while (first) {
+            const int bit = __builtin_ctzll(first);
+            first &= first - 1;
+            L.tile[bit][vi] = uo;
+          }
         }
         need &= ~__shfl(incl, (l & ~3) | 3, kWave);
       }
@@ -821,10 +839,14 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     // ---- write phase ----
     const int32_t v2 = (int32_t)(vb + l);
     if (v2 < n) {
-      // The vertex's plane words, read again (the compute phase just brought
-      // them into the cache) rather than kept in LDS.
+      // The vertex's plane words: from LDS for up to 4 planes, else read again.
       uint64_t q[K];
-      load_planes<T, K>(s, v2, word, q);
+      if (K <= 4) {
+#pragma unroll
+        for (int k = 0; k < (K < 4 ? K : 4); ++k) q[k] = L.pl[l][k];
+      } else {
+        load_planes<T, K>(s, s.new_of[v2], word, q);
+      }
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int bit = 16 * wv + i;
@@ -898,8 +920,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t wor
   emit_heavy_phase<T, K>(self_grid(), s, word, in_off, in_col, heavy, n_heavy);
 }
 
+// Minimum workgroups per CU the standalone emit's register budget is sized
+// for: 8 (64 VGPRs) hides more of its dependent gathers than the 6 waves per
+// SIMD its natural 76 VGPRs allow (C3: 27.5 -> 24.8 ms of emit per step).
+#ifndef XB_EMIT_MIN_BLOCKS
+#define XB_EMIT_MIN_BLOCKS 8
+#endif
+
 template <int T, int K>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
+__global__ __launch_bounds__(kBlock, XB_EMIT_MIN_BLOCKS) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
                                                           const int32_t* __restrict__ src,
                                                           const int32_t* __restrict__ in_off,
                                                           const int32_t* __restrict__ in_col, int32_t n,
@@ -1087,7 +1116,7 @@ __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const Til
     for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
   if (!team_barrier(ts, g, gerr, XB_K_EMIT_HEAVY)) return false;
   for (int t = 0; t < words; ++t)
-    nl_emit_light<T, K>(g, s, t, job.cnt, job.src, gv.in_off, gv.in_col, gv.n, gv.cut_medium, job.lvl_out,
+    nl_emit_light<T, K>(g, s, t, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.n, gv.cut_medium, job.lvl_out,
                         job.par_out);
   return true;
 }
@@ -1100,6 +1129,9 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   TeamCtl* ctl = tm.ctl;
   BPState s = tm.s;
   s.hidx = gv.hidx;
+  s.new_of = gv.new_of;
+  s.orig_of = gv.orig_of;
+  s.in_col_o = gv.in_col_o;
   const int32_t n = gv.n;
   const int64_t nw = (int64_t)n * T;
   // Tile set-up: seen and planes cleared, heavy parents unresolved; fronts,
