@@ -190,6 +190,7 @@ struct xb_graph {
   int32_t* d_col = nullptr;
   int32_t* d_oin_off = nullptr;  // original-order in-offsets (the emit); == d_off when symmetric
   int32_t* d_oin_col = nullptr;  // original-order in-lists with engine-id entries (the emit)
+  int32_t* d_oin_colo = nullptr; // the same in-lists with original ids; == d_col when symmetric
   // Bit-parallel engine graph on engine ids (descending out-degree order).
   std::vector<int32_t> h_new_of;    // original id -> engine id
   std::vector<int32_t> h_ein_off;   // engine in-edge offsets (layout of the pull and emit)
@@ -228,6 +229,7 @@ struct xb_graph {
     dfree(d_new_of);
     dfree(d_orig_of);
     if (d_oin_off != d_off) dfree(d_oin_off);
+    if (d_oin_colo != d_col) dfree(d_oin_colo);
     dfree(d_oin_col);
     dfree(d_off);
     dfree(d_col);
@@ -404,9 +406,12 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
     in_off = ioff.data();
     in_col = icol.data();
     g->d_oin_off = dalloc<int32_t>((size_t)n + 1);
+    g->d_oin_colo = dalloc<int32_t>((size_t)m);
     h2d(g->d_oin_off, ioff.data(), sizeof(int32_t) * ((size_t)n + 1), g->stream);
+    h2d(g->d_oin_colo, icol.data(), sizeof(int32_t) * (size_t)m, g->stream);
   } else {
     g->d_oin_off = g->d_off;
+    g->d_oin_colo = g->d_col;
   }
   build_engine_graph(g.get(), off, colv.data(), in_off, in_col, (flags & XB_GRAPH_DIRECTED) != 0);
   return g.release();
@@ -853,19 +858,23 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   int K = 0;
   while ((1 << K) <= d + 1) ++K;
   const int words = (cnt + 63) / 64;
-  for (int t = 0; t < words; ++t) {
-    if (g->n_heavy > 0) {
+  if (g->n_heavy > 0)
+    for (int t = 0; t < words; ++t) {
       prof.begin(XB_K_EMIT_HEAVY);
       xb::launch_bp_emit_heavy(w.st, T, s, t, K, g->d_ein_off, g->d_ein_col, g->d_heavy, g->n_heavy);
       prof.end();
     }
+  // Light vertices, one launch per word group (up to 4 words, 256 rows).
+  const int W = xb::emit_group_words(T);
+  for (int grp = 0; grp * W < words; ++grp) {
     prof.begin(XB_K_EMIT);
-    xb::launch_bp_emit(w.st, T, s, t, K, cnt, w.d_src, g->d_oin_off, g->d_oin_col, n, o.cut_medium, lvl_rows,
+    xb::launch_bp_emit(w.st, T, s, grp, K, cnt, w.d_src, g->d_oin_off, g->d_oin_col, g->d_oin_colo, n, o.cut_medium,
+                       lvl_rows,
                        par_rows, rows);
     prof.end();
-    const int rows_w = std::min(64, cnt - 64 * t);
-    // level + parent int32 per (source, vertex) of this word
-    prof.add(XB_K_EMIT, (lvl_rows ? 8.0 : 4.0) * rows_w * (double)n);
+    const int rows_g = std::min(64 * W, cnt - 64 * W * grp);
+    // level + parent int32 per (source, vertex) of this group's rows
+    prof.add(XB_K_EMIT, (lvl_rows ? 8.0 : 4.0) * rows_g * (double)n);
   }
   if (st) {
     st->max_level = std::max(st->max_level, d);
@@ -1259,6 +1268,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     v.orig_of = g->d_orig_of;
     v.oin_off = g->d_oin_off;
     v.oin_col = g->d_oin_col;
+    v.oin_colo = g->d_oin_colo;
     v.perm = g->d_perm;
     for (int c = 0; c < 3; ++c) v.bucket_end[c] = g->bucket_end[c];
     v.n_hub_chunks = g->n_hub_chunks;

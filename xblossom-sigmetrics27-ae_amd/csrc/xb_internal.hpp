@@ -21,14 +21,14 @@ namespace xb {
 //
 // Bit-parallel tile state.  seen, front and nextw are vertex-major with T
 // 64-bit source words per vertex ([v*T + t], engine ids).  planes hold 8
-// bit-sliced
-// planes of the level code (code = level + 1, 0 = unreached), bit b of plane
-// k = bit k of code(source 64t+b, v), stored word-major in two halves:
-//   plane k < 4:   planes[(t*n + v)*4 + k]
-//   plane k >= 4:  planes[T*n*4 + (t*n + v)*4 + k - 4]
-// so that the emit of word t with up to 4 planes (levels <= 14) reads one
-// contiguous n x 32 B block, which stays in the 256 MB Infinity Cache for
-// graphs of up to ~8M vertices while its in-neighbour reads hop at random.
+// bit-sliced planes of the level code (code = level + 1, 0 = unreached), bit
+// b of plane k = bit k of code(source 64t+b, v), in word groups of
+// W = min(T, 4) words and two halves (W is the emit's words per pass):
+//   plane k < 4:   planes[((t/W * n + v) * W + t%W) * 4 + k]
+//   plane k >= 4:  planes[T*n*4 + ((t/W * n + v) * W + t%W) * 4 + k - 4]
+// so a vertex's first 4 planes of a word group are one 128-byte line (four
+// words' codes up to 15, i.e. levels <= 14): the emit's random in-neighbour
+// reads use every byte they fetch.
 struct BPState {
   int64_t n;            // vertices (plane addressing)
   uint64_t* seen;
@@ -71,6 +71,7 @@ struct GraphView {
   const int32_t* orig_of;   // engine id -> original id
   const int32_t* oin_off;   // in-CSR in original vertex and list order, entries as engine ids (emit)
   const int32_t* oin_col;
+  const int32_t* oin_colo;  // the same entries as original ids
   const int32_t* perm;
   int32_t bucket_end[3];
   int32_t n_hub_chunks;
@@ -173,9 +174,11 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
                           const int32_t* in_col, const int32_t* heavy, int32_t n_heavy);
 // src: the tile's source ids (level-1 parents are the sources themselves).
-void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
-                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
-                    int32_t* par_out, RowMap map);
+// Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W = min(T, 4)).
+int emit_group_words(int T);
+void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, int32_t rows, const int32_t* src,
+                    const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
+                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map);
 
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

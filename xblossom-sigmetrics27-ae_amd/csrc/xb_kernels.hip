@@ -53,8 +53,15 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 8192;
 constexpr int kPushR = 4;                // edges per thread per chunk
 constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
-constexpr int kPullU = 4;                // independent neighbour loads per lane per step
-constexpr int kEmitU = 4;
+// Independent neighbour loads per lane per step of the pull and the emit.
+#ifndef XB_PULL_U
+#define XB_PULL_U 4
+#endif
+#ifndef XB_EMIT_U
+#define XB_EMIT_U 4
+#endif
+constexpr int kPullU = XB_PULL_U;
+constexpr int kEmitU = XB_EMIT_U;
 constexpr int kStageCap = 1536;
 
 // ---------------------------------------------------------------------------
@@ -138,9 +145,12 @@ struct QueueLds {
   int s_lohi[2];
 };
 
+// The emit's block tile: 64*W rows x 64/W vertices (+1 column of padding),
+// i.e. 64x65, 128x33 or 256x17 int32 for W = 1, 2, 4.
+constexpr int kEmitTileInts = 256 * 17;
 struct EmitLds {
-  int32_t tile[64][65];
-  uint64_t pl[64][4];  // the block's vertices' first 4 code planes, compute -> write phase
+  int32_t tile[kEmitTileInts];
+  uint64_t pl[64][4];  // per (vertex, word) of the block: the first 4 code planes, compute -> write phase
 };
 
 union Lds {
@@ -354,27 +364,53 @@ __device__ __forceinline__ uint64_t word_mask(int32_t cnt, int t) {
 }
 
 // Level-code planes of (v, word t) in the word-major split layout.
+// Words per emit pass (and per plane group), at most XB_EMIT_GROUP_MAX.  One
+// word per pass keeps the output row segments a block writes at 256 bytes;
+// measured on C3, groups of 2 and 4 words (128- and 64-byte segments) made
+// the emit 7% and 20% slower despite using every fetched plane byte.
+#ifndef XB_EMIT_GROUP_MAX
+#define XB_EMIT_GROUP_MAX 1
+#endif
+template <int T>
+constexpr int kGroupWords = T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX;
+
+template <int T>
+__device__ __forceinline__ int64_t plane_index(const BPState& s, int64_t v, int t) {
+  constexpr int W = kGroupWords<T>;
+  return (((int64_t)(t / W) * s.n + v) * W + (t % W)) * 4;
+}
 template <int T>
 __device__ __forceinline__ uint64_t* plane_lo(const BPState& s, int64_t v, int t) {
-  return s.planes + ((int64_t)t * s.n + v) * 4;
+  return s.planes + plane_index<T>(s, v, t);
 }
 template <int T>
 __device__ __forceinline__ uint64_t* plane_hi(const BPState& s, int64_t v, int t) {
-  return s.planes + (int64_t)T * s.n * 4 + ((int64_t)t * s.n + v) * 4;
+  return s.planes + (int64_t)T * s.n * 4 + plane_index<T>(s, v, t);
 }
 
-// OR `bits` into the planes of (v, t) whose bit is set in `code`.
+// OR `bits` into the planes of (v, t) whose bit is set in `code`.  One lane
+// owns a (v, t) in a level; with XB_PLANES_ATOMIC the OR is a no-return
+// atomic done in the L2, so the lane does not wait for the line.
+#ifndef XB_PLANES_ATOMIC
+#define XB_PLANES_ATOMIC 0
+#endif
+__device__ __forceinline__ void or_word(uint64_t* p, uint64_t bits) {
+  if (XB_PLANES_ATOMIC)
+    (void)__hip_atomic_fetch_or(p, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    *p |= bits;
+}
 template <int T>
 __device__ __forceinline__ void planes_or(const BPState& s, int64_t v, int t, uint32_t code, uint64_t bits) {
   uint64_t* lo = plane_lo<T>(s, v, t);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    if ((code >> k) & 1u) lo[k] |= bits;
+    if ((code >> k) & 1u) or_word(&lo[k], bits);
   if (code >> 4) {
     uint64_t* hi = plane_hi<T>(s, v, t);
 #pragma unroll
     for (int k = 4; k < 8; ++k)
-      if ((code >> k) & 1u) hi[k - 4] |= bits;
+      if ((code >> k) & 1u) or_word(&hi[k - 4], bits);
   }
 }
 
@@ -746,26 +782,40 @@ __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t*
   }
 }
 
-// Light vertices: a block takes 64 consecutive vertices.  Compute phase: 4
-// lanes per vertex, each taking every 4th in-neighbour (kEmitU per step) and
-// testing all 64 sources of the word at once; a segmented prefix-OR over the
-// vertex's 4 lanes, in ascending neighbour position, gives the first
-// (lowest-id) provider of every bit, staged in an LDS tile [row][vertex].
-// Heavy vertices copy their recorded parents into the tile.  Write phase: each
-// wave takes 16 rows and each lane one vertex, so every store writes 256
-// contiguous bytes of one row of the level and parent tables.
+// Light vertices, one word group (W = min(T, 4) words, 64*W rows) per pass.
+// A block takes VG = 64/W consecutive vertices (output order); lane (vertex,
+// partition p, word w) takes every 4th in-neighbour of the vertex (kEmitU per
+// step) and tests the 64 sources of its word at once.  The W word-lanes of a
+// (vertex, partition) read the same neighbour's 128-byte plane line, so the
+// random gathers use every byte they fetch.  A segmented prefix-OR over the
+// 4 partitions, in ascending neighbour position, gives the first (lowest-id)
+// provider of every bit, staged in an LDS tile [row][vertex]; heavy vertices
+// copy their recorded parents into it.  Write phase: VG consecutive lanes
+// write one row's VG vertices (4*VG contiguous bytes), 256/VG rows per store.
 template <int T, int K>
-__device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
+__device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
-                                 const int32_t* __restrict__ in_col, int32_t n, int32_t heavy_cut,
-                                 int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
+                                 const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
+                                 int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out,
+                                 int32_t* __restrict__ par_out, RowMap map) {
+  constexpr int W = kGroupWords<T>;
+  constexpr int PART = 4;
+  constexpr int VG = 64 / W;
+  constexpr int LD = VG + 1;  // tile row pitch
+  constexpr int KL = K < 4 ? K : 4;
+  static_assert(64 * W * LD <= kEmitTileInts, "emit tile");
   EmitLds& L = g_lds.e;
+  const int tid = threadIdx.x;
   const int l = lane_id();
-  const int wv = threadIdx.x / kWave;
-  const int vi = wv * 16 + (l >> 2);
-  const int p = l & 3;
+  const int w = tid % W;
+  const int p = (tid / W) % PART;
+  const int vi = tid / (W * PART);
+  const int t = grp * W + w;          // this lane's word of the tile
+  const int rb = 64 * w;              // its rows in the block tile
+  const int wcol = tid % VG;          // write phase: vertex column
+  const int wrow = tid / VG;          // write phase: first row
   __syncthreads();
-  for (int64_t vb = (int64_t)g.bid * 64; vb < n; vb += (int64_t)g.nb * 64) {
+  for (int64_t vb = (int64_t)g.bid * VG; vb < n; vb += (int64_t)g.nb * VG) {
     // ---- compute phase ----
     const int32_t v = (int32_t)(vb + vi);
     const bool act = v < n;
@@ -775,38 +825,40 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
     int32_t b = 0, e = 0, vn = 0;
     if (act) {
       vn = s.new_of[v];
-      load_planes<T, K>(s, vn, word, pk);
+      load_planes<T, K>(s, vn, t, pk);
       b = in_off[v];
       e = in_off[v + 1];
     }
     if (p == 0) {
 #pragma unroll
-      for (int k = 0; k < (K < 4 ? K : 4); ++k) L.pl[vi][k] = pk[k];
+      for (int k = 0; k < KL; ++k) L.pl[vi * W + w][k] = pk[k];
     }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
     uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
     if (heavy) {
-      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[vn] * T + word) * 64;
+      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[vn] * T + t) * 64;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) L.tile[16 * p + i][vi] = (int32_t)hp[16 * p + i];
+      for (int i = 0; i < 16; ++i) L.tile[(rb + 16 * p + i) * LD + vi] = (int32_t)hp[16 * p + i];
     }
-    for (int32_t k0 = 0;; k0 += 4 * kEmitU) {
+    for (int32_t k0 = 0;; k0 += PART * kEmitU) {
       const bool more = need != 0 && (b + k0 + p < e);
       if (!__any(more)) break;
-      int32_t us[kEmitU];
+      int32_t us[kEmitU], uo[kEmitU];
       uint64_t eqs[kEmitU];
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
-        const int32_t j = b + k0 + i * 4 + p;
-        us[i] = (more && j < e) ? in_col[j] : -1;
+        const int32_t j = b + k0 + i * PART + p;
+        const bool in = more && j < e;
+        us[i] = in ? in_col[j] : -1;
+        uo[i] = in ? in_col_orig[j] : -1;
       }
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
         uint64_t eq = 0;
         if (us[i] >= 0) {
           uint64_t pu[K];
-          load_planes<T, K>(s, us[i], word, pu);
+          load_planes<T, K>(s, us[i], t, pu);
           eq = ~0ull;
 #pragma unroll
           for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
@@ -817,45 +869,46 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t word, int32_t rows,
       for (int i = 0; i < kEmitU; ++i) {
         const uint64_t c = eqs[i] & need;
         uint64_t incl = c;
-        uint64_t y = __shfl_up(incl, 1, kWave);
+        uint64_t y = __shfl_up(incl, W, kWave);
         if (p >= 1) incl |= y;
-        y = __shfl_up(incl, 2, kWave);
+        y = __shfl_up(incl, 2 * W, kWave);
         if (p >= 2) incl |= y;
-        uint64_t excl = __shfl_up(incl, 1, kWave);
+        uint64_t excl = __shfl_up(incl, W, kWave);
         if (p == 0) excl = 0;
         uint64_t first = c & ~excl;
-        if (first) {
-          const int32_t uo = s.orig_of[us[i]];
-          while (first) {
-            const int bit = __builtin_ctzll(first);
-            first &= first - 1;
-            L.tile[bit][vi] = uo;
-          }
+        while (first) {
+          const int bit = __builtin_ctzll(first);
+          first &= first - 1;
+          L.tile[(rb + bit) * LD + vi] = uo[i];
         }
-        need &= ~__shfl(incl, (l & ~3) | 3, kWave);
+        need &= ~__shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
       }
     }
     __syncthreads();
     // ---- write phase ----
-    const int32_t v2 = (int32_t)(vb + l);
+    const int32_t v2 = (int32_t)(vb + wcol);
     if (v2 < n) {
-      // The vertex's plane words: from LDS for up to 4 planes, else read again.
+      int wq = -1;
       uint64_t q[K];
-      if (K <= 4) {
-#pragma unroll
-        for (int k = 0; k < (K < 4 ? K : 4); ++k) q[k] = L.pl[l][k];
-      } else {
-        load_planes<T, K>(s, s.new_of[v2], word, q);
-      }
 #pragma unroll 4
-      for (int i = 0; i < 16; ++i) {
-        const int bit = 16 * wv + i;
-        const int r = 64 * word + bit;
+      for (int rr = wrow; rr < 64 * W; rr += 256 / VG) {
+        const int w2 = rr >> 6;
+        const int bit = rr & 63;
+        const int r = 64 * (grp * W + w2) + bit;
         if (r >= rows) break;
+        if (w2 != wq) {
+          wq = w2;
+          if (K <= 4) {
+#pragma unroll
+            for (int k = 0; k < KL; ++k) q[k] = L.pl[wcol * W + w2][k];
+          } else {
+            load_planes<T, K>(s, s.new_of[v2], grp * W + w2, q);
+          }
+        }
         uint32_t c = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
-        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[bit][l]));
+        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[rr * LD + wcol]));
         const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v2;
         const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v2;
         if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[lo]);
@@ -931,10 +984,12 @@ template <int T, int K>
 __global__ __launch_bounds__(kBlock, XB_EMIT_MIN_BLOCKS) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
                                                           const int32_t* __restrict__ src,
                                                           const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col, int32_t n,
+                                                          const int32_t* __restrict__ in_col,
+                                                          const int32_t* __restrict__ in_col_orig, int32_t n,
                                                           int32_t heavy_cut, int32_t* __restrict__ lvl_out,
                                                           int32_t* __restrict__ par_out, RowMap map) {
-  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out, map);
+  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out,
+                         map);
 }
 
 // ============================================================================
@@ -1103,9 +1158,10 @@ __device__ __noinline__ void nl_emit_heavy(Grid g, BPState s, int32_t word, cons
 }
 template <int T, int K>
 __device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
-                                           const int32_t* in_off, const int32_t* in_col, int32_t n,
-                                           int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
-  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, n, heavy_cut, lvl_out, par_out, RowMap{});
+                                           const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
+                                           int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
+  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out,
+                         RowMap{});
 }
 
 template <int T, int K>
@@ -1115,8 +1171,9 @@ __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const Til
   if (gv.n_heavy > 0)
     for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
   if (!team_barrier(ts, g, gerr, XB_K_EMIT_HEAVY)) return false;
-  for (int t = 0; t < words; ++t)
-    nl_emit_light<T, K>(g, s, t, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.n, gv.cut_medium, job.lvl_out,
+  for (int grp = 0; grp * kGroupWords<T> < words; ++grp)
+    nl_emit_light<T, K>(g, s, grp, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.oin_colo, gv.n, gv.cut_medium,
+                        job.lvl_out,
                         job.par_out);
   return true;
 }
@@ -1484,10 +1541,10 @@ static void emit_heavy_launch(hipStream_t st, BPState s, int32_t word, const int
 
 template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
-                              const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut,
-                              int32_t* lvl_out, int32_t* par_out, RowMap map) {
-  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64)), dim3(kBlock), 0, st, s, word, rows, src, in_off,
-                     in_col, n, heavy_cut, lvl_out, par_out, map);
+                              const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
+                              int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
+  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64 / kGroupWords<T>)), dim3(kBlock), 0, st, s, word,
+                     rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
 }
 
 void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
@@ -1496,11 +1553,13 @@ void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_
   XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_heavy_launch<TT, KK>(st, s, word, in_off, in_col, heavy, n_heavy))));
 }
 
+int emit_group_words(int T) { return T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX; }
+
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
-                    const int32_t* in_off, const int32_t* in_col, int32_t n, int32_t heavy_cut, int32_t* lvl_out,
-                    int32_t* par_out, RowMap map) {
-  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, n, heavy_cut,
-                                                               lvl_out, par_out, map))));
+                    const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
+                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
+  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, in_col_orig, n,
+                                                               heavy_cut, lvl_out, par_out, map))));
 }
 
 int persistent_blocks_per_cu() {
