@@ -90,6 +90,16 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
   return x;
 }
 
+// DPP quad permutation of a 64-bit value: lane 4q+i receives lane 4q+sel(i)
+// of its quad (a VALU move, no LDS crossbar round trip).
+constexpr int quad_sel(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+template <int CTRL>
+__device__ __forceinline__ uint64_t quad_perm64(uint64_t x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, false);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
+}
+
 __device__ __forceinline__ uint64_t wave_excl_or(uint64_t x) {
   const int l = lane_id();
   uint64_t incl = x;
@@ -150,7 +160,7 @@ struct QueueLds {
 constexpr int kEmitTileInts = 256 * 17;
 struct EmitLds {
   int32_t tile[kEmitTileInts];
-  uint64_t pl[64][4];  // per (vertex, word) of the block: the first 4 code planes, compute -> write phase
+  uint64_t pl[4][64];  // plane k of (vertex, word) of the block, compute -> write phase (lane-contiguous)
 };
 
 union Lds {
@@ -831,7 +841,7 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
     }
     if (p == 0) {
 #pragma unroll
-      for (int k = 0; k < KL; ++k) L.pl[vi * W + w][k] = pk[k];
+      for (int k = 0; k < KL; ++k) L.pl[k][vi * W + w] = pk[k];
     }
     const CodeMasks<K> cm = code_masks<K>(pk);
     const bool heavy = act && (e - b > heavy_cut);
@@ -868,20 +878,30 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
         const uint64_t c = eqs[i] & need;
-        uint64_t incl = c;
-        uint64_t y = __shfl_up(incl, W, kWave);
-        if (p >= 1) incl |= y;
-        y = __shfl_up(incl, 2 * W, kWave);
-        if (p >= 2) incl |= y;
-        uint64_t excl = __shfl_up(incl, W, kWave);
-        if (p == 0) excl = 0;
+        uint64_t incl = c, excl, all;
+        if constexpr (W == 1) {
+          // The 4 partitions of a vertex are one lane quad.
+          incl |= quad_perm64<quad_sel(0, 0, 1, 2)>(incl);  // lane p <- p-1 (lane 0 <- itself)
+          incl |= quad_perm64<quad_sel(0, 1, 0, 1)>(incl);  // lane p <- p-2 (lanes 0, 1 <- themselves)
+          excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
+          if (p == 0) excl = 0;
+          all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
+        } else {
+          uint64_t y = __shfl_up(incl, W, kWave);
+          if (p >= 1) incl |= y;
+          y = __shfl_up(incl, 2 * W, kWave);
+          if (p >= 2) incl |= y;
+          excl = __shfl_up(incl, W, kWave);
+          if (p == 0) excl = 0;
+          all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
+        }
         uint64_t first = c & ~excl;
         while (first) {
           const int bit = __builtin_ctzll(first);
           first &= first - 1;
           L.tile[(rb + bit) * LD + vi] = uo[i];
         }
-        need &= ~__shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
+        need &= ~all;
       }
     }
     __syncthreads();
@@ -900,7 +920,7 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
           wq = w2;
           if (K <= 4) {
 #pragma unroll
-            for (int k = 0; k < KL; ++k) q[k] = L.pl[wcol * W + w2][k];
+            for (int k = 0; k < KL; ++k) q[k] = L.pl[k][wcol * W + w2];
           } else {
             load_planes<T, K>(s, s.new_of[v2], grp * W + w2, q);
           }
