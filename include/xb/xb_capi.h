@@ -135,6 +135,25 @@ int32_t xb_graph_load_text(const char* row_offsets_path,
                            const char* col_indices_path, uint32_t flags,
                            xb_graph** out);
 
+/* Binary CSR files: a 40-byte header (magic "XBCSR01", graph flags, n+1, m,
+ * a 64-bit FNV-1a checksum over the 32-bit words of both arrays), then the
+ * int32 row offsets and column indices, little-endian.  No counterpart in
+ * the reference, which only reads the text pair (ReadFileIntoHVector,
+ * apps/xb/src/include/common/utils_gpu.h:9-25) and converts datasets with
+ * scripts (scripts/2-datasets/...); loading a binary file skips the text
+ * parse.  The xb_csr_* calls are host-only (no device needed). */
+int32_t xb_csr_save(const char* path, const int32_t* row_offsets, int64_t n_plus_1,
+                    const int32_t* col_indices, int64_t m, uint32_t flags);
+int32_t xb_csr_load_header(const char* path, int64_t* n_plus_1, int64_t* m, uint32_t* flags);
+/* Buffers must hold exactly the header's n_plus_1 and m entries; XB_ERR_IO
+ * on a truncated file or checksum mismatch. */
+int32_t xb_csr_load(const char* path, int32_t* row_offsets, int64_t n_plus_1,
+                    int32_t* col_indices, int64_t m);
+/* xb_csr_load + xb_graph_create_csr with the file's flags. */
+int32_t xb_graph_load_binary(const char* path, xb_graph** out);
+/* Writes the graph's CSR (adjacency lists as stored: sorted on ingest). */
+int32_t xb_graph_save_binary(const xb_graph* g, const char* path);
+
 int32_t xb_graph_info(const xb_graph* g, int64_t* n, int64_t* m,
                       uint32_t* flags);
 

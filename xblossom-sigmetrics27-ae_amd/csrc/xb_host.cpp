@@ -417,7 +417,13 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
   return g.release();
 }
 
-// Reads whitespace-separated int32 values (ReadFileIntoHVector's format).
+// Reads whitespace-separated int32 values (ReadFileIntoHVector's format,
+// apps/xb/src/include/common/utils_gpu.h:9-25), but fails on a missing file,
+// a non-integer token or an out-of-range value instead of stopping.  The
+// buffer is split at whitespace into one chunk per thread; each chunk is
+// parsed with the same rules and the earliest failure in file order wins.
+bool is_ws(char c) { return c == ' ' || c == '\n' || c == '\r' || c == '\t'; }
+
 std::vector<int32_t> read_ints(const char* path) {
   FILE* f = std::fopen(path, "rb");
   if (!f) fail(XB_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
@@ -428,32 +434,146 @@ std::vector<int32_t> read_ints(const char* path) {
   size_t rd = sz > 0 ? std::fread(&buf[0], 1, (size_t)sz, f) : 0;
   std::fclose(f);
   if ((long)rd != sz) fail(XB_ERR_IO, std::string("short read on ") + path);
-  std::vector<int32_t> out;
-  out.reserve((size_t)sz / 4 + 1);
-  const char* p = buf.data();
-  const char* end = p + buf.size();
-  while (p < end) {
-    while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) ++p;
-    if (p >= end) break;
-    bool neg = false;
-    if (*p == '-' || *p == '+') {
-      neg = (*p == '-');
-      ++p;
-    }
-    if (p >= end || *p < '0' || *p > '9') fail(XB_ERR_IO, std::string("non-integer token in ") + path);
-    int64_t v = 0;
-    while (p < end && *p >= '0' && *p <= '9') {
-      v = v * 10 + (*p - '0');
-      if (v > (int64_t)INT32_MAX + 1) fail(XB_ERR_IO, std::string("integer overflow in ") + path);
-      ++p;
-    }
-    if (p < end && !(*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t'))
-      fail(XB_ERR_IO, std::string("non-integer token in ") + path);
-    v = neg ? -v : v;
-    if (v > INT32_MAX) fail(XB_ERR_IO, std::string("integer overflow in ") + path);
-    out.push_back((int32_t)v);
+  const char* base = buf.data();
+  const size_t len = buf.size();
+  int T = (len < ((size_t)1 << 22)) ? 1 : resolve_threads();
+  std::vector<size_t> cut((size_t)T + 1, len);
+  cut[0] = 0;
+  for (int t = 1; t < T; ++t) {
+    size_t c = std::max(cut[t - 1], len / T * t);
+    while (c < len && !is_ws(base[c])) ++c;  // never split a token
+    cut[t] = c;
   }
+  std::vector<std::vector<int32_t>> part((size_t)T);
+  std::vector<std::string> err((size_t)T);
+  auto parse = [&](int t) {
+    const char* p = base + cut[t];
+    const char* end = base + cut[t + 1];
+    std::vector<int32_t>& out = part[(size_t)t];
+    out.reserve((size_t)(end - p) / 4 + 1);
+    while (p < end) {
+      while (p < end && is_ws(*p)) ++p;
+      if (p >= end) break;
+      bool neg = false;
+      if (*p == '-' || *p == '+') {
+        neg = (*p == '-');
+        ++p;
+      }
+      if (p >= end || *p < '0' || *p > '9') {
+        err[(size_t)t] = std::string("non-integer token in ") + path;
+        return;
+      }
+      int64_t v = 0;
+      while (p < end && *p >= '0' && *p <= '9') {
+        v = v * 10 + (*p - '0');
+        if (v > (int64_t)INT32_MAX + 1) {
+          err[(size_t)t] = std::string("integer overflow in ") + path;
+          return;
+        }
+        ++p;
+      }
+      if (p < end && !is_ws(*p)) {
+        err[(size_t)t] = std::string("non-integer token in ") + path;
+        return;
+      }
+      v = neg ? -v : v;
+      if (v > INT32_MAX) {
+        err[(size_t)t] = std::string("integer overflow in ") + path;
+        return;
+      }
+      out.push_back((int32_t)v);
+    }
+  };
+  if (T == 1) {
+    parse(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(parse, t);
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < T; ++t)
+    if (!err[(size_t)t].empty()) fail(XB_ERR_IO, err[(size_t)t]);
+  size_t total = 0;
+  for (auto& v : part) total += v.size();
+  std::vector<int32_t> out;
+  out.reserve(total);
+  for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
   return out;
+}
+
+// ---------------------------------------------------------------------------
+// Binary CSR files
+//   char magic[8] "XBCSR01" (NUL-padded); uint32 flags (XB_GRAPH_*); uint32 0;
+//   int64 n_plus_1; int64 m; uint64 checksum; int32 row_offsets[n_plus_1];
+//   int32 col_indices[m]; all little-endian.  The checksum is a 64-bit
+//   FNV-1a over the 32-bit words of row_offsets then col_indices.
+// ---------------------------------------------------------------------------
+struct CsrFileHeader {
+  char magic[8];
+  uint32_t flags;
+  uint32_t zero;
+  int64_t n_plus_1;
+  int64_t m;
+  uint64_t checksum;
+};
+static_assert(sizeof(CsrFileHeader) == 40, "CSR file header layout");
+const char kCsrMagic[8] = {'X', 'B', 'C', 'S', 'R', '0', '1', 0};
+
+uint64_t csr_words_hash(uint64_t h, const int32_t* w, int64_t count) {
+  for (int64_t i = 0; i < count; ++i) {
+    h ^= (uint32_t)w[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+uint64_t csr_checksum(const int32_t* off, int64_t n_plus_1, const int32_t* col, int64_t m) {
+  return csr_words_hash(csr_words_hash(0xcbf29ce484222325ull, off, n_plus_1), col, m);
+}
+
+void write_csr_file(const char* path, const int32_t* off, int64_t n_plus_1, const int32_t* col, int64_t m,
+                    uint32_t flags) {
+  CsrFileHeader h{};
+  std::memcpy(h.magic, kCsrMagic, 8);
+  h.flags = flags;
+  h.n_plus_1 = n_plus_1;
+  h.m = m;
+  h.checksum = csr_checksum(off, n_plus_1, col, m);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) fail(XB_ERR_IO, std::string("cannot create ") + path + ": " + std::strerror(errno));
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+  ok = ok && std::fwrite(off, sizeof(int32_t), (size_t)n_plus_1, f) == (size_t)n_plus_1;
+  if (m > 0) ok = ok && std::fwrite(col, sizeof(int32_t), (size_t)m, f) == (size_t)m;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) fail(XB_ERR_IO, std::string("write failed on ") + path);
+}
+
+CsrFileHeader read_csr_header(FILE* f, const char* path) {
+  CsrFileHeader h{};
+  if (std::fread(&h, sizeof(h), 1, f) != 1) fail(XB_ERR_IO, std::string("truncated header in ") + path);
+  if (std::memcmp(h.magic, kCsrMagic, 8) != 0) fail(XB_ERR_IO, std::string("not a binary CSR file: ") + path);
+  if (h.flags & ~XB_GRAPH_DIRECTED) fail(XB_ERR_IO, std::string("unknown graph flags in ") + path);
+  if (h.n_plus_1 < 1 || h.n_plus_1 - 1 > INT32_MAX - 1 || h.m < 0 || h.m > INT32_MAX)
+    fail(XB_ERR_IO, std::string("n or m out of range in ") + path);
+  return h;
+}
+
+// Reads the arrays into caller buffers of exactly n_plus_1 and m entries and
+// checks the checksum (structure is validated where the graph is built).
+void read_csr_file(const char* path, int32_t* off, int64_t n_plus_1, int32_t* col, int64_t m, uint32_t* flags) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) fail(XB_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  struct Closer {
+    FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  const CsrFileHeader h = read_csr_header(f, path);
+  if (h.n_plus_1 != n_plus_1 || h.m != m) fail(XB_ERR_INVALID_ARG, "buffer sizes do not match the file header");
+  if (std::fread(off, sizeof(int32_t), (size_t)n_plus_1, f) != (size_t)n_plus_1 ||
+      (m > 0 && std::fread(col, sizeof(int32_t), (size_t)m, f) != (size_t)m))
+    fail(XB_ERR_IO, std::string("truncated arrays in ") + path);
+  if (csr_checksum(off, n_plus_1, col, m) != h.checksum) fail(XB_ERR_IO, std::string("checksum mismatch in ") + path);
+  if (flags) *flags = h.flags;
 }
 
 // ---------------------------------------------------------------------------
@@ -1002,6 +1122,70 @@ int32_t xb_graph_load_text(const char* row_path, const char* col_path, uint32_t 
     std::vector<int32_t> col = read_ints(col_path);
     if (off.empty()) fail(XB_ERR_IO, std::string("no row offsets in ") + row_path);
     *out = build_graph(off.data(), (int64_t)off.size(), col.data(), (int64_t)col.size(), flags);
+  });
+}
+
+int32_t xb_csr_save(const char* path, const int32_t* row_offsets, int64_t n_plus_1, const int32_t* col_indices,
+                    int64_t m, uint32_t flags) {
+  return guarded([&] {
+    if (!path || !row_offsets || n_plus_1 < 1 || m < 0 || (m > 0 && !col_indices))
+      fail(XB_ERR_INVALID_ARG, "null or empty CSR arrays");
+    if (flags & ~XB_GRAPH_DIRECTED) fail(XB_ERR_INVALID_ARG, "unknown graph flag");
+    write_csr_file(path, row_offsets, n_plus_1, col_indices, m, flags);
+  });
+}
+
+int32_t xb_csr_load_header(const char* path, int64_t* n_plus_1, int64_t* m, uint32_t* flags) {
+  return guarded([&] {
+    if (!path) fail(XB_ERR_INVALID_ARG, "null path");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) fail(XB_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    struct Closer {
+      FILE* f;
+      ~Closer() { std::fclose(f); }
+    } closer{f};
+    const CsrFileHeader h = read_csr_header(f, path);
+    if (n_plus_1) *n_plus_1 = h.n_plus_1;
+    if (m) *m = h.m;
+    if (flags) *flags = h.flags;
+  });
+}
+
+int32_t xb_csr_load(const char* path, int32_t* row_offsets, int64_t n_plus_1, int32_t* col_indices, int64_t m) {
+  return guarded([&] {
+    if (!path || !row_offsets || (m > 0 && !col_indices)) fail(XB_ERR_INVALID_ARG, "null argument");
+    read_csr_file(path, row_offsets, n_plus_1, col_indices, m, nullptr);
+  });
+}
+
+int32_t xb_graph_load_binary(const char* path, xb_graph** out) {
+  return guarded([&] {
+    if (!path || !out) fail(XB_ERR_INVALID_ARG, "null argument");
+    int64_t np1 = 0, m = 0;
+    {
+      FILE* f = std::fopen(path, "rb");
+      if (!f) fail(XB_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+      struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+      } closer{f};
+      const CsrFileHeader h = read_csr_header(f, path);
+      np1 = h.n_plus_1;
+      m = h.m;
+    }
+    std::vector<int32_t> off((size_t)np1), col((size_t)std::max<int64_t>(m, 1));
+    uint32_t flags = 0;
+    read_csr_file(path, off.data(), np1, col.data(), m, &flags);
+    *out = build_graph(off.data(), np1, col.data(), m, flags);
+  });
+}
+
+int32_t xb_graph_save_binary(const xb_graph* g, const char* path) {
+  return guarded([&] {
+    if (!g || !path) fail(XB_ERR_INVALID_ARG, "null argument");
+    std::vector<int32_t> col((size_t)std::max<int64_t>(g->m, 1));
+    d2h(col.data(), g->d_col, sizeof(int32_t) * (size_t)g->m, g->stream);
+    write_csr_file(path, g->h_off.data(), (int64_t)g->h_off.size(), col.data(), g->m, g->flags);
   });
 }
 

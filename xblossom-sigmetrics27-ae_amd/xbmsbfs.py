@@ -55,6 +55,11 @@ EXPORTED_SYMBOLS = (
     "xb_opts_default",
     "xb_graph_create_csr",
     "xb_graph_load_text",
+    "xb_csr_save",
+    "xb_csr_load_header",
+    "xb_csr_load",
+    "xb_graph_load_binary",
+    "xb_graph_save_binary",
     "xb_graph_info",
     "xb_graph_destroy",
     "xb_msbfs_run",
@@ -211,6 +216,11 @@ def load_engine():
     lib.xb_graph_create_csr.argtypes = [_P32, I64, _P32, I64, U32, ctypes.POINTER(VP)]
     lib.xb_graph_load_text.argtypes = [ctypes.c_char_p, ctypes.c_char_p, U32, ctypes.POINTER(VP)]
     lib.xb_graph_info.argtypes = [VP, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(U32)]
+    lib.xb_csr_save.argtypes = [ctypes.c_char_p, _P32, I64, _P32, I64, U32]
+    lib.xb_csr_load_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(U32)]
+    lib.xb_csr_load.argtypes = [ctypes.c_char_p, _P32, I64, _P32, I64]
+    lib.xb_graph_load_binary.argtypes = [ctypes.c_char_p, ctypes.POINTER(VP)]
+    lib.xb_graph_save_binary.argtypes = [VP, ctypes.c_char_p]
     lib.xb_graph_destroy.argtypes = [VP]
     lib.xb_msbfs_run.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts), ctypes.POINTER(RunStats)]
     lib.xb_count_traversed.argtypes = [VP, _P32, I32, ctypes.POINTER(I64)]
@@ -276,6 +286,21 @@ class HGraph:
         n, m, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
         _check(lib.xb_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(f)))
         self.nnodes, self.nedges, self.directed = n.value, m.value, bool(f.value & 1)
+
+    @classmethod
+    def from_binary(cls, path: str) -> "HGraph":
+        """Loads a binary CSR file (xb_graph_load_binary)."""
+        g = cls.__new__(cls)
+        lib = load_engine()
+        g._h = ctypes.c_void_p()
+        _check(lib.xb_graph_load_binary(os.fsencode(path), ctypes.byref(g._h)))
+        n, m, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+        _check(lib.xb_graph_info(g._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(f)))
+        g.nnodes, g.nedges, g.directed = n.value, m.value, bool(f.value & 1)
+        return g
+
+    def save_binary(self, path: str):
+        _check(load_engine().xb_graph_save_binary(self._h, os.fsencode(path)))
 
     @property
     def handle(self):
@@ -600,6 +625,25 @@ def _take(c: _Csr) -> Csr:
     co = np.ctypeslib.as_array(c.col, (max(c.m, 1),))[: c.m].copy()
     g.xbgen_free(ctypes.byref(c))
     return Csr(ro, co)
+
+
+def save_csr_binary(path: str, row_offsets, col_indices, directed: bool = False):
+    """Writes a binary CSR file (host only)."""
+    ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
+    ci = np.ascontiguousarray(col_indices, dtype=np.int32)
+    _check(load_engine().xb_csr_save(os.fsencode(path), _ptr(ro), ro.size, _ptr(ci), ci.size,
+                                     XB_GRAPH_DIRECTED if directed else XB_GRAPH_SYMMETRIC))
+
+
+def load_csr_binary(path: str):
+    """Reads a binary CSR file (host only): returns (row_offsets, col_indices, directed)."""
+    lib = load_engine()
+    np1, m, fl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_uint32()
+    _check(lib.xb_csr_load_header(os.fsencode(path), ctypes.byref(np1), ctypes.byref(m), ctypes.byref(fl)))
+    ro = np.empty(np1.value, dtype=np.int32)
+    ci = np.empty(max(m.value, 1), dtype=np.int32)
+    _check(lib.xb_csr_load(os.fsencode(path), _ptr(ro), np1.value, _ptr(ci), m.value))
+    return ro, ci[: m.value], bool(fl.value & XB_GRAPH_DIRECTED)
 
 
 def gen_rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, symmetric: bool = True,
