@@ -384,9 +384,18 @@ __device__ __forceinline__ uint64_t word_mask(int32_t cnt, int t) {
 template <int T>
 constexpr int kGroupWords = T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX;
 
+// Words per plane group (the plane layout), at most XB_PLANE_GROUP_MAX; the
+// emit's words per pass must divide it.
+#ifndef XB_PLANE_GROUP_MAX
+#define XB_PLANE_GROUP_MAX XB_EMIT_GROUP_MAX
+#endif
+template <int T>
+constexpr int kPlaneGroup = T < XB_PLANE_GROUP_MAX ? T : XB_PLANE_GROUP_MAX;
+
 template <int T>
 __device__ __forceinline__ int64_t plane_index(const BPState& s, int64_t v, int t) {
-  constexpr int W = kGroupWords<T>;
+  constexpr int W = kPlaneGroup<T>;
+  static_assert(kPlaneGroup<T> % kGroupWords<T> == 0, "emit words per pass must divide the plane group");
   return (((int64_t)(t / W) * s.n + v) * W + (t % W)) * 4;
 }
 template <int T>
