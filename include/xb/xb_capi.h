@@ -227,6 +227,38 @@ int32_t xb_memcpy_h2d(void* dst, const void* src, size_t bytes);
 /* Copies the last error message of the calling thread into buf. */
 int32_t xb_last_error(char* buf, size_t len);
 
+/* ---- forest mode: maximum matching -------------------------------------
+ * Replaces the phase loop of XBlossomGpuEngine10::FindMaximumMatch
+ * (apps/xb/src/include/xbsm_gpu/max_match_engine.h:93-113) with its
+ * alternating-forest search (apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:
+ * 83-359: InitAlternatingForest, FindAndFlipAugmentingPathInAlternatingForest,
+ * ExpandAlternatingForest) and path flip (path_table.h:273-330).  Each
+ * phase grows a forest from every exposed vertex and flips a maximal set of
+ * tree-disjoint augmenting paths; phases repeat until one finds none.  All
+ * choices are deterministic (lowest root id, then lowest vertex id), so a
+ * run's matching is reproducible.  The blossom transform of the reference
+ * (TransformOddNodesInBlossom, :362-566) is not part of this engine: the
+ * result is a maximum matching when the final phase met no blossom edge
+ * (blossom_edges == 0; always so on bipartite graphs), else a matching that
+ * no forest without blossom contraction can augment.  Symmetric graphs only;
+ * unmatched vertices are -1. */
+typedef struct xb_match_stats {
+  int64_t matched_vertices;  /* 2 x the matching's cardinality               */
+  int64_t augmentations;     /* paths flipped over all phases                */
+  int64_t blossom_edges;     /* same-tree even-even edges met in the final phase */
+  int32_t phases;
+  int32_t levels;            /* forest levels grown over all phases          */
+  double ms_total;
+  int32_t reserved[8];
+} xb_match_stats;
+
+/* init_mate: NULL (empty matching) or n entries (a valid matching, -1 for
+ * exposed); mate_out: n entries.  XB_OUT_DEVICE in flags: both are device
+ * pointers.  An invalid init_mate (out of range, asymmetric, or a pair that
+ * is not an edge) is XB_ERR_INVALID_ARG. */
+int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, uint32_t flags,
+                     xb_match_stats* stats);
+
 /* ---- autotuner ---------------------------------------------------------
  * No counterpart in the reference, whose thresholds are hand-set constants
  * (the |E|/20 push/pull switch of apps/ligra/ligra/ligra.h:236-277 and the

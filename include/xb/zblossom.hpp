@@ -63,6 +63,9 @@ inline std::vector<int32_t> ReadIntFile(const std::string& path) {
 }
 
 // Host CSR graph (int32 rowOffsets / colIndices).
+// One entry per vertex: its mate (apps/xb/src/include/utils/mm_types_gpu.h:9).
+using Matching = std::vector<int32_t>;
+
 class HGraph {
  public:
   HGraph() = default;
@@ -158,6 +161,26 @@ class XBlossomGpuEngine10 {
     XbCheck(xb_msbfs_run(graph_, sources, S, levels_dev, parents_dev, &o, &stats_));
   }
 
+  // FindMaximumMatch of apps/xb/src/include/xbsm_gpu/max_match_engine.h:84-113
+  // (used by apps/xb/runner/xblsm_gpu/run_xb_pp_by_dataset.cu:42-44): entry v
+  // is v's mate, nnodes for an exposed vertex, as in the reference GPU engine.
+  // The forest search has no blossom transform; LastMatchStats().blossom_edges
+  // == 0 certifies the result maximum (always so on bipartite graphs).
+  Matching FindMaximumMatch() { return FindMaximumMatch(Matching(n_, (int32_t)n_)); }
+
+  Matching FindMaximumMatch(const Matching& initial_matching) {
+    if (initial_matching.size() != n_)
+      throw XbException(XB_ERR_INVALID_ARG, "initial matching must have nnodes entries");
+    Matching mate(initial_matching);
+    for (auto& m : mate)
+      if (m == (int32_t)n_) m = -1;
+    XbCheck(xb_match_run(graph_, mate.data(), mate.data(), 0, &match_stats_));
+    for (auto& m : mate)
+      if (m < 0) m = (int32_t)n_;
+    return mate;
+  }
+  const xb_match_stats& LastMatchStats() const { return match_stats_; }
+
   xb_graph* handle() const { return graph_; }
 
  private:
@@ -166,6 +189,7 @@ class XBlossomGpuEngine10 {
   size_t path_table_buffer_ratio_ = 0;
   xb_msbfs_opts opts_{};
   xb_run_stats stats_{};
+  xb_match_stats match_stats_{};
 };
 
 }  // namespace zblossom

@@ -1,0 +1,462 @@
+/*
+ * match_oracle.c -- CPU oracle for forest mode (maximum matching).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file.  Only tests/ and the cpu_baseline leg of bench.py use it,
+ * as the checker or as the timed CPU baseline.
+ *
+ * Functions
+ * ---------
+ * oracle_match_forest   Sequential restatement of the deterministic
+ *                       alternating-forest phase loop that the engine runs on
+ *                       the GPU.  It follows the reference's
+ *                       FindMaximumMatch loop (apps/xb/src/include/xbsm_gpu/
+ *                       max_match_engine.h:93-113) and its search
+ *                       (apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:
+ *                       83-359): exposed vertices become even roots
+ *                       (:119-129), the even frontier claims its unvisited
+ *                       matched neighbours (:270-359), and an edge between
+ *                       even vertices of two trees closes an augmenting path,
+ *                       which is flipped back to both roots (path_table.h:
+ *                       273-330).  The reference lets the first atomic winner
+ *                       decide every race; this restatement fixes each
+ *                       choice:
+ *                         - a vertex is claimed by the even neighbour with the
+ *                           lowest key (root << 32 | claimer);
+ *                         - of a matched pair claimed from both sides, the
+ *                           member with the lower (key, id) becomes odd;
+ *                         - per level, rounds take every pair of trees that
+ *                           are each other's lowest-rooted neighbour tree
+ *                           over even-even edges; the path's bridge is the
+ *                           lowest (vertex in the lower tree << 32 | vertex in
+ *                           the higher tree); taken trees leave the later
+ *                           rounds of the phase.
+ *                       The GPU result must equal this one entry for entry.
+ * oracle_match_edmonds  Maximum cardinality matching of a general graph by
+ *                       Edmonds' blossom algorithm (J. Edmonds, "Paths, trees,
+ *                       and flowers", 1965), in the textbook O(n^3) form: a
+ *                       BFS from each exposed vertex with blossom bases, LCA
+ *                       by marking, and path relabelling on contraction.
+ * oracle_match_hk       Maximum matching of a bipartite graph by Hopcroft and
+ *                       Karp ("An n^{5/2} algorithm for maximum matchings in
+ *                       bipartite graphs", 1973): BFS layering from the free
+ *                       left vertices, then augmenting DFS on the layers (with
+ *                       an explicit stack).  Returns -1 when the graph is not
+ *                       bipartite (2-colouring by BFS).
+ * oracle_match_validate Counts the entries that break the matching rules.
+ *
+ * Parity pins (tests/test_match_oracle.py): closed forms (paths, even and
+ * odd cycles, complete bipartite and complete graphs, grids, stars, the
+ * Petersen graph), networkx's maximum cardinality matching and scipy's
+ * Hopcroft-Karp on random graphs, and the certificate that the forest
+ * result is maximum whenever its final phase met no same-tree even-even
+ * edge (checked against oracle_match_edmonds).
+ *
+ * All graphs are symmetric int32 CSR; mate[v] = -1 marks an exposed vertex.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct oracle_match_stats {
+  int64_t matched_vertices;
+  int64_t augmentations;
+  int64_t blossom_edges;
+  int32_t phases;
+  int32_t levels;
+} oracle_match_stats;
+
+#define NO_KEY (~0ull)
+
+static void* xcalloc(size_t count, size_t size) {
+  void* p = calloc(count ? count : 1, size);
+  if (!p) abort();
+  return p;
+}
+
+int64_t oracle_match_validate(int32_t n, const int32_t* off, const int32_t* col, const int32_t* mate) {
+  int64_t bad = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t m = mate[v];
+    if (m < 0) {
+      bad += (m != -1);
+      continue;
+    }
+    if (m >= n || m == v || mate[m] != v) {
+      ++bad;
+      continue;
+    }
+    int edge = 0;
+    for (int32_t j = off[v]; j < off[v + 1]; ++j) edge |= (col[j] == m);
+    bad += !edge;
+  }
+  return bad;
+}
+
+static int64_t count_matched(int32_t n, const int32_t* mate) {
+  int64_t c = 0;
+  for (int32_t v = 0; v < n; ++v) c += mate[v] >= 0;
+  return c;
+}
+
+/* ---- deterministic alternating forest --------------------------------- */
+
+int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate,
+                            oracle_match_stats* st) {
+  oracle_match_stats s;
+  memset(&s, 0, sizeof(s));
+  int32_t* root = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* lvl = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* par = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* best = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* acc = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* roots = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* front = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* next = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* claimed = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* pairs = xcalloc((size_t)n, sizeof(int32_t));
+  uint64_t* claim = xcalloc((size_t)n, sizeof(uint64_t));
+  uint64_t* bridge = xcalloc((size_t)n, sizeof(uint64_t));
+  uint8_t* used = xcalloc((size_t)n, 1);
+  for (;;) {
+    ++s.phases;
+    int64_t blossoms = 0;
+    int32_t nroots = 0;
+    for (int32_t v = 0; v < n; ++v) {
+      const int ex = mate[v] < 0;
+      root[v] = ex ? v : -1;
+      lvl[v] = ex ? 0 : -1;
+      par[v] = -1;
+      claim[v] = NO_KEY;
+      best[v] = INT32_MAX;
+      acc[v] = -1;
+      bridge[v] = NO_KEY;
+      used[v] = 0;
+      if (ex) roots[nroots++] = v;
+    }
+    if (nroots < 2) break;
+    memcpy(front, roots, sizeof(int32_t) * (size_t)nroots);
+    int32_t fn = nroots, level = 0;
+    int64_t found = 0;
+    while (fn > 0) {
+      for (;;) {
+        for (int32_t i = 0; i < nroots; ++i) best[roots[i]] = INT32_MAX;
+        for (int32_t i = 0; i < fn; ++i) {
+          const int32_t v = front[i];
+          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+            const int32_t w = col[j];
+            if (lvl[w] < 0 || (lvl[w] & 1) || w == v) continue;
+            const int32_t rv = root[v], rw = root[w];
+            if (rv == rw) {
+              ++blossoms;
+              continue;
+            }
+            if (used[rv] || used[rw]) continue;
+            if (rw < best[rv]) best[rv] = rw;
+            if (rv < best[rw]) best[rw] = rv;
+          }
+        }
+        int32_t np = 0;
+        for (int32_t i = 0; i < nroots; ++i) {
+          const int32_t t = roots[i], b = best[t];
+          if (b != INT32_MAX && t < b && best[b] == t) {
+            acc[t] = b;
+            pairs[np++] = t;
+          }
+        }
+        if (np == 0) break;
+        for (int32_t i = 0; i < fn; ++i) {
+          const int32_t v = front[i];
+          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+            const int32_t w = col[j];
+            if (lvl[w] < 0 || (lvl[w] & 1)) continue;
+            const int32_t rv = root[v], rw = root[w];
+            if (rv == rw) continue;
+            const int32_t lo = rv < rw ? rv : rw, hi = rv < rw ? rw : rv;
+            if (acc[lo] != hi) continue;
+            const uint32_t a = (uint32_t)(rv == lo ? v : w), c = (uint32_t)(rv == lo ? w : v);
+            const uint64_t key = ((uint64_t)a << 32) | c;
+            if (key < bridge[lo]) bridge[lo] = key;
+          }
+        }
+        for (int32_t i = 0; i < np; ++i) {
+          const int32_t t = pairs[i];
+          const int32_t a = (int32_t)(bridge[t] >> 32), c = (int32_t)(bridge[t] & 0xFFFFFFFFu);
+          for (int side = 0; side < 2; ++side) {
+            int32_t cur = side ? c : a;
+            while (par[cur] >= 0) {
+              const int32_t odd = par[cur], up = par[odd];
+              mate[odd] = up;
+              mate[up] = odd;
+              cur = up;
+            }
+          }
+          mate[a] = c;
+          mate[c] = a;
+          used[t] = 1;
+          used[acc[t]] = 1;
+        }
+        found += np;
+      }
+      if (found) break;
+      int32_t tn = 0;
+      for (int32_t i = 0; i < fn; ++i) {
+        const int32_t v = front[i];
+        const uint64_t key = ((uint64_t)(uint32_t)root[v] << 32) | (uint32_t)v;
+        for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+          const int32_t w = col[j];
+          if (lvl[w] >= 0) continue;
+          if (claim[w] == NO_KEY) claimed[tn++] = w;
+          if (key < claim[w]) claim[w] = key;
+        }
+      }
+      int32_t nn = 0;
+      for (int32_t i = 0; i < tn; ++i) {
+        const int32_t w = claimed[i], x = mate[w];
+        const uint64_t kw = claim[w], kx = claim[x];
+        if (kw < kx || (kw == kx && w < x)) {
+          const int32_t rv = (int32_t)(kw >> 32);
+          root[w] = rv;
+          par[w] = (int32_t)(kw & 0xFFFFFFFFu);
+          lvl[w] = level + 1;
+          root[x] = rv;
+          par[x] = w;
+          lvl[x] = level + 2;
+          next[nn++] = x;
+        }
+      }
+      int32_t* t = front;
+      front = next;
+      next = t;
+      fn = nn;
+      level += 2;
+      ++s.levels;
+    }
+    s.augmentations += found;
+    if (!found) {
+      s.blossom_edges = blossoms;
+      break;
+    }
+  }
+  s.matched_vertices = count_matched(n, mate);
+  free(root);
+  free(lvl);
+  free(par);
+  free(best);
+  free(acc);
+  free(roots);
+  free(front);
+  free(next);
+  free(claimed);
+  free(pairs);
+  free(claim);
+  free(bridge);
+  free(used);
+  if (st) *st = s;
+  return s.matched_vertices;
+}
+
+/* ---- Edmonds' blossom algorithm ----------------------------------------- */
+
+typedef struct {
+  int32_t n;
+  const int32_t *off, *col;
+  int32_t *match, *p, *base, *queue;
+  uint8_t *used, *blossom, *seen;
+  int32_t qh, qt;
+} Edm;
+
+static int32_t edm_lca(Edm* e, int32_t a, int32_t b) {
+  memset(e->seen, 0, (size_t)e->n);
+  for (;;) {
+    a = e->base[a];
+    e->seen[a] = 1;
+    if (e->match[a] < 0) break;
+    a = e->p[e->match[a]];
+  }
+  for (;;) {
+    b = e->base[b];
+    if (e->seen[b]) return b;
+    b = e->p[e->match[b]];
+  }
+}
+
+static void edm_mark(Edm* e, int32_t v, int32_t b, int32_t child) {
+  while (e->base[v] != b) {
+    e->blossom[e->base[v]] = 1;
+    e->blossom[e->base[e->match[v]]] = 1;
+    e->p[v] = child;
+    child = e->match[v];
+    v = e->p[e->match[v]];
+  }
+}
+
+/* BFS from root over alternating paths; returns an exposed end or -1. */
+static int32_t edm_find(Edm* e, int32_t root) {
+  const int32_t n = e->n;
+  memset(e->used, 0, (size_t)n);
+  for (int32_t i = 0; i < n; ++i) {
+    e->p[i] = -1;
+    e->base[i] = i;
+  }
+  e->used[root] = 1;
+  e->qh = e->qt = 0;
+  e->queue[e->qt++] = root;
+  while (e->qh < e->qt) {
+    const int32_t v = e->queue[e->qh++];
+    for (int32_t j = e->off[v]; j < e->off[v + 1]; ++j) {
+      const int32_t to = e->col[j];
+      if (to == v || e->base[v] == e->base[to] || e->match[v] == to) continue;
+      if (to == root || (e->match[to] >= 0 && e->p[e->match[to]] >= 0)) {
+        const int32_t cb = edm_lca(e, v, to);
+        memset(e->blossom, 0, (size_t)n);
+        edm_mark(e, v, cb, to);
+        edm_mark(e, to, cb, v);
+        for (int32_t i = 0; i < n; ++i) {
+          if (e->blossom[e->base[i]]) {
+            e->base[i] = cb;
+            if (!e->used[i]) {
+              e->used[i] = 1;
+              e->queue[e->qt++] = i;
+            }
+          }
+        }
+      } else if (e->p[to] < 0) {
+        e->p[to] = v;
+        if (e->match[to] < 0) return to;
+        e->used[e->match[to]] = 1;
+        e->queue[e->qt++] = e->match[to];
+      }
+    }
+  }
+  return -1;
+}
+
+int64_t oracle_match_edmonds(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate) {
+  Edm e;
+  e.n = n;
+  e.off = off;
+  e.col = col;
+  e.match = mate;
+  e.p = xcalloc((size_t)n, sizeof(int32_t));
+  e.base = xcalloc((size_t)n, sizeof(int32_t));
+  e.queue = xcalloc((size_t)n + 1, sizeof(int32_t));
+  e.used = xcalloc((size_t)n, 1);
+  e.blossom = xcalloc((size_t)n, 1);
+  e.seen = xcalloc((size_t)n, 1);
+  for (int32_t v = 0; v < n; ++v) mate[v] = -1;
+  for (int32_t v = 0; v < n; ++v) {
+    if (mate[v] >= 0) continue;
+    int32_t u = edm_find(&e, v);
+    while (u >= 0) {
+      const int32_t pv = e.p[u], ppv = mate[pv];
+      mate[u] = pv;
+      mate[pv] = u;
+      u = ppv;
+    }
+  }
+  free(e.p);
+  free(e.base);
+  free(e.queue);
+  free(e.used);
+  free(e.blossom);
+  free(e.seen);
+  return count_matched(n, mate);
+}
+
+/* ---- Hopcroft-Karp -------------------------------------------------------- */
+
+int64_t oracle_match_hk(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate) {
+  int32_t* color = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* queue = xcalloc((size_t)n + 1, sizeof(int32_t));
+  int32_t* dist = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* it = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* stk = xcalloc((size_t)n + 1, sizeof(int32_t));
+  int32_t* via = xcalloc((size_t)n + 1, sizeof(int32_t));
+  int64_t result = 0;
+  for (int32_t v = 0; v < n; ++v) color[v] = -1;
+  for (int32_t s = 0; s < n && result >= 0; ++s) {
+    if (color[s] >= 0) continue;
+    color[s] = 0;
+    int32_t qh = 0, qt = 0;
+    queue[qt++] = s;
+    while (qh < qt && result >= 0) {
+      const int32_t v = queue[qh++];
+      for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+        const int32_t w = col[j];
+        if (color[w] < 0) {
+          color[w] = color[v] ^ 1;
+          queue[qt++] = w;
+        } else if (color[w] == color[v]) {
+          result = -1;
+          break;
+        }
+      }
+    }
+  }
+  if (result < 0) goto done;
+  for (int32_t v = 0; v < n; ++v) mate[v] = -1;
+  const int32_t INF = INT32_MAX;
+  for (;;) {
+    int32_t qh = 0, qt = 0, found = 0;
+    for (int32_t u = 0; u < n; ++u) {
+      if (color[u] != 0) continue;
+      if (mate[u] < 0) {
+        dist[u] = 0;
+        queue[qt++] = u;
+      } else {
+        dist[u] = INF;
+      }
+    }
+    while (qh < qt) {
+      const int32_t u = queue[qh++];
+      for (int32_t j = off[u]; j < off[u + 1]; ++j) {
+        const int32_t x = mate[col[j]];
+        if (x < 0) {
+          found = 1;
+        } else if (dist[x] == INF) {
+          dist[x] = dist[u] + 1;
+          queue[qt++] = x;
+        }
+      }
+    }
+    if (!found) break;
+    for (int32_t u = 0; u < n; ++u) it[u] = off[u];
+    for (int32_t u0 = 0; u0 < n; ++u0) {
+      if (color[u0] != 0 || mate[u0] >= 0) continue;
+      int32_t top = 0;
+      stk[0] = u0;
+      while (top >= 0) {
+        const int32_t x = stk[top];
+        if (it[x] == off[x + 1]) {
+          dist[x] = INF;
+          --top;
+          continue;
+        }
+        const int32_t w = col[it[x]++];
+        const int32_t y = mate[w];
+        if (y < 0) {
+          via[top] = w;
+          for (int32_t k = 0; k <= top; ++k) {
+            mate[stk[k]] = via[k];
+            mate[via[k]] = stk[k];
+          }
+          break;
+        }
+        if (dist[y] != INF && dist[y] == dist[x] + 1) {
+          via[top] = w;
+          stk[++top] = y;
+        }
+      }
+    }
+  }
+  result = count_matched(n, mate);
+done:
+  free(color);
+  free(queue);
+  free(dist);
+  free(it);
+  free(stk);
+  free(via);
+  return result;
+}

@@ -18,6 +18,11 @@ _P64 = ctypes.POINTER(ctypes.c_int64)
 _lib = None
 
 
+class MatchStats(ctypes.Structure):
+    _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
+                ("blossom_edges", ctypes.c_int64), ("phases", ctypes.c_int32), ("levels", ctypes.c_int32)]
+
+
 def load():
     global _lib
     if _lib is None:
@@ -35,6 +40,14 @@ def load():
         lib.oracle_edge_map.restype = I64
         lib.oracle_exclusive_scan.argtypes = [_P32, I64, _P64]
         lib.oracle_exclusive_scan.restype = I64
+        lib.oracle_match_forest.argtypes = [I32, _P32, _P32, _P32, ctypes.POINTER(MatchStats)]
+        lib.oracle_match_forest.restype = I64
+        lib.oracle_match_edmonds.argtypes = [I32, _P32, _P32, _P32]
+        lib.oracle_match_edmonds.restype = I64
+        lib.oracle_match_hk.argtypes = [I32, _P32, _P32, _P32]
+        lib.oracle_match_hk.restype = I64
+        lib.oracle_match_validate.argtypes = [I32, _P32, _P32, _P32]
+        lib.oracle_match_validate.restype = I64
         _lib = lib
     return _lib
 
@@ -80,3 +93,48 @@ def exclusive_scan(degrees):
     out = np.empty(d.size + 1, dtype=np.int64)
     lib.oracle_exclusive_scan(_p(d), d.size, out.ctypes.data_as(_P64))
     return out
+
+
+def _csr(row_offsets, col_indices):
+    ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
+    ci = np.ascontiguousarray(col_indices, dtype=np.int32)
+    if ci.size == 0:
+        ci = np.zeros(1, dtype=np.int32)
+    return ro, ci, ro.size - 1
+
+
+def match_forest(row_offsets, col_indices, init_mate=None):
+    """Deterministic alternating-forest matching: (mate, MatchStats); -1 = exposed."""
+    lib = load()
+    ro, ci, n = _csr(row_offsets, col_indices)
+    mate = np.full(max(n, 1), -1, dtype=np.int32)
+    if init_mate is not None:
+        mate[:n] = np.asarray(init_mate, dtype=np.int32)
+    st = MatchStats()
+    lib.oracle_match_forest(n, _p(ro), _p(ci), _p(mate), ctypes.byref(st))
+    return mate[:n], st
+
+
+def match_edmonds(row_offsets, col_indices):
+    """Maximum cardinality matching (Edmonds' blossom algorithm), -1 = exposed."""
+    lib = load()
+    ro, ci, n = _csr(row_offsets, col_indices)
+    mate = np.full(max(n, 1), -1, dtype=np.int32)
+    lib.oracle_match_edmonds(n, _p(ro), _p(ci), _p(mate))
+    return mate[:n]
+
+
+def match_hk(row_offsets, col_indices):
+    """Maximum matching of a bipartite graph (Hopcroft-Karp); None if not bipartite."""
+    lib = load()
+    ro, ci, n = _csr(row_offsets, col_indices)
+    mate = np.full(max(n, 1), -1, dtype=np.int32)
+    r = lib.oracle_match_hk(n, _p(ro), _p(ci), _p(mate))
+    return None if r < 0 else mate[:n]
+
+
+def match_violations(row_offsets, col_indices, mate):
+    lib = load()
+    ro, ci, n = _csr(row_offsets, col_indices)
+    m = np.ascontiguousarray(mate, dtype=np.int32)
+    return int(lib.oracle_match_validate(n, _p(ro), _p(ci), _p(m)))

@@ -3,7 +3,8 @@
 // (apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:57-73): load a CSR from the two
 // text files, build the engine, run, check.  Here the check is the closed form
 // of BFS on an R x C grid (level = Manhattan distance, parent = lowest-id
-// neighbour one step closer), bit-exact, in both engine modes.
+// neighbour one step closer), bit-exact, in both engine modes; and a
+// maximum matching of the same grid through FindMaximumMatch.
 //
 // Usage: shim_test <tmpdir>      exit 0 = pass
 #include <algorithm>
@@ -83,6 +84,23 @@ int main(int argc, char** argv) {
     threw = e.code() == XB_ERR_SOURCE_RANGE;
   }
   EXPECT(threw);
+  // FindMaximumMatch as the reference runner calls it (run_xb_pp_by_dataset.cu:
+  // 42-44): initial matching all nnodes.  The grid is bipartite with an odd
+  // vertex count, so a maximum matching leaves exactly one vertex exposed.
+  const zblossom::Matching m = engine.FindMaximumMatch(zblossom::Matching(n, n));
+  int exposed = 0;
+  for (int v = 0; v < n; ++v) {
+    if (m[v] == n) {
+      ++exposed;
+      continue;
+    }
+    const int d = std::abs(m[v] - v);
+    EXPECT(m[v] >= 0 && m[v] < n && m[m[v]] == v && (d == C || (d == 1 && m[v] / C == v / C)));
+  }
+  EXPECT(exposed == 1);
+  EXPECT(engine.LastMatchStats().blossom_edges == 0);
+  EXPECT(engine.LastMatchStats().matched_vertices == n - 1);
+  EXPECT(engine.FindMaximumMatch() == m);  // deterministic
   if (fails) return 1;
   std::printf("shim_test ok\n");
   return 0;

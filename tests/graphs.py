@@ -77,6 +77,69 @@ def two_components(a, b):
     return csr_from_adj(adj)
 
 
+def complete_bipartite(a, b):
+    """K_{a,b}: left 0..a-1, right a..a+b-1."""
+    return csr_from_adj([list(range(a, a + b)) for _ in range(a)] + [list(range(a)) for _ in range(b)])
+
+
+def petersen():
+    outer = [[(i + 1) % 5, (i - 1) % 5, i + 5] for i in range(5)]
+    inner = [[5 + (i + 2) % 5, 5 + (i - 2) % 5, i] for i in range(5)]
+    return csr_from_adj(outer + inner)
+
+
+def from_edges(n, u, v):
+    """Symmetric CSR (ascending, deduplicated, no self-loops) from an edge list."""
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    keep = u != v
+    a = np.concatenate([u[keep], v[keep]])
+    b = np.concatenate([v[keep], u[keep]])
+    key = np.unique(a * n + b)
+    src, dst = key // n, key % n
+    ro = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(ro, src + 1, 1)
+    return np.cumsum(ro).astype(np.int32), dst.astype(np.int32)
+
+
+def gnp(n, p, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.binomial(n * (n - 1) // 2, p)
+    return from_edges(n, rng.integers(0, n, m), rng.integers(0, n, m))
+
+
+def random_bipartite(nl, nr, avg_deg, seed):
+    """Left 0..nl-1, right nl..nl+nr-1, about avg_deg edges per left vertex."""
+    rng = np.random.default_rng(seed)
+    m = int(nl * avg_deg)
+    return from_edges(nl + nr, rng.integers(0, nl, m), nl + rng.integers(0, nr, m))
+
+
+def matching_size(mate):
+    mate = np.asarray(mate)
+    return int(np.count_nonzero(mate >= 0)) // 2
+
+
+def check_matching(ro, ci, mate):
+    """Asserts the matching rules (ValidateMatchingGPU, apps/xb/test/include/
+    common/test_utils.h:8-19, with -1 for exposed): symmetric, in range, and
+    every pair an edge of the graph."""
+    mate = np.asarray(mate)
+    n = ro.size - 1
+    assert mate.shape == (n,)
+    assert np.all(mate >= -1) and np.all(mate < n)
+    m = np.flatnonzero(mate >= 0)
+    assert np.array_equal(mate[mate[m]], m), "matching not symmetric"
+    assert not np.any(mate[m] == m), "vertex matched to itself"
+    if m.size:
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ro))
+        keys = rows * n + ci.astype(np.int64)  # ascending: rows ascending, lists ascending
+        q = m.astype(np.int64) * n + mate[m]
+        k = np.minimum(np.searchsorted(keys, q), keys.size - 1)
+        bad = m[keys[k] != q]
+        assert bad.size == 0, f"pair ({bad[0]}, {mate[bad[0]]}) is not an edge"
+
+
 # ---- closed forms (level, parent) ---------------------------------------
 
 def expect_path(n, s):

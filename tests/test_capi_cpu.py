@@ -118,6 +118,13 @@ def test_server_defaults_and_null_arguments():
     assert lib.xb_server_destroy(None) == xb.XB_OK
 
 
+def test_match_null_arguments():
+    lib = xb.load_engine()
+    mate = np.zeros(4, dtype=np.int32)
+    st = xb.MatchStats()
+    assert lib.xb_match_run(None, None, xb._ptr(mate), 0, ctypes.byref(st)) == xb.XB_ERR_INVALID_ARG
+
+
 def test_binary_csr_round_trip_and_corruption(tmp_path):
     g = xb.gen_rmat(10, 8, 0.57, 0.19, 0.19, seed=3)
     path = str(tmp_path / "g.xbcsr")

@@ -1,0 +1,165 @@
+"""Forest mode on the GPU (xb_match_run): bit-exact against the oracle.
+
+The engine's matching, and its phase / level / augmentation / blossom-edge
+counts, must equal the deterministic CPU restatement's entry for entry, on the
+graphs the oracle is pinned on and on larger random graphs; at full size the
+size-independent properties hold (a valid matching, maximum on bipartite
+graphs by Hopcroft-Karp, identical on repeated runs).  The reference's own
+matching test asserts validity and the maximum size
+(apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:56-74).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import graphs as tg
+import oracle
+import xbmsbfs as xb
+
+pytestmark = pytest.mark.gpu
+
+STATS = ("matched_vertices", "augmentations", "blossom_edges", "phases", "levels")
+
+
+def _same(ro, ci, init=None, label=""):
+    hg = xb.HGraph(ro, ci)
+    got, gs = xb.match(hg, init)
+    want, ws = oracle.match_forest(ro, ci, init)
+    assert np.array_equal(got, want), f"{label}: mate differs at {np.flatnonzero(got != want)[:8]}"
+    for k in STATS:
+        assert getattr(gs, k) == getattr(ws, k), f"{label}: {k} {getattr(gs, k)} != {getattr(ws, k)}"
+    tg.check_matching(ro, ci, got)
+    hg.close()
+    return got, gs
+
+
+def _graphs():
+    yield "path9", tg.path(9)
+    yield "path10", tg.path(10)
+    yield "cycle10", tg.cycle(10)
+    yield "cycle11", tg.cycle(11)
+    yield "complete8", tg.complete(8)
+    yield "star9", tg.star(9)
+    yield "grid37x53", tg.grid(37, 53)
+    yield "K5,9", tg.complete_bipartite(5, 9)
+    yield "petersen", tg.petersen()
+    yield "binary_tree127", tg.binary_tree(127)
+    yield "two_paths", tg.two_components(7, 8)
+    for name, ro, ci, _s, _l, _p, directed in tg.golden_cases():
+        if not directed:
+            yield name, (ro, ci)
+
+
+@pytest.mark.parametrize("name,csr", list(_graphs()), ids=[n for n, _ in _graphs()])
+def test_small_graphs_equal_oracle(name, csr):
+    _same(*csr, label=name)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_bipartite_equal_oracle_and_maximum(seed):
+    nl, nr = 20_000 + 5_000 * seed, 18_000 + 7_000 * seed
+    ro, ci = tg.random_bipartite(nl, nr, 1.0 + 0.7 * seed, 40 + seed)
+    got, st = _same(ro, ci, label=f"bipartite seed {seed}")
+    assert st.blossom_edges == 0
+    assert tg.matching_size(got) == tg.matching_size(oracle.match_hk(ro, ci))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_general_equal_oracle(seed):
+    n = 2_000 + 3_000 * seed
+    ro, ci = tg.gnp(n, (1.2 + 0.6 * seed) / n, 70 + seed)
+    got, st = _same(ro, ci, label=f"gnp seed {seed}")
+    if n <= 5_000:
+        best = tg.matching_size(oracle.match_edmonds(ro, ci))
+        assert tg.matching_size(got) <= best
+        if st.blossom_edges == 0:
+            assert tg.matching_size(got) == best
+
+
+def test_rmat_equal_oracle():
+    g = xb.gen_rmat(16, 8, 0.57, 0.19, 0.19, seed=21)
+    _same(g.row_offsets, g.col_indices, label="rmat16")
+
+
+def test_initial_matching_host_and_device():
+    ro, ci = tg.random_bipartite(30_000, 25_000, 2.0, 9)
+    n = ro.size - 1
+    init = np.full(n, -1, dtype=np.int32)
+    # Greedy start: every left vertex takes its first free neighbour.
+    for v in range(30_000):
+        for w in ci[ro[v]:ro[v + 1]]:
+            if init[w] < 0:
+                init[v], init[w] = w, v
+                break
+    got, st = _same(ro, ci, init, label="greedy start")
+    assert st.augmentations == tg.matching_size(got) - tg.matching_size(init)
+    hg = xb.HGraph(ro, ci)
+    d_init = xb.DeviceBuffer((n,))
+    d_init.upload(init)
+    d_out = xb.DeviceBuffer((n,))
+    _, ds = xb.match(hg, d_init.ptr, d_out.ptr, device=True)
+    assert np.array_equal(d_out.numpy(), got) and ds.augmentations == st.augmentations
+    # In place on the device (init and output the same buffer).
+    xb.match(hg, d_init.ptr, d_init.ptr, device=True)
+    assert np.array_equal(d_init.numpy(), got)
+    d_init.free()
+    d_out.free()
+    hg.close()
+
+
+def test_find_maximum_match_shim_sentinel():
+    ro, ci = tg.grid(41, 41)  # 1681 vertices, bipartite: one exposed
+    n = ro.size - 1
+    eng = xb.XBlossomGpuEngine10(xb.HGraph(ro, ci))
+    m = eng.FindMaximumMatch(np.full(n, n, dtype=np.int32))  # the reference runner's initial matching
+    assert np.count_nonzero(m == n) == 1
+    mm = m.copy()
+    mm[mm == n] = -1
+    tg.check_matching(ro, ci, mm)
+    assert eng.last_match_stats.blossom_edges == 0
+    assert np.array_equal(eng.FindMaximumMatch(), m)
+
+
+def test_rejections():
+    ro, ci = tg.path(6)
+    hg = xb.HGraph(ro, ci)
+    for bad in ([2, -1, 0, -1, -1, -1],   # (0, 2) is not an edge
+                [1, 2, -1, -1, -1, -1],   # asymmetric
+                [1, 0, 7, -1, -1, -1],    # out of range
+                [-3, -1, -1, -1, -1, -1]):
+        with pytest.raises(xb.XbError) as ei:
+            xb.match(hg, np.array(bad, dtype=np.int32))
+        assert ei.value.code == xb.XB_ERR_INVALID_ARG
+    lib = xb.load_engine()
+    out = np.zeros(6, dtype=np.int32)
+    assert lib.xb_match_run(hg.handle, None, xb._ptr(out), 8, None) == xb.XB_ERR_INVALID_ARG
+    hg.close()
+    d = xb.HGraph(np.array([0, 1, 1], dtype=np.int32), np.array([1], dtype=np.int32), directed=True)
+    with pytest.raises(xb.XbError) as ei:
+        xb.match(d)
+    assert ei.value.code == xb.XB_ERR_UNSUPPORTED
+    d.close()
+
+
+def test_edgeless_and_tiny():
+    ro = np.zeros(6, dtype=np.int32)
+    ci = np.zeros(0, dtype=np.int32)
+    got, st = _same(ro, ci, label="edgeless")
+    assert np.all(got == -1) and st.phases == 1
+    _same(*tg.path(2), label="one edge")
+
+
+def test_full_size_bipartite_properties():
+    """1.1M vertices, 3.6M edges: valid, maximum (Hopcroft-Karp), identical on reruns."""
+    ro, ci = tg.random_bipartite(600_000, 500_000, 3.0, 1)
+    hg = xb.HGraph(ro, ci)
+    a, st = xb.match(hg)
+    b, _ = xb.match(hg)
+    assert np.array_equal(a, b)
+    tg.check_matching(ro, ci, a)
+    assert st.blossom_edges == 0
+    assert tg.matching_size(a) == tg.matching_size(oracle.match_hk(ro, ci))
+    want, ws = oracle.match_forest(ro, ci)
+    assert np.array_equal(a, want) and st.levels == ws.levels
+    hg.close()

@@ -1,0 +1,172 @@
+"""Pins the matching oracle (oracle/match_oracle.c) before it checks the engine.
+
+* Edmonds' algorithm: closed forms (paths, cycles, complete graphs, stars,
+  grids, complete bipartite graphs, the Petersen graph) and networkx's maximum
+  cardinality matching on random graphs.
+* Hopcroft-Karp: closed forms, scipy's maximum_bipartite_matching, and the
+  refusal of non-bipartite input.
+* The deterministic forest restatement (the engine's checker): valid
+  matchings everywhere; maximum on every bipartite graph; maximum whenever its
+  final phase met no same-tree even-even edge (the Hungarian-forest
+  certificate), checked against Edmonds; never larger than maximum.
+The reference's own matching tests assert validity and the maximum size on
+datasets that are not shipped (apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:
+56-74); the closed forms and library results stand in for those sizes.
+"""
+import numpy as np
+import pytest
+
+import graphs as tg
+import oracle
+
+CLOSED = {
+    "path9": (lambda: tg.path(9), 4, True),
+    "path10": (lambda: tg.path(10), 5, True),
+    "cycle10": (lambda: tg.cycle(10), 5, True),
+    "cycle11": (lambda: tg.cycle(11), 5, False),
+    "complete7": (lambda: tg.complete(7), 3, False),
+    "complete8": (lambda: tg.complete(8), 4, False),
+    "star9": (lambda: tg.star(9), 1, True),
+    "grid5x7": (lambda: tg.grid(5, 7), 17, True),
+    "grid6x6": (lambda: tg.grid(6, 6), 18, True),
+    "K3,8": (lambda: tg.complete_bipartite(3, 8), 3, True),
+    "K6,6": (lambda: tg.complete_bipartite(6, 6), 6, True),
+    "petersen": (lambda: tg.petersen(), 5, False),
+    "binary_tree15": (lambda: tg.binary_tree(15), 5, True),  # the 5 even-depth vertices cover every edge
+    "two_paths": (lambda: tg.two_components(5, 6), 5, True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CLOSED))
+def test_edmonds_closed_forms(name):
+    mk, size, _ = CLOSED[name]
+    ro, ci = mk()
+    mate = oracle.match_edmonds(ro, ci)
+    tg.check_matching(ro, ci, mate)
+    assert tg.matching_size(mate) == size
+
+
+@pytest.mark.parametrize("name", sorted(CLOSED))
+def test_hk_closed_forms(name):
+    mk, size, bip = CLOSED[name]
+    ro, ci = mk()
+    mate = oracle.match_hk(ro, ci)
+    if not bip:
+        assert mate is None
+        return
+    tg.check_matching(ro, ci, mate)
+    assert tg.matching_size(mate) == size
+
+
+@pytest.mark.parametrize("name", sorted(CLOSED))
+def test_forest_closed_forms(name):
+    mk, size, bip = CLOSED[name]
+    ro, ci = mk()
+    mate, st = oracle.match_forest(ro, ci)
+    tg.check_matching(ro, ci, mate)
+    got = tg.matching_size(mate)
+    assert st.matched_vertices == 2 * got
+    assert got <= size
+    if bip or st.blossom_edges == 0:
+        assert got == size
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_edmonds_against_networkx(seed):
+    nx = pytest.importorskip("networkx")
+    n = 40 + 30 * seed
+    ro, ci = tg.gnp(n, 2.5 / n, seed)
+    mate = oracle.match_edmonds(ro, ci)
+    tg.check_matching(ro, ci, mate)
+    G = nx.Graph()
+    G.add_nodes_from(range(n))
+    for v in range(n):
+        G.add_edges_from((v, int(w)) for w in ci[ro[v]:ro[v + 1]])
+    assert tg.matching_size(mate) == len(nx.max_weight_matching(G, maxcardinality=True))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_hk_against_scipy(seed):
+    sp = pytest.importorskip("scipy.sparse")
+    csgraph = pytest.importorskip("scipy.sparse.csgraph")
+    nl, nr = 300 + 50 * seed, 250 + 70 * seed
+    ro, ci = tg.random_bipartite(nl, nr, 1.0 + 0.5 * seed, seed)
+    mate = oracle.match_hk(ro, ci)
+    tg.check_matching(ro, ci, mate)
+    rows = np.repeat(np.arange(nl), np.diff(ro)[:nl])
+    cols = ci[:ro[nl]] - nl
+    bi = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=(nl, nr))
+    ref = csgraph.maximum_bipartite_matching(bi, perm_type="column")
+    assert tg.matching_size(mate) == int(np.count_nonzero(ref >= 0))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_forest_maximum_on_bipartite(seed):
+    nl, nr = 200 + 40 * seed, 220 + 30 * seed
+    ro, ci = tg.random_bipartite(nl, nr, 0.8 + 0.4 * seed, 100 + seed)
+    mate, st = oracle.match_forest(ro, ci)
+    tg.check_matching(ro, ci, mate)
+    assert st.blossom_edges == 0
+    assert tg.matching_size(mate) == tg.matching_size(oracle.match_hk(ro, ci))
+    assert st.augmentations == tg.matching_size(mate)
+
+
+def test_forest_certificate_on_general_graphs():
+    """Maximum whenever the final phase met no blossom edge; both outcomes occur."""
+    certified = uncertified = short = 0
+    for seed in range(60):
+        n = 30 + 7 * (seed % 10)
+        ro, ci = tg.gnp(n, (1.5 + (seed % 4)) / n, 1000 + seed)
+        mate, st = oracle.match_forest(ro, ci)
+        tg.check_matching(ro, ci, mate)
+        best = tg.matching_size(oracle.match_edmonds(ro, ci))
+        got = tg.matching_size(mate)
+        assert got <= best
+        if st.blossom_edges == 0:
+            assert got == best, f"seed {seed}: certified but not maximum"
+            certified += 1
+        else:
+            uncertified += 1
+            short += got < best
+    assert certified > 0 and uncertified > 0
+    assert short > 0  # some graphs need the blossom transform
+
+
+def test_forest_from_an_initial_matching():
+    ro, ci = tg.random_bipartite(500, 450, 2.0, 7)
+    n = ro.size - 1
+    # Greedy initial matching in vertex order.
+    init = np.full(n, -1, dtype=np.int32)
+    for v in range(n):
+        if init[v] >= 0:
+            continue
+        for w in ci[ro[v]:ro[v + 1]]:
+            if init[w] < 0:
+                init[v], init[w] = w, v
+                break
+    tg.check_matching(ro, ci, init)
+    mate, st = oracle.match_forest(ro, ci, init)
+    tg.check_matching(ro, ci, mate)
+    assert tg.matching_size(mate) == tg.matching_size(oracle.match_hk(ro, ci))
+    assert st.augmentations == tg.matching_size(mate) - tg.matching_size(init)
+    # A maximum matching as the start: one phase, nothing to flip.
+    again, st2 = oracle.match_forest(ro, ci, mate)
+    assert np.array_equal(again, mate) and st2.phases == 1 and st2.augmentations == 0
+
+
+def test_violation_counter():
+    ro, ci = tg.path(6)
+    good = np.array([1, 0, 3, 2, 5, 4], dtype=np.int32)
+    assert oracle.match_violations(ro, ci, good) == 0
+    assert oracle.match_violations(ro, ci, np.array([2, -1, 0, -1, -1, -1], dtype=np.int32)) == 2  # not an edge
+    assert oracle.match_violations(ro, ci, np.array([1, 2, -1, -1, -1, -1], dtype=np.int32)) == 2  # asymmetric
+    assert oracle.match_violations(ro, ci, np.array([-2, -1, -1, -1, -1, -1], dtype=np.int32)) == 1
+
+
+def test_edge_cases():
+    for ro, ci in [(np.zeros(1, np.int32), np.zeros(0, np.int32)),
+                   (np.zeros(5, np.int32), np.zeros(0, np.int32)),
+                   tg.path(2), tg.path(3)]:
+        mate, st = oracle.match_forest(ro, ci)
+        tg.check_matching(ro, ci, mate)
+        assert tg.matching_size(mate) == tg.matching_size(oracle.match_edmonds(ro, ci))

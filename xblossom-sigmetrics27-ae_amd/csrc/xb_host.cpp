@@ -1792,6 +1792,172 @@ int32_t xb_msbfs_run_batched(xb_graph* const* graphs, int32_t G, const int32_t* 
 }  // extern "C"
 
 // ===========================================================================
+// Forest mode: maximum matching by alternating forests
+// ===========================================================================
+namespace {
+
+// Device arrays of one matching run (one entry per vertex unless noted).
+struct MatchState {
+  int32_t* mate = nullptr;
+  int32_t* root = nullptr;
+  int32_t* lvl = nullptr;
+  int32_t* par = nullptr;
+  int32_t* best = nullptr;
+  int32_t* acc = nullptr;
+  int32_t* roots = nullptr;  // the phase's even roots (exposed vertices)
+  int32_t* pairs = nullptr;  // the round's taken tree pairs, by lower root
+  int32_t* tlist = nullptr;  // first claims of the level
+  unsigned long long* claim = nullptr;
+  unsigned long long* bridge = nullptr;
+  uint8_t* used = nullptr;
+  // [0] blossom edges, [1] matched count / validation failures,
+  // [2] low word: taken pairs, [3] low word: first claims.
+  unsigned long long* ctl = nullptr;
+
+  explicit MatchState(int32_t n) {
+    const size_t N = (size_t)n;
+    mate = dalloc<int32_t>(N);
+    root = dalloc<int32_t>(N);
+    lvl = dalloc<int32_t>(N);
+    par = dalloc<int32_t>(N);
+    best = dalloc<int32_t>(N);
+    acc = dalloc<int32_t>(N);
+    roots = dalloc<int32_t>(N);
+    pairs = dalloc<int32_t>(N);
+    tlist = dalloc<int32_t>(N);
+    claim = dalloc<unsigned long long>(N);
+    bridge = dalloc<unsigned long long>(N);
+    used = dalloc<uint8_t>(N);
+    ctl = dalloc<unsigned long long>(4);
+  }
+  ~MatchState() {
+    for (void* p : {(void*)mate, (void*)root, (void*)lvl, (void*)par, (void*)best, (void*)acc, (void*)roots,
+                    (void*)pairs, (void*)tlist, (void*)claim, (void*)bridge, (void*)used, (void*)ctl})
+      dfree(p);
+  }
+  uint32_t* npairs() { return (uint32_t*)(ctl + 2); }
+  uint32_t* tcount() { return (uint32_t*)(ctl + 3); }
+};
+
+unsigned long long read_u64(Workspace& w, const unsigned long long* d) {
+  HIPCHK(hipMemcpyAsync(w.hctr, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  return *w.hctr;
+}
+
+// Phase loop of FindMaximumMatch (max_match_engine.h:93-113) over the
+// alternating-forest search (xblossom_gpu_engine_10.cu:72-81): grow the
+// forest level by level from all exposed vertices; at every level, take
+// rounds of tree-disjoint augmenting paths until none is left, and end the
+// phase after the first level that yields any; stop when a whole phase
+// (the forest grown until no vertex is added) yields none.
+void run_match(xb_graph* g, MatchState& ms, xb_match_stats* out) {
+  Workspace& w = g->ws;
+  ensure_queues(g, w);
+  const int32_t n = g->n;
+  const hipStream_t st = w.st;
+  xb_match_stats s{};
+  for (;;) {
+    ++s.phases;
+    HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 4, st));
+    HIPCHK(hipMemsetAsync(w.q[0].ctr, 0, sizeof(unsigned long long), st));
+    xb::launch_mt_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.claim, ms.best, ms.acc, ms.bridge,
+                       ms.used, w.q[0]);
+    int32_t qn = 0, E = 0;
+    read_ctr(w, w.q[0].ctr, &qn, &E);
+    const int32_t nroots = qn;
+    if (nroots < 2) break;  // at most one exposed vertex: no augmenting path exists
+    HIPCHK(hipMemcpyAsync(ms.roots, w.q[0].v, sizeof(int32_t) * (size_t)nroots, hipMemcpyDeviceToDevice, st));
+    int cur = 0;
+    int32_t level = 0;
+    int64_t found = 0;
+    while (qn > 0) {
+      for (;;) {
+        xb::launch_mt_reset_best(st, ms.roots, nroots, ms.best);
+        xb::launch_mt_scan(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.used, ms.best, ms.ctl);
+        HIPCHK(hipMemsetAsync(ms.npairs(), 0, sizeof(uint32_t), st));
+        xb::launch_mt_accept(st, ms.roots, nroots, ms.best, ms.acc, ms.pairs, ms.npairs());
+        const int32_t np = (int32_t)(read_u64(w, ms.ctl + 2) & 0xFFFFFFFFull);
+        if (np == 0) break;
+        xb::launch_mt_bridge(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.acc, ms.bridge);
+        xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.mate, ms.used);
+        found += np;
+      }
+      if (found) break;
+      HIPCHK(hipMemsetAsync(ms.tcount(), 0, sizeof(uint32_t), st));
+      HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
+      xb::launch_mt_claim(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.claim, ms.tlist,
+                          ms.tcount());
+      // First claims come one per frontier edge at most, and one per vertex.
+      xb::launch_mt_commit(st, ms.tlist, ms.tcount(), std::min(E, n), level + 2, ms.mate, g->d_off, ms.claim,
+                           ms.root, ms.lvl, ms.par, w.q[cur ^ 1]);
+      read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
+      cur ^= 1;
+      level += 2;
+      ++s.levels;
+    }
+    s.augmentations += found;
+    if (!found) {
+      s.blossom_edges = (int64_t)read_u64(w, ms.ctl);
+      break;
+    }
+  }
+  HIPCHK(hipMemsetAsync(ms.ctl + 1, 0, sizeof(unsigned long long), st));
+  xb::launch_mt_count(st, n, ms.mate, ms.ctl + 1);
+  s.matched_vertices = (int64_t)read_u64(w, ms.ctl + 1);
+  *out = s;
+}
+
+}  // namespace
+
+extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, uint32_t flags,
+                                xb_match_stats* stats) {
+  return guarded([&] {
+    if (!g || !mate_out) fail(XB_ERR_INVALID_ARG, "null graph or mate_out");
+    if (g->flags & XB_GRAPH_DIRECTED) fail(XB_ERR_UNSUPPORTED, "matching needs a symmetric graph");
+    if (flags & ~(uint32_t)XB_OUT_DEVICE) fail(XB_ERR_INVALID_ARG, "unknown matching flags");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const int32_t n = g->n;
+    if (n == 0) return;
+    const bool dev = (flags & XB_OUT_DEVICE) != 0;
+    const hipStream_t st = g->stream;
+    MatchState ms(n);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    struct Ev {
+      hipEvent_t a, b;
+      ~Ev() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    } evs{e0, e1};
+    HIPCHK(hipEventRecord(e0, st));
+    if (init_mate) {
+      HIPCHK(hipMemcpyAsync(ms.mate, init_mate, sizeof(int32_t) * (size_t)n,
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemsetAsync(ms.ctl + 1, 0, sizeof(unsigned long long), st));
+      xb::launch_mt_validate(st, n, ms.mate, g->d_off, g->d_col, ms.ctl + 1);
+      const unsigned long long bad = read_u64(g->ws, ms.ctl + 1);
+      if (bad) fail(XB_ERR_INVALID_ARG, "init_mate is not a matching of the graph (" + std::to_string(bad) +
+                                            " bad entries)");
+    } else {
+      HIPCHK(hipMemsetAsync(ms.mate, 0xFF, sizeof(int32_t) * (size_t)n, st));
+    }
+    xb_match_stats s{};
+    run_match(g, ms, &s);
+    HIPCHK(hipMemcpyAsync(mate_out, ms.mate, sizeof(int32_t) * (size_t)n,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms_total = 0;
+    HIPCHK(hipEventElapsedTime(&ms_total, e0, e1));
+    s.ms_total = ms_total;
+    if (stats) *stats = s;
+  });
+}
+
+// ===========================================================================
 // Autotuner: coordinate descent over the engine's tunables
 // ===========================================================================
 namespace {

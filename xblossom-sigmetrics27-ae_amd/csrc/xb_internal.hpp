@@ -180,6 +180,31 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, in
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                     int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map);
 
+// ---- forest mode (alternating-forest search of maximum matching) ----
+void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
+                    int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
+                    uint8_t* used, Queue q);
+void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                    const int32_t* root, const int32_t* lvl, const uint8_t* used, int32_t* best,
+                    unsigned long long* blossoms);
+void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* best, int32_t* acc,
+                      int32_t* pairs, uint32_t* npairs);
+void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                      const int32_t* root, const int32_t* lvl, const int32_t* acc, unsigned long long* bridge);
+void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs, int32_t max_pairs,
+                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par, int32_t* mate,
+                    uint8_t* used);
+void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best);
+void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                     const int32_t* root, const int32_t* lvl, unsigned long long* claim, int32_t* tlist,
+                     uint32_t* tcount);
+void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
+                      int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
+                      int32_t* root, int32_t* lvl, int32_t* par, Queue nxt);
+void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
+                        unsigned long long* bad);
+void launch_mt_count(hipStream_t st, int32_t n, const int32_t* mate, unsigned long long* out);
+
 // ---- utilities ----
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,
                             unsigned long long* out);

@@ -80,6 +80,12 @@ __device__ __forceinline__ int64_t gnwaves(Grid g) { return (int64_t)g.nb * (kBl
 __device__ __forceinline__ int64_t gthread(Grid g) { return (int64_t)g.bid * kBlock + threadIdx.x; }
 __device__ __forceinline__ int64_t gnthreads(Grid g) { return (int64_t)g.nb * kBlock; }
 
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
 __device__ __forceinline__ int wave_incl_scan(int x) {
   const int l = lane_id();
 #pragma unroll
@@ -1353,6 +1359,289 @@ __global__ void k_reset_ctl(TeamCtl* ctl, int32_t n_teams, TileJob* jobs, int32_
 }
 
 // ============================================================================
+// Forest mode: the alternating-forest search of maximum matching
+// ============================================================================
+//
+// One phase grows an alternating forest from every exposed vertex (the even
+// roots) and looks for edges between even vertices of different trees (an
+// augmenting path each).  Reference semantics: InitAlternatingForest,
+// FindAndFlipAugmentingPathInAlternatingForest and ExpandAlternatingForest of
+// apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:83-359, with the path
+// flip of apps/xb/src/include/path_table/path_table.h:273-330.  Every choice
+// the reference leaves to the first atomic is fixed here:
+//  * an unvisited matched vertex is claimed by the even neighbour with the
+//    lowest (root, vertex) key, and of a matched pair claimed from both
+//    sides the member with the lower (claim, id) becomes odd;
+//  * augmenting paths are chosen in rounds: every tree records its lowest
+//    neighbouring tree, mutually lowest pairs are taken, their bridge is the
+//    lowest (vertex in the lower tree, vertex in the higher tree) edge, and
+//    taken trees drop out of later rounds -- so each phase flips a maximal
+//    set of tree-disjoint paths, identically on every run.
+// Edges closing an odd cycle inside one tree (blossoms) are counted; the
+// blossom transform is not part of this engine.
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+// Resets the forest and queues every exposed vertex as an even root.
+__global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __restrict__ mate,
+                                                    const int32_t* __restrict__ off, int32_t* __restrict__ root,
+                                                    int32_t* __restrict__ lvl, int32_t* __restrict__ par,
+                                                    unsigned long long* __restrict__ claim,
+                                                    int32_t* __restrict__ best, int32_t* __restrict__ acc,
+                                                    unsigned long long* __restrict__ bridge,
+                                                    uint8_t* __restrict__ used, Queue q) {
+  AppendStage& sg = g_lds.q.sg;
+  stage_init(sg);
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < n; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = b0 + threadIdx.x;
+    bool ex = false;
+    int32_t dv = 0;
+    if (v < n) {
+      ex = mate[v] < 0;
+      root[v] = ex ? (int32_t)v : -1;
+      lvl[v] = ex ? 0 : -1;
+      par[v] = -1;
+      claim[v] = kNoKey;
+      best[v] = INT32_MAX;
+      acc[v] = -1;
+      bridge[v] = kNoKey;
+      used[v] = 0;
+      if (ex) dv = off[v + 1] - off[v];
+    }
+    stage_put(sg, ex, (int32_t)v, dv);
+    stage_sync_flush(sg, false, q);
+  }
+  stage_sync_flush(sg, true, q);
+}
+
+// Calls f(v, w) for every edge (v, w) out of the frontier queue (edge-balanced
+// chunks of kChunk edges per block, as in the push).
+template <class F>
+__device__ __forceinline__ void for_frontier_edges(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                   Queue q, int32_t qn, int32_t E, F&& f) {
+  QueueLds& L = g_lds.q;
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+    const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, L);
+#pragma unroll 1
+    for (int r = 0; r < kPushR; ++r) {
+      const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
+      if (e < c1) {
+        const int i = chunk_owner(co, q.eo, L.s_eo, e);
+        const int32_t v = q.v[i];
+        f(v, col[off[v] + (e - q.eo[i])]);
+      }
+    }
+    __syncthreads();  // the chunk's owner staging is reused by the next chunk
+  }
+}
+
+// Augmenting-path scan: every tree records its lowest neighbouring tree over
+// even-even edges (both trees not taken yet); same-tree even-even edges are
+// counted as blossom edges.
+__global__ __launch_bounds__(kBlock) void k_mt_scan(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                    Queue q, int32_t qn, int32_t E,
+                                                    const int32_t* __restrict__ root,
+                                                    const int32_t* __restrict__ lvl, const uint8_t* __restrict__ used,
+                                                    int32_t* __restrict__ best,
+                                                    unsigned long long* __restrict__ blossoms) {
+  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w) {
+    const int32_t lw = lvl[w];
+    if (lw < 0 || (lw & 1) || w == v) return;
+    const int32_t rv = root[v], rw = root[w];
+    if (rv == rw) {
+      atomicAdd(blossoms, 1ull);
+      return;
+    }
+    if (used[rv] || used[rw]) return;
+    atomicMin(&best[rv], rw);
+    atomicMin(&best[rw], rv);
+  });
+}
+
+// Mutually lowest tree pairs are taken (recorded at the lower root).
+__global__ __launch_bounds__(kBlock) void k_mt_accept(const int32_t* __restrict__ roots, int32_t nroots,
+                                                      const int32_t* __restrict__ best, int32_t* __restrict__ acc,
+                                                      int32_t* __restrict__ pairs, uint32_t* __restrict__ npairs) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nroots; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t t = roots[i];
+    const int32_t b = best[t];
+    if (b != INT32_MAX && t < b && best[b] == t) {
+      acc[t] = b;
+      pairs[atomicAdd(npairs, 1u)] = t;
+    }
+  }
+}
+
+// Clears the trees' lowest-neighbour records before the next round.
+__global__ __launch_bounds__(kBlock) void k_mt_reset_best(const int32_t* __restrict__ roots, int32_t nroots,
+                                                          int32_t* __restrict__ best) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nroots; i += (int64_t)gridDim.x * kBlock)
+    best[roots[i]] = INT32_MAX;
+}
+
+// Bridge of every taken pair: the lowest (vertex in the lower tree, vertex in
+// the higher tree) even-even edge between the two trees.
+__global__ __launch_bounds__(kBlock) void k_mt_bridge(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                      Queue q, int32_t qn, int32_t E,
+                                                      const int32_t* __restrict__ root,
+                                                      const int32_t* __restrict__ lvl, const int32_t* __restrict__ acc,
+                                                      unsigned long long* __restrict__ bridge) {
+  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w) {
+    const int32_t lw = lvl[w];
+    if (lw < 0 || (lw & 1)) return;
+    const int32_t rv = root[v], rw = root[w];
+    if (rv == rw) return;
+    const int32_t lo = rv < rw ? rv : rw, hi = rv < rw ? rw : rv;
+    if (acc[lo] != hi) return;
+    const uint32_t a = (uint32_t)(rv == lo ? v : w), c = (uint32_t)(rv == lo ? w : v);
+    atomicMin(&bridge[lo], ((unsigned long long)a << 32) | c);
+  });
+}
+
+// Flips each taken path (one thread per path; paths are vertex-disjoint):
+// from either end of the bridge up to its root, every (odd, even claimer)
+// pair becomes matched, then the bridge itself.
+__global__ __launch_bounds__(kBlock) void k_mt_flip(const int32_t* __restrict__ pairs, const uint32_t* __restrict__ npairs,
+                                                    const int32_t* __restrict__ acc,
+                                                    const unsigned long long* __restrict__ bridge,
+                                                    const int32_t* __restrict__ par, int32_t* __restrict__ mate,
+                                                    uint8_t* __restrict__ used) {
+  const uint32_t np = *npairs;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < np; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t t = pairs[i];
+    const unsigned long long key = bridge[t];
+    if (key == kNoKey) continue;  // a taken pair always has a bridge; never walk from an empty one
+    const int32_t a = (int32_t)(key >> 32), c = (int32_t)(key & 0xFFFFFFFFull);
+    for (int side = 0; side < 2; ++side) {
+      int32_t cur = side ? c : a;
+      while (par[cur] >= 0) {
+        const int32_t odd = par[cur];
+        const int32_t up = par[odd];
+        mate[odd] = up;
+        mate[up] = odd;
+        cur = up;
+      }
+    }
+    mate[a] = c;
+    mate[c] = a;
+    used[t] = 1;
+    used[acc[t]] = 1;
+  }
+}
+
+// Expansion, claim step: unvisited (hence matched) neighbours of the even
+// frontier take the lowest (root, claimer) key; first claims are listed.
+__global__ __launch_bounds__(kBlock) void k_mt_claim(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                     Queue q, int32_t qn, int32_t E,
+                                                     const int32_t* __restrict__ root,
+                                                     const int32_t* __restrict__ lvl,
+                                                     unsigned long long* __restrict__ claim,
+                                                     int32_t* __restrict__ tlist, uint32_t* __restrict__ tcount) {
+  QueueLds& L = g_lds.q;
+  AppendStage& sg = L.sg;
+  stage_init(sg);
+  for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < E; c0 += (int64_t)gridDim.x * kChunk) {
+    const int c1 = (int)((c0 + kChunk < E) ? c0 + kChunk : E);
+    const ChunkOwners co = stage_owners(q.eo, qn, (int)c0, c1, L);
+#pragma unroll 1
+    for (int r = 0; r < kPushR; ++r) {
+      const int e = (int)c0 + r * kBlock + (int)threadIdx.x;
+      bool first = false;
+      int32_t w = 0;
+      if (e < c1) {
+        const int i = chunk_owner(co, q.eo, L.s_eo, e);
+        const int32_t v = q.v[i];
+        w = col[off[v] + (e - q.eo[i])];
+        if (lvl[w] < 0) {
+          const unsigned long long key = ((unsigned long long)(uint32_t)root[v] << 32) | (uint32_t)v;
+          first = atomicMin(&claim[w], key) == kNoKey;
+        }
+      }
+      stage_put(sg, first, w, 0);
+      stage_sync_flush(sg, false, Queue{}, tlist, tcount);
+    }
+  }
+  stage_sync_flush(sg, true, Queue{}, tlist, tcount);
+}
+
+// Expansion, commit step: of a claimed matched pair (w, x) the member with
+// the lower (claim, id) becomes odd (level+1) and its mate even (level+2,
+// queued for the next level).
+__global__ __launch_bounds__(kBlock) void k_mt_commit(const int32_t* __restrict__ tlist,
+                                                      const uint32_t* __restrict__ tcount, int32_t next_level,
+                                                      const int32_t* __restrict__ mate, const int32_t* __restrict__ off,
+                                                      const unsigned long long* __restrict__ claim,
+                                                      int32_t* __restrict__ root, int32_t* __restrict__ lvl,
+                                                      int32_t* __restrict__ par, Queue nxt) {
+  AppendStage& sg = g_lds.q.sg;
+  stage_init(sg);
+  const uint32_t cnt = *tcount;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < (int64_t)cnt; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t idx = b0 + threadIdx.x;
+    bool odd = false;
+    int32_t x = 0, dx = 0;
+    if (idx < (int64_t)cnt) {
+      const int32_t w = tlist[idx];
+      x = mate[w];
+      const unsigned long long kw = claim[w], kx = claim[x];
+      odd = kw < kx || (kw == kx && w < x);
+      if (odd) {
+        const int32_t rv = (int32_t)(kw >> 32);
+        root[w] = rv;
+        par[w] = (int32_t)(kw & 0xFFFFFFFFull);
+        lvl[w] = next_level - 1;
+        root[x] = rv;
+        par[x] = w;
+        lvl[x] = next_level;
+        dx = off[x + 1] - off[x];
+      }
+    }
+    stage_put(sg, odd, x, dx);
+    stage_sync_flush(sg, false, nxt);
+  }
+  stage_sync_flush(sg, true, nxt);
+}
+
+// Validates a matching: ids in range, symmetric, and every pair an edge
+// (binary search in the sorted adjacency list); counts violations.
+__global__ __launch_bounds__(kBlock) void k_mt_validate(int32_t n, const int32_t* __restrict__ mate,
+                                                        const int32_t* __restrict__ off,
+                                                        const int32_t* __restrict__ col,
+                                                        unsigned long long* __restrict__ bad) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    const int32_t m = mate[v];
+    if (m < 0) {
+      if (m != -1) atomicAdd(bad, 1ull);
+      continue;
+    }
+    if (m >= n || m == v || mate[m] != (int32_t)v) {
+      atomicAdd(bad, 1ull);
+      continue;
+    }
+    int32_t lo = off[v], hi = off[v + 1];
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (col[mid] < m)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo >= off[v + 1] || col[lo] != m) atomicAdd(bad, 1ull);
+  }
+}
+
+// Matched vertices: mate[v] >= 0 (for the result count).
+__global__ __launch_bounds__(kBlock) void k_mt_count(int32_t n, const int32_t* __restrict__ mate,
+                                                     unsigned long long* __restrict__ out) {
+  unsigned long long c = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    c += mate[v] >= 0 ? 1ull : 0ull;
+  c = wave_sum64(c);
+  if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+// ============================================================================
 // Utilities and test hooks
 // ============================================================================
 
@@ -1604,6 +1893,74 @@ void launch_bp_persistent(hipStream_t st, int grid_blocks, const Team* teams, in
   hipLaunchKernelGGL(k_reset_ctl, dim3((nr + 255) / 256), dim3(256), 0, st, ctl, n_teams, jobs, n_jobs, gerr);
   hipLaunchKernelGGL(k_bp_persistent, dim3(grid_blocks), dim3(kBlock), 0, st, teams, n_teams, graphs, jobs, alpha,
                      gerr);
+}
+
+// ---- forest mode ----
+static int edge_grid(int32_t E) {
+  const int64_t g = ((int64_t)E + kChunk - 1) / kChunk;
+  return (int)(g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g));
+}
+static int item_grid(int64_t n) {
+  const int64_t g = (n + kBlock - 1) / kBlock;
+  return (int)(g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g));
+}
+
+void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
+                    int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
+                    uint8_t* used, Queue q) {
+  hipLaunchKernelGGL(k_mt_init, dim3(item_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, claim, best,
+                     acc, bridge, used, q);
+}
+void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                    const int32_t* root, const int32_t* lvl, const uint8_t* used, int32_t* best,
+                    unsigned long long* blossoms) {
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_mt_scan, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, used, best,
+                     blossoms);
+}
+void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* best, int32_t* acc,
+                      int32_t* pairs, uint32_t* npairs) {
+  if (nroots <= 0) return;
+  hipLaunchKernelGGL(k_mt_accept, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, best, acc, pairs,
+                     npairs);
+}
+void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                      const int32_t* root, const int32_t* lvl, const int32_t* acc, unsigned long long* bridge) {
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_mt_bridge, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, acc,
+                     bridge);
+}
+void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs, int32_t max_pairs,
+                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par, int32_t* mate,
+                    uint8_t* used) {
+  if (max_pairs <= 0) return;
+  hipLaunchKernelGGL(k_mt_flip, dim3(item_grid(max_pairs)), dim3(kBlock), 0, st, pairs, npairs, acc, bridge, par,
+                     mate, used);
+}
+void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best) {
+  if (nroots <= 0) return;
+  hipLaunchKernelGGL(k_mt_reset_best, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, best);
+}
+void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                     const int32_t* root, const int32_t* lvl, unsigned long long* claim, int32_t* tlist,
+                     uint32_t* tcount) {
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_mt_claim, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, claim,
+                     tlist, tcount);
+}
+void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
+                      int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
+                      int32_t* root, int32_t* lvl, int32_t* par, Queue nxt) {
+  if (max_t <= 0) return;
+  hipLaunchKernelGGL(k_mt_commit, dim3(item_grid(max_t)), dim3(kBlock), 0, st, tlist, tcount, next_level, mate, off,
+                     claim, root, lvl, par, nxt);
+}
+void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
+                        unsigned long long* bad) {
+  hipLaunchKernelGGL(k_mt_validate, dim3(item_grid(n)), dim3(kBlock), 0, st, n, mate, off, col, bad);
+}
+void launch_mt_count(hipStream_t st, int32_t n, const int32_t* mate, unsigned long long* out) {
+  hipLaunchKernelGGL(k_mt_count, dim3(item_grid(n) < 1024 ? item_grid(n) : 1024), dim3(kBlock), 0, st, n, mate, out);
 }
 
 void launch_count_traversed(hipStream_t st, const int32_t* off, const int32_t* levels, int32_t S, int32_t n,

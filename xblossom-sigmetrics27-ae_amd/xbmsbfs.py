@@ -83,6 +83,7 @@ EXPORTED_SYMBOLS = (
     "xb_server_release",
     "xb_server_stats",
     "xb_server_destroy",
+    "xb_match_run",
 )
 
 XB_SERVE_COUNT_EDGES = 1
@@ -188,6 +189,15 @@ class ServerCounters(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class MatchStats(ctypes.Structure):
+    _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
+                ("blossom_edges", ctypes.c_int64), ("phases", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("ms_total", ctypes.c_double), ("reserved", ctypes.c_int32 * 8)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 _lib = None
 _gen = None
 
@@ -247,6 +257,7 @@ def load_engine():
     lib.xb_server_release.argtypes = [VP, ctypes.c_uint64]
     lib.xb_server_stats.argtypes = [VP, ctypes.POINTER(ServerCounters)]
     lib.xb_server_destroy.argtypes = [VP]
+    lib.xb_match_run.argtypes = [VP, _P32, _P32, U32, ctypes.POINTER(MatchStats)]
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name).restype = ctypes.c_int32
     _lib = lib
@@ -366,6 +377,24 @@ class XBlossomGpuEngine10:
         out = ctypes.c_int64()
         _check(lib.xb_count_traversed(self.graph.handle, _ptr(levels_dev), S, ctypes.byref(out)))
         return out.value
+
+    def FindMaximumMatch(self, initial_matching=None) -> np.ndarray:
+        """FindMaximumMatch of apps/xb/src/include/xbsm_gpu/max_match_engine.h:
+        84-113: entry v is v's mate, nnodes for an exposed vertex (the
+        reference GPU engine's sentinel).  The forest search has no blossom
+        transform: ``last_match_stats.blossom_edges == 0`` certifies that the
+        result is maximum (always so on bipartite graphs)."""
+        n = self.graph.nnodes
+        init = None
+        if initial_matching is not None:
+            init = np.array(initial_matching, dtype=np.int32).reshape(-1)
+            if init.size != n:
+                raise XbError(XB_ERR_INVALID_ARG, "initial matching must have nnodes entries")
+            init[init == n] = -1
+        mate, st = match(self.graph, init)
+        self.last_match_stats = st
+        mate[mate < 0] = n
+        return mate
 
 
 class DeviceBuffer:
@@ -534,6 +563,28 @@ def autotune(graph: HGraph, sample, **base):
     _check(load_engine().xb_autotune(graph.handle, _ptr(src), src.size, ctypes.byref(o), ctypes.byref(best),
                                      ctypes.byref(rep)))
     return {k: int(getattr(best, k)) for k in TUNABLES}, rep
+
+
+def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False):
+    """xb_match_run: (mate, MatchStats), -1 for exposed vertices.
+
+    Host mode: ``init_mate`` is None or n int32 entries and a new numpy mate
+    array is returned.  Device mode: ``init_mate`` (or None) and ``mate_out``
+    are device addresses of n int32 entries."""
+    lib = load_engine()
+    n = graph.nnodes
+    st = MatchStats()
+    if device:
+        if mate_out is None:
+            raise XbError(XB_ERR_INVALID_ARG, "device mode needs mate_out")
+        _check(lib.xb_match_run(graph.handle, _ptr(init_mate), _ptr(mate_out), XB_OUT_DEVICE, ctypes.byref(st)))
+        return mate_out, st
+    init = None if init_mate is None else np.ascontiguousarray(init_mate, dtype=np.int32)
+    if init is not None and init.size != n:
+        raise XbError(XB_ERR_INVALID_ARG, "init_mate must have n entries")
+    out = np.empty(max(n, 1), dtype=np.int32)
+    _check(lib.xb_match_run(graph.handle, _ptr(init), _ptr(out), 0, ctypes.byref(st)))
+    return out[:n], st
 
 
 def device_checksum(ptr: int, count: int) -> int:
