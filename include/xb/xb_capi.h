@@ -95,6 +95,10 @@ typedef struct xb_msbfs_opts {
 #define XB_K_PLAIN_INIT 9 /* plain engine: row initialisation                   */
 #define XB_K_PLAIN_PUSH 10 /* plain engine: top-down push level                 */
 #define XB_K_PERSISTENT 11 /* persistent batch kernel (whole batch, one launch) */
+#define XB_K_MT_SCAN 12    /* matching: augmenting-edge scan and bridge passes  */
+#define XB_K_MT_EXPAND 13  /* matching: claim and commit (forest growth)        */
+#define XB_K_MT_TREE 14    /* matching: forest reset, tree pairing, path flips  */
+#define XB_K_MT_BLOSSOM 15 /* matching: blossom labelling                       */
 #define XB_K_CLASSES 16
 
 /* Per-run statistics (filled when the pointer is non-null).  kernel_ms and
@@ -246,15 +250,21 @@ typedef struct xb_match_stats {
   int64_t matched_vertices;  /* 2 x the matching's cardinality               */
   int64_t augmentations;     /* paths flipped over all phases                */
   int64_t blossom_edges;     /* same-tree even-even edges met in the final phase */
+  int64_t edges_scanned;     /* frontier edges read by the scan, bridge and claim passes */
   int32_t phases;
   int32_t levels;            /* forest levels grown over all phases          */
   double ms_total;
+  /* Per kernel class (XB_K_MT_*): HIP-event time with XB_PROFILE, algorithmic
+   * bytes and launches always. */
+  double kernel_ms[XB_K_CLASSES];
+  double kernel_bytes[XB_K_CLASSES];
+  int64_t kernel_launches[XB_K_CLASSES];
   int32_t reserved[8];
 } xb_match_stats;
 
 /* init_mate: NULL (empty matching) or n entries (a valid matching, -1 for
- * exposed); mate_out: n entries.  XB_OUT_DEVICE in flags: both are device
- * pointers.  An invalid init_mate (out of range, asymmetric, or a pair that
+ * exposed); mate_out: n entries.  flags: XB_OUT_DEVICE (both are device
+ * pointers) | XB_PROFILE (per-class kernel times).  An invalid init_mate (out of range, asymmetric, or a pair that
  * is not an edge) is XB_ERR_INVALID_ARG. */
 int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, uint32_t flags,
                      xb_match_stats* stats);

@@ -62,6 +62,7 @@ typedef struct oracle_match_stats {
   int64_t matched_vertices;
   int64_t augmentations;
   int64_t blossom_edges;
+  int64_t edges_scanned; /* frontier edges read by the scan, bridge and claim passes */
   int32_t phases;
   int32_t levels;
 } oracle_match_stats;
@@ -101,8 +102,15 @@ static int64_t count_matched(int32_t n, const int32_t* mate) {
 
 /* ---- deterministic alternating forest --------------------------------- */
 
+static int64_t frontier_edges(const int32_t* off, const int32_t* front, int32_t fn) {
+  int64_t e = 0;
+  for (int32_t i = 0; i < fn; ++i) e += off[front[i] + 1] - off[front[i]];
+  return e;
+}
+
+/* max_phases > 0 stops after that many phases (a bounded sample for timing). */
 int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate,
-                            oracle_match_stats* st) {
+                            oracle_match_stats* st, int32_t max_phases) {
   oracle_match_stats s;
   memset(&s, 0, sizeof(s));
   int32_t* root = xcalloc((size_t)n, sizeof(int32_t));
@@ -118,7 +126,9 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   uint64_t* claim = xcalloc((size_t)n, sizeof(uint64_t));
   uint64_t* bridge = xcalloc((size_t)n, sizeof(uint64_t));
   uint8_t* used = xcalloc((size_t)n, 1);
+  int32_t* cursor = xcalloc((size_t)n, sizeof(int32_t));
   for (;;) {
+    if (max_phases > 0 && s.phases == max_phases) break;
     ++s.phases;
     int64_t blossoms = 0;
     int32_t nroots = 0;
@@ -132,6 +142,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
       acc[v] = -1;
       bridge[v] = NO_KEY;
       used[v] = 0;
+      cursor[v] = off[v];
       if (ex) roots[nroots++] = v;
     }
     if (nroots < 2) break;
@@ -139,21 +150,42 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
     int32_t fn = nroots, level = 0;
     int64_t found = 0;
     while (fn > 0) {
+      int64_t fe = frontier_edges(off, front, fn);
       for (;;) {
-        for (int32_t i = 0; i < nroots; ++i) best[roots[i]] = INT32_MAX;
-        for (int32_t i = 0; i < fn; ++i) {
-          const int32_t v = front[i];
-          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
-            const int32_t w = col[j];
-            if (lvl[w] < 0 || (lvl[w] & 1) || w == v) continue;
-            const int32_t rv = root[v], rw = root[w];
-            if (rv == rw) {
-              ++blossoms;
-              continue;
+        if (level == 0) {
+          /* Every tree is one root: its lowest neighbouring tree is its first
+           * adjacency entry that is an untaken root, found by a cursor that
+           * only moves forward (taken roots never return). */
+          for (int32_t i = 0; i < nroots; ++i) {
+            const int32_t t = roots[i], e = off[t + 1];
+            int32_t j = cursor[t];
+            const int32_t j0 = j;
+            while (j < e) {
+              const int32_t w = col[j];
+              if (w != t && lvl[w] == 0 && !used[w]) break;
+              ++j;
             }
-            if (used[rv] || used[rw]) continue;
-            if (rw < best[rv]) best[rv] = rw;
-            if (rv < best[rw]) best[rw] = rv;
+            s.edges_scanned += (j - j0) + (j < e);
+            cursor[t] = j;
+            best[t] = j < e ? col[j] : INT32_MAX;
+          }
+        } else {
+          s.edges_scanned += fe;
+          for (int32_t i = 0; i < nroots; ++i) best[roots[i]] = INT32_MAX;
+          for (int32_t i = 0; i < fn; ++i) {
+            const int32_t v = front[i];
+            for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+              const int32_t w = col[j];
+              if (lvl[w] < 0 || (lvl[w] & 1) || w == v) continue;
+              const int32_t rv = root[v], rw = root[w];
+              if (rv == rw) {
+                ++blossoms;
+                continue;
+              }
+              if (used[rv] || used[rw]) continue;
+              if (rw < best[rv]) best[rv] = rw;
+              if (rv < best[rw]) best[rw] = rv;
+            }
           }
         }
         int32_t np = 0;
@@ -161,22 +193,26 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           const int32_t t = roots[i], b = best[t];
           if (b != INT32_MAX && t < b && best[b] == t) {
             acc[t] = b;
+            if (level == 0) bridge[t] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)b;
             pairs[np++] = t;
           }
         }
         if (np == 0) break;
-        for (int32_t i = 0; i < fn; ++i) {
-          const int32_t v = front[i];
-          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
-            const int32_t w = col[j];
-            if (lvl[w] < 0 || (lvl[w] & 1)) continue;
-            const int32_t rv = root[v], rw = root[w];
-            if (rv == rw) continue;
-            const int32_t lo = rv < rw ? rv : rw, hi = rv < rw ? rw : rv;
-            if (acc[lo] != hi) continue;
-            const uint32_t a = (uint32_t)(rv == lo ? v : w), c = (uint32_t)(rv == lo ? w : v);
-            const uint64_t key = ((uint64_t)a << 32) | c;
-            if (key < bridge[lo]) bridge[lo] = key;
+        if (level != 0) {
+          s.edges_scanned += fe;
+          for (int32_t i = 0; i < fn; ++i) {
+            const int32_t v = front[i];
+            for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+              const int32_t w = col[j];
+              if (lvl[w] < 0 || (lvl[w] & 1)) continue;
+              const int32_t rv = root[v], rw = root[w];
+              if (rv == rw) continue;
+              const int32_t lo = rv < rw ? rv : rw, hi = rv < rw ? rw : rv;
+              if (acc[lo] != hi) continue;
+              const uint32_t a = (uint32_t)(rv == lo ? v : w), c = (uint32_t)(rv == lo ? w : v);
+              const uint64_t key = ((uint64_t)a << 32) | c;
+              if (key < bridge[lo]) bridge[lo] = key;
+            }
           }
         }
         for (int32_t i = 0; i < np; ++i) {
@@ -197,8 +233,19 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           used[acc[t]] = 1;
         }
         found += np;
+        /* The next round sees only the frontier and roots of trees not taken. */
+        int32_t k = 0;
+        for (int32_t i = 0; i < fn; ++i)
+          if (!used[root[front[i]]]) front[k++] = front[i];
+        fn = k;
+        k = 0;
+        for (int32_t i = 0; i < nroots; ++i)
+          if (!used[roots[i]]) roots[k++] = roots[i];
+        nroots = k;
+        fe = frontier_edges(off, front, fn);
       }
       if (found) break;
+      s.edges_scanned += fe;
       int32_t tn = 0;
       for (int32_t i = 0; i < fn; ++i) {
         const int32_t v = front[i];
@@ -252,6 +299,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   free(claim);
   free(bridge);
   free(used);
+  free(cursor);
   if (st) *st = s;
   return s.matched_vertices;
 }

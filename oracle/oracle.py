@@ -20,7 +20,8 @@ _lib = None
 
 class MatchStats(ctypes.Structure):
     _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
-                ("blossom_edges", ctypes.c_int64), ("phases", ctypes.c_int32), ("levels", ctypes.c_int32)]
+                ("blossom_edges", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
+                ("phases", ctypes.c_int32), ("levels", ctypes.c_int32)]
 
 
 def load():
@@ -40,7 +41,7 @@ def load():
         lib.oracle_edge_map.restype = I64
         lib.oracle_exclusive_scan.argtypes = [_P32, I64, _P64]
         lib.oracle_exclusive_scan.restype = I64
-        lib.oracle_match_forest.argtypes = [I32, _P32, _P32, _P32, ctypes.POINTER(MatchStats)]
+        lib.oracle_match_forest.argtypes = [I32, _P32, _P32, _P32, ctypes.POINTER(MatchStats), I32]
         lib.oracle_match_forest.restype = I64
         lib.oracle_match_edmonds.argtypes = [I32, _P32, _P32, _P32]
         lib.oracle_match_edmonds.restype = I64
@@ -103,15 +104,17 @@ def _csr(row_offsets, col_indices):
     return ro, ci, ro.size - 1
 
 
-def match_forest(row_offsets, col_indices, init_mate=None):
-    """Deterministic alternating-forest matching: (mate, MatchStats); -1 = exposed."""
+def match_forest(row_offsets, col_indices, init_mate=None, max_phases: int = 0):
+    """Deterministic alternating-forest matching: (mate, MatchStats); -1 = exposed.
+
+    max_phases > 0 stops after that many phases (a bounded timing sample)."""
     lib = load()
     ro, ci, n = _csr(row_offsets, col_indices)
     mate = np.full(max(n, 1), -1, dtype=np.int32)
     if init_mate is not None:
         mate[:n] = np.asarray(init_mate, dtype=np.int32)
     st = MatchStats()
-    lib.oracle_match_forest(n, _p(ro), _p(ci), _p(mate), ctypes.byref(st))
+    lib.oracle_match_forest(n, _p(ro), _p(ci), _p(mate), ctypes.byref(st), max_phases)
     return mate[:n], st
 
 

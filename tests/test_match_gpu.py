@@ -19,7 +19,7 @@ import xbmsbfs as xb
 
 pytestmark = pytest.mark.gpu
 
-STATS = ("matched_vertices", "augmentations", "blossom_edges", "phases", "levels")
+STATS = ("matched_vertices", "augmentations", "blossom_edges", "edges_scanned", "phases", "levels")
 
 
 def _same(ro, ci, init=None, label=""):

@@ -1804,14 +1804,18 @@ struct MatchState {
   int32_t* par = nullptr;
   int32_t* best = nullptr;
   int32_t* acc = nullptr;
-  int32_t* roots = nullptr;  // the phase's even roots (exposed vertices)
+  int32_t* roots = nullptr;  // the phase's even roots (exposed vertices) not yet taken
+  int32_t* roots2 = nullptr; // compaction target for roots
   int32_t* pairs = nullptr;  // the round's taken tree pairs, by lower root
   int32_t* tlist = nullptr;  // first claims of the level
+  int32_t* cursor = nullptr; // level-0 scan position in each root's adjacency
   unsigned long long* claim = nullptr;
   unsigned long long* bridge = nullptr;
   uint8_t* used = nullptr;
   // [0] blossom edges, [1] matched count / validation failures,
-  // [2] low word: taken pairs, [3] low word: first claims.
+  // [2] low word: taken pairs, [3] low word: first claims, [4] low word:
+  // roots kept by a compaction (0-4 cleared per phase); [6] adjacency
+  // entries read by the level-0 scans of the run.
   unsigned long long* ctl = nullptr;
 
   explicit MatchState(int32_t n) {
@@ -1823,20 +1827,24 @@ struct MatchState {
     best = dalloc<int32_t>(N);
     acc = dalloc<int32_t>(N);
     roots = dalloc<int32_t>(N);
+    roots2 = dalloc<int32_t>(N);
     pairs = dalloc<int32_t>(N);
     tlist = dalloc<int32_t>(N);
+    cursor = dalloc<int32_t>(N);
     claim = dalloc<unsigned long long>(N);
     bridge = dalloc<unsigned long long>(N);
     used = dalloc<uint8_t>(N);
-    ctl = dalloc<unsigned long long>(4);
+    ctl = dalloc<unsigned long long>(8);
   }
   ~MatchState() {
     for (void* p : {(void*)mate, (void*)root, (void*)lvl, (void*)par, (void*)best, (void*)acc, (void*)roots,
+                    (void*)roots2, (void*)cursor,
                     (void*)pairs, (void*)tlist, (void*)claim, (void*)bridge, (void*)used, (void*)ctl})
       dfree(p);
   }
   uint32_t* npairs() { return (uint32_t*)(ctl + 2); }
   uint32_t* tcount() { return (uint32_t*)(ctl + 3); }
+  uint32_t* kept() { return (uint32_t*)(ctl + 4); }
 };
 
 unsigned long long read_u64(Workspace& w, const unsigned long long* d) {
@@ -1851,21 +1859,32 @@ unsigned long long read_u64(Workspace& w, const unsigned long long* d) {
 // rounds of tree-disjoint augmenting paths until none is left, and end the
 // phase after the first level that yields any; stop when a whole phase
 // (the forest grown until no vertex is added) yields none.
-void run_match(xb_graph* g, MatchState& ms, xb_match_stats* out) {
+void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out) {
   Workspace& w = g->ws;
   ensure_queues(g, w);
   const int32_t n = g->n;
   const hipStream_t st = w.st;
   xb_match_stats s{};
+  // Algorithmic bytes: per frontier edge col + level (+ root of an even
+  // end) for the scans, col + level + the 8-byte claim for the claim pass;
+  // per frontier vertex its queue entry, offsets and root; per committed
+  // pair its list entry, mates, claims and the six label writes; per vertex
+  // the 41 bytes of forest state the reset writes.
+  constexpr double kScanEdge = 12, kClaimEdge = 16, kFrontVertex = 16, kCommitPair = 64, kResetVertex = 41,
+                   kRootItem = 12;
+  HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 8, st));
   for (;;) {
     ++s.phases;
-    HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 4, st));
+    HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 5, st));
     HIPCHK(hipMemsetAsync(w.q[0].ctr, 0, sizeof(unsigned long long), st));
+    prof.begin(XB_K_MT_TREE);
     xb::launch_mt_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.claim, ms.best, ms.acc, ms.bridge,
-                       ms.used, w.q[0]);
+                       ms.used, ms.cursor, w.q[0]);
+    prof.end();
+    prof.add(XB_K_MT_TREE, (kResetVertex + 8) * n);
     int32_t qn = 0, E = 0;
     read_ctr(w, w.q[0].ctr, &qn, &E);
-    const int32_t nroots = qn;
+    int32_t nroots = qn;
     if (nroots < 2) break;  // at most one exposed vertex: no augmenting path exists
     HIPCHK(hipMemcpyAsync(ms.roots, w.q[0].v, sizeof(int32_t) * (size_t)nroots, hipMemcpyDeviceToDevice, st));
     int cur = 0;
@@ -1873,25 +1892,70 @@ void run_match(xb_graph* g, MatchState& ms, xb_match_stats* out) {
     int64_t found = 0;
     while (qn > 0) {
       for (;;) {
-        xb::launch_mt_reset_best(st, ms.roots, nroots, ms.best);
-        xb::launch_mt_scan(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.used, ms.best, ms.ctl);
+        // Level 0: the frontier is the root list, every tree one root.
+        if (level == 0) {
+          prof.begin(XB_K_MT_SCAN);
+          xb::launch_mt_scan0(st, ms.roots, nroots, g->d_off, g->d_col, ms.lvl, ms.used, ms.cursor, ms.best,
+                              ms.ctl + 6);
+          prof.end();
+          // Cursor, offsets and best per root; col + level + taken flag per entry
+          // read (the entry count is added at the end of the run).
+          prof.add(XB_K_MT_SCAN, 16.0 * nroots);
+        } else {
+          prof.begin(XB_K_MT_TREE);
+          xb::launch_mt_reset_best(st, ms.roots, nroots, ms.best);
+          prof.end();
+          prof.begin(XB_K_MT_SCAN);
+          xb::launch_mt_scan(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.used, ms.best, ms.ctl);
+          prof.end();
+          prof.add(XB_K_MT_SCAN, kScanEdge * E + kFrontVertex * qn);
+          s.edges_scanned += E;
+        }
         HIPCHK(hipMemsetAsync(ms.npairs(), 0, sizeof(uint32_t), st));
-        xb::launch_mt_accept(st, ms.roots, nroots, ms.best, ms.acc, ms.pairs, ms.npairs());
+        prof.begin(XB_K_MT_TREE);
+        xb::launch_mt_accept(st, ms.roots, nroots, ms.best, ms.acc, ms.pairs, ms.npairs(),
+                             level == 0 ? ms.bridge : nullptr);
+        prof.end();
+        prof.add(XB_K_MT_TREE, 2 * kRootItem * nroots);
         const int32_t np = (int32_t)(read_u64(w, ms.ctl + 2) & 0xFFFFFFFFull);
         if (np == 0) break;
-        xb::launch_mt_bridge(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.acc, ms.bridge);
+        if (level != 0) {
+          prof.begin(XB_K_MT_SCAN);
+          xb::launch_mt_bridge(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.acc, ms.bridge);
+          prof.end();
+          prof.add(XB_K_MT_SCAN, kScanEdge * E + kFrontVertex * qn);
+          s.edges_scanned += E;
+        }
+        prof.begin(XB_K_MT_TREE);
         xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.mate, ms.used);
+        // The next round sees only the frontier and roots of trees not taken.
+        HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
+        HIPCHK(hipMemsetAsync(ms.kept(), 0, sizeof(uint32_t), st));
+        xb::launch_mt_compact(st, w.q[cur].v, qn, ms.root, ms.used, g->d_off, w.q[cur ^ 1], nullptr, nullptr);
+        xb::launch_mt_compact(st, ms.roots, nroots, ms.root, ms.used, g->d_off, xb::Queue{}, ms.roots2,
+                              ms.kept());
+        prof.end();
+        prof.add(XB_K_MT_TREE, 17.0 * qn + 9.0 * nroots);
         found += np;
+        cur ^= 1;
+        std::swap(ms.roots, ms.roots2);
+        read_ctr(w, w.q[cur].ctr, &qn, &E);
+        nroots = (int32_t)(read_u64(w, ms.ctl + 4) & 0xFFFFFFFFull);
       }
       if (found) break;
       HIPCHK(hipMemsetAsync(ms.tcount(), 0, sizeof(uint32_t), st));
       HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
+      prof.begin(XB_K_MT_EXPAND);
       xb::launch_mt_claim(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.claim, ms.tlist,
                           ms.tcount());
       // First claims come one per frontier edge at most, and one per vertex.
       xb::launch_mt_commit(st, ms.tlist, ms.tcount(), std::min(E, n), level + 2, ms.mate, g->d_off, ms.claim,
                            ms.root, ms.lvl, ms.par, w.q[cur ^ 1]);
+      prof.end();
+      prof.add(XB_K_MT_EXPAND, kClaimEdge * E + kFrontVertex * qn);
+      s.edges_scanned += E;
       read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
+      prof.add(XB_K_MT_EXPAND, kCommitPair * qn);
       cur ^= 1;
       level += 2;
       ++s.levels;
@@ -1905,6 +1969,9 @@ void run_match(xb_graph* g, MatchState& ms, xb_match_stats* out) {
   HIPCHK(hipMemsetAsync(ms.ctl + 1, 0, sizeof(unsigned long long), st));
   xb::launch_mt_count(st, n, ms.mate, ms.ctl + 1);
   s.matched_vertices = (int64_t)read_u64(w, ms.ctl + 1);
+  const int64_t examined = (int64_t)read_u64(w, ms.ctl + 6);
+  s.edges_scanned += examined;
+  prof.add(XB_K_MT_SCAN, 9.0 * examined);
   *out = s;
 }
 
@@ -1915,7 +1982,7 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
   return guarded([&] {
     if (!g || !mate_out) fail(XB_ERR_INVALID_ARG, "null graph or mate_out");
     if (g->flags & XB_GRAPH_DIRECTED) fail(XB_ERR_UNSUPPORTED, "matching needs a symmetric graph");
-    if (flags & ~(uint32_t)XB_OUT_DEVICE) fail(XB_ERR_INVALID_ARG, "unknown matching flags");
+    if (flags & ~(uint32_t)(XB_OUT_DEVICE | XB_PROFILE)) fail(XB_ERR_INVALID_ARG, "unknown matching flags");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const int32_t n = g->n;
     if (n == 0) return;
@@ -1945,7 +2012,10 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
       HIPCHK(hipMemsetAsync(ms.mate, 0xFF, sizeof(int32_t) * (size_t)n, st));
     }
     xb_match_stats s{};
-    run_match(g, ms, &s);
+    Profiler prof;
+    prof.on = (flags & XB_PROFILE) != 0;
+    prof.st = st;
+    run_match(g, ms, prof, &s);
     HIPCHK(hipMemcpyAsync(mate_out, ms.mate, sizeof(int32_t) * (size_t)n,
                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(e1, st));
@@ -1953,6 +2023,11 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
     float ms_total = 0;
     HIPCHK(hipEventElapsedTime(&ms_total, e0, e1));
     s.ms_total = ms_total;
+    xb_run_stats ks{};
+    prof.collect(&ks);
+    std::memcpy(s.kernel_ms, ks.kernel_ms, sizeof(s.kernel_ms));
+    std::memcpy(s.kernel_bytes, ks.kernel_bytes, sizeof(s.kernel_bytes));
+    std::memcpy(s.kernel_launches, ks.kernel_launches, sizeof(s.kernel_launches));
     if (stats) *stats = s;
   });
 }

@@ -183,18 +183,27 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, in
 // ---- forest mode (alternating-forest search of maximum matching) ----
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
-                    uint8_t* used, Queue q);
+                    uint8_t* used, int32_t* cursor, Queue q);
+// Level 0: each root's lowest neighbouring untaken root, by a forward-only
+// cursor over its sorted adjacency; adds the entries read to `examined`.
+void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
+                     const int32_t* lvl, const uint8_t* used, int32_t* cursor, int32_t* best,
+                     unsigned long long* examined);
 void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                     const int32_t* root, const int32_t* lvl, const uint8_t* used, int32_t* best,
                     unsigned long long* blossoms);
 void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* best, int32_t* acc,
-                      int32_t* pairs, uint32_t* npairs);
+                      int32_t* pairs, uint32_t* npairs, unsigned long long* bridge0);
 void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                       const int32_t* root, const int32_t* lvl, const int32_t* acc, unsigned long long* bridge);
 void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs, int32_t max_pairs,
                     const int32_t* acc, const unsigned long long* bridge, const int32_t* par, int32_t* mate,
                     uint8_t* used);
 void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best);
+// Keeps the entries of `in` whose tree is not taken: into the queue `out`
+// (list == nullptr) or into list/ctr32.
+void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,
+                       const int32_t* off, Queue out, int32_t* list, uint32_t* ctr32);
 void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                      const int32_t* root, const int32_t* lvl, unsigned long long* claim, int32_t* tlist,
                      uint32_t* tcount);

@@ -1389,7 +1389,8 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
                                                     unsigned long long* __restrict__ claim,
                                                     int32_t* __restrict__ best, int32_t* __restrict__ acc,
                                                     unsigned long long* __restrict__ bridge,
-                                                    uint8_t* __restrict__ used, Queue q) {
+                                                    uint8_t* __restrict__ used, int32_t* __restrict__ cursor,
+                                                    Queue q) {
   AppendStage& sg = g_lds.q.sg;
   stage_init(sg);
   for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < n; b0 += (int64_t)gridDim.x * kBlock) {
@@ -1398,6 +1399,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
     int32_t dv = 0;
     if (v < n) {
       ex = mate[v] < 0;
+      cursor[v] = off[v];
       root[v] = ex ? (int32_t)v : -1;
       lvl[v] = ex ? 0 : -1;
       par[v] = -1;
@@ -1459,18 +1461,112 @@ __global__ __launch_bounds__(kBlock) void k_mt_scan(const int32_t* __restrict__ 
   });
 }
 
-// Mutually lowest tree pairs are taken (recorded at the lower root).
+// Level-0 scan.  At level 0 every even vertex is a root, so a root's lowest
+// neighbouring tree is its first adjacency entry (lists ascend) that is an
+// untaken root other than itself.  A per-root cursor keeps the position of
+// that entry across rounds: taken roots never return, so entries passed once
+// are never read again.  Each lane walks a short remainder itself; a long one
+// is walked by the whole wave, 64 entries per step.  `examined` counts the
+// entries read up to and including the one found.
+constexpr int32_t kShortWalk = 16;
+
+__global__ __launch_bounds__(kBlock) void k_mt_scan0(const int32_t* __restrict__ roots, int32_t nroots,
+                                                     const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ lvl, const uint8_t* __restrict__ used,
+                                                     int32_t* __restrict__ cursor, int32_t* __restrict__ best,
+                                                     unsigned long long* __restrict__ examined) {
+  unsigned long long cnt = 0;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < nroots; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = b0 + threadIdx.x;
+    int32_t v = -1, j = 0, e = 0;
+    if (i < nroots) {
+      v = roots[i];
+      j = cursor[v];
+      e = off[v + 1];
+    }
+    const bool lng = v >= 0 && e - j > kShortWalk;
+    if (v >= 0 && !lng) {
+      const int32_t j0 = j;
+      while (j < e) {
+        const int32_t w = col[j];
+        if (w != v && lvl[w] == 0 && !used[w]) break;
+        ++j;
+      }
+      cnt += (unsigned long long)(j - j0) + (j < e ? 1ull : 0ull);
+      cursor[v] = j;
+      best[v] = j < e ? col[j] : INT32_MAX;
+    }
+    uint64_t m = __ballot(lng);
+    while (m) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int32_t vv = __shfl(v, src, kWave), j0 = __shfl(j, src, kWave), ee = __shfl(e, src, kWave);
+      int32_t found = ee;
+      for (int32_t k = j0; k < ee; k += kWave) {
+        const int32_t idx = k + lane_id();
+        bool ok = false;
+        if (idx < ee) {
+          const int32_t w = col[idx];
+          ok = w != vv && lvl[w] == 0 && !used[w];
+        }
+        const uint64_t hit = __ballot(ok);
+        if (hit) {
+          found = k + __ffsll((unsigned long long)hit) - 1;
+          break;
+        }
+      }
+      if (lane_id() == src) {
+        cnt += (unsigned long long)(found - j0) + (found < ee ? 1ull : 0ull);
+        cursor[vv] = found;
+        best[vv] = found < ee ? col[found] : INT32_MAX;
+      }
+    }
+  }
+  cnt = wave_sum64(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd(examined, cnt);
+}
+
+// Mutually lowest tree pairs are taken (recorded at the lower root).  At
+// level 0 (bridge0 non-null) both trees are single roots and the pair's
+// bridge is the edge between them.
 __global__ __launch_bounds__(kBlock) void k_mt_accept(const int32_t* __restrict__ roots, int32_t nroots,
                                                       const int32_t* __restrict__ best, int32_t* __restrict__ acc,
-                                                      int32_t* __restrict__ pairs, uint32_t* __restrict__ npairs) {
+                                                      int32_t* __restrict__ pairs, uint32_t* __restrict__ npairs,
+                                                      unsigned long long* __restrict__ bridge0) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nroots; i += (int64_t)gridDim.x * kBlock) {
     const int32_t t = roots[i];
     const int32_t b = best[t];
     if (b != INT32_MAX && t < b && best[b] == t) {
       acc[t] = b;
+      if (bridge0) bridge0[t] = ((unsigned long long)(uint32_t)t << 32) | (uint32_t)b;
       pairs[atomicAdd(npairs, 1u)] = t;
     }
   }
+}
+
+// Drops the entries whose tree was taken in this round: with a queue
+// (list == nullptr) the frontier and its degree prefix, else a plain list of
+// roots counted in ctr32.
+__global__ __launch_bounds__(kBlock) void k_mt_compact(const int32_t* __restrict__ in, int32_t cnt,
+                                                       const int32_t* __restrict__ root,
+                                                       const uint8_t* __restrict__ used,
+                                                       const int32_t* __restrict__ off, Queue out,
+                                                       int32_t* __restrict__ list, uint32_t* __restrict__ ctr32) {
+  AppendStage& sg = g_lds.q.sg;
+  stage_init(sg);
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < cnt; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = b0 + threadIdx.x;
+    bool keep = false;
+    int32_t v = 0, dv = 0;
+    if (i < cnt) {
+      v = in[i];
+      keep = !used[root[v]];
+      if (keep && !list) dv = off[v + 1] - off[v];
+    }
+    stage_put(sg, keep, v, dv);
+    stage_sync_flush(sg, false, out, list, ctr32);
+  }
+  stage_sync_flush(sg, true, out, list, ctr32);
 }
 
 // Clears the trees' lowest-neighbour records before the next round.
@@ -1907,9 +2003,16 @@ static int item_grid(int64_t n) {
 
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
-                    uint8_t* used, Queue q) {
+                    uint8_t* used, int32_t* cursor, Queue q) {
   hipLaunchKernelGGL(k_mt_init, dim3(item_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, claim, best,
-                     acc, bridge, used, q);
+                     acc, bridge, used, cursor, q);
+}
+void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
+                     const int32_t* lvl, const uint8_t* used, int32_t* cursor, int32_t* best,
+                     unsigned long long* examined) {
+  if (nroots <= 0) return;
+  hipLaunchKernelGGL(k_mt_scan0, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, off, col, lvl, used,
+                     cursor, best, examined);
 }
 void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                     const int32_t* root, const int32_t* lvl, const uint8_t* used, int32_t* best,
@@ -1919,10 +2022,10 @@ void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queu
                      blossoms);
 }
 void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* best, int32_t* acc,
-                      int32_t* pairs, uint32_t* npairs) {
+                      int32_t* pairs, uint32_t* npairs, unsigned long long* bridge0) {
   if (nroots <= 0) return;
   hipLaunchKernelGGL(k_mt_accept, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, best, acc, pairs,
-                     npairs);
+                     npairs, bridge0);
 }
 void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                       const int32_t* root, const int32_t* lvl, const int32_t* acc, unsigned long long* bridge) {
@@ -1936,6 +2039,12 @@ void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs
   if (max_pairs <= 0) return;
   hipLaunchKernelGGL(k_mt_flip, dim3(item_grid(max_pairs)), dim3(kBlock), 0, st, pairs, npairs, acc, bridge, par,
                      mate, used);
+}
+void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,
+                       const int32_t* off, Queue out, int32_t* list, uint32_t* ctr32) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(k_mt_compact, dim3(item_grid(cnt)), dim3(kBlock), 0, st, in, cnt, root, used, off, out, list,
+                     ctr32);
 }
 void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best) {
   if (nroots <= 0) return;

@@ -114,6 +114,8 @@ class Opts(ctypes.Structure):
 KERNEL_CLASSES = ("memset", "seed", "push", "commit", "pull", "pull_hub", "clear", "emit_heavy", "emit",
                   "plain_init", "plain_push", "persistent")
 K_CLASSES = 16
+# Kernel classes of xb_match_stats (XB_K_MT_*).
+MATCH_KERNEL_CLASSES = ("",) * 12 + ("mt_scan", "mt_expand", "mt_tree", "mt_blossom")
 
 
 class RunStats(ctypes.Structure):
@@ -191,11 +193,19 @@ class ServerCounters(ctypes.Structure):
 
 class MatchStats(ctypes.Structure):
     _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
-                ("blossom_edges", ctypes.c_int64), ("phases", ctypes.c_int32), ("levels", ctypes.c_int32),
-                ("ms_total", ctypes.c_double), ("reserved", ctypes.c_int32 * 8)]
+                ("blossom_edges", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
+                ("phases", ctypes.c_int32), ("levels", ctypes.c_int32), ("ms_total", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double * K_CLASSES), ("kernel_bytes", ctypes.c_double * K_CLASSES),
+                ("kernel_launches", ctypes.c_int64 * K_CLASSES), ("reserved", ctypes.c_int32 * 8)]
+
+    def kernels(self) -> dict:
+        return {name: {"ms": self.kernel_ms[i], "bytes": self.kernel_bytes[i],
+                       "launches": int(self.kernel_launches[i])}
+                for i, name in enumerate(MATCH_KERNEL_CLASSES) if name and self.kernel_launches[i]}
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if k not in ("reserved", "kernel_ms", "kernel_bytes", "kernel_launches")}
 
 
 _lib = None
@@ -565,7 +575,7 @@ def autotune(graph: HGraph, sample, **base):
     return {k: int(getattr(best, k)) for k in TUNABLES}, rep
 
 
-def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False):
+def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False, profile: bool = False):
     """xb_match_run: (mate, MatchStats), -1 for exposed vertices.
 
     Host mode: ``init_mate`` is None or n int32 entries and a new numpy mate
@@ -574,16 +584,18 @@ def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False)
     lib = load_engine()
     n = graph.nnodes
     st = MatchStats()
+    prof = XB_PROFILE if profile else 0
     if device:
         if mate_out is None:
             raise XbError(XB_ERR_INVALID_ARG, "device mode needs mate_out")
-        _check(lib.xb_match_run(graph.handle, _ptr(init_mate), _ptr(mate_out), XB_OUT_DEVICE, ctypes.byref(st)))
+        _check(lib.xb_match_run(graph.handle, _ptr(init_mate), _ptr(mate_out), XB_OUT_DEVICE | prof,
+                                ctypes.byref(st)))
         return mate_out, st
     init = None if init_mate is None else np.ascontiguousarray(init_mate, dtype=np.int32)
     if init is not None and init.size != n:
         raise XbError(XB_ERR_INVALID_ARG, "init_mate must have n entries")
     out = np.empty(max(n, 1), dtype=np.int32)
-    _check(lib.xb_match_run(graph.handle, _ptr(init), _ptr(out), 0, ctypes.byref(st)))
+    _check(lib.xb_match_run(graph.handle, _ptr(init), _ptr(out), prof, ctypes.byref(st)))
     return out[:n], st
 
 
