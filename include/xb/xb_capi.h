@@ -236,21 +236,22 @@ int32_t xb_last_error(char* buf, size_t len);
  * (apps/xb/src/include/xbsm_gpu/max_match_engine.h:93-113) with its
  * alternating-forest search (apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:
  * 83-359: InitAlternatingForest, FindAndFlipAugmentingPathInAlternatingForest,
- * ExpandAlternatingForest) and path flip (path_table.h:273-330).  Each
- * phase grows a forest from every exposed vertex and flips a maximal set of
- * tree-disjoint augmenting paths; phases repeat until one finds none.  All
- * choices are deterministic (lowest root id, then lowest vertex id), so a
- * run's matching is reproducible.  The blossom transform of the reference
- * (TransformOddNodesInBlossom, :362-566) is not part of this engine: the
- * result is a maximum matching when the final phase met no blossom edge
- * (blossom_edges == 0; always so on bipartite graphs), else a matching that
- * no forest without blossom contraction can augment.  Symmetric graphs only;
+ * ExpandAlternatingForest, TransformOddNodesInBlossom) and path flip
+ * (path_table.h:273-330).  Each phase grows a forest from every exposed
+ * vertex; an even-even edge inside a tree forms a blossom whose odd vertices
+ * become even; an even-even edge between two trees closes an augmenting path.
+ * A phase flips a maximal set of tree-disjoint augmenting paths; phases repeat
+ * until one finds none, and the result is a maximum matching on every graph.
+ * All choices are deterministic (lowest root id, then lowest vertex or edge
+ * position), so a run's matching is reproducible.  Symmetric graphs only;
  * unmatched vertices are -1. */
 typedef struct xb_match_stats {
   int64_t matched_vertices;  /* 2 x the matching's cardinality               */
   int64_t augmentations;     /* paths flipped over all phases                */
-  int64_t blossom_edges;     /* same-tree even-even edges met in the final phase */
-  int64_t edges_scanned;     /* frontier edges read by the scan, bridge and claim passes */
+  int64_t blossom_edges;     /* same-tree even-even edges met in the final phase (informational) */
+  int64_t edges_scanned;     /* frontier edges read by the scan, bridge, claim and blossom passes */
+  int64_t blossoms;          /* blossoms formed over the run                  */
+  int64_t blossom_vertices;  /* odd vertices made even by them               */
   int32_t phases;
   int32_t levels;            /* forest levels grown over all phases          */
   double ms_total;

@@ -62,7 +62,9 @@ typedef struct oracle_match_stats {
   int64_t matched_vertices;
   int64_t augmentations;
   int64_t blossom_edges;
-  int64_t edges_scanned; /* frontier edges read by the scan, bridge and claim passes */
+  int64_t edges_scanned; /* frontier edges read by the scan, bridge, claim and blossom passes */
+  int64_t blossoms;         /* blossoms formed */
+  int64_t blossom_vertices; /* odd vertices made even by them */
   int32_t phases;
   int32_t levels;
 } oracle_match_stats;
@@ -108,6 +110,110 @@ static int64_t frontier_edges(const int32_t* off, const int32_t* front, int32_t 
   return e;
 }
 
+#define DUMMY (-1) /* the non-outer vertex above a root on every path */
+
+typedef struct {
+  const int32_t *off, *col;
+  int32_t *lvl, *par, *fp;
+  uint64_t *elab, *flag;
+} Forest;
+
+/* Outer (even) vertices have an even level; transformed odd vertices get one. */
+static int is_outer(const Forest* f, int32_t u) { return f->lvl[u] >= 0 && !(f->lvl[u] & 1); }
+
+/* FIRST(v): the first non-outer vertex on P(v), DUMMY above the root. */
+static int32_t first_of(const Forest* f, int32_t v) {
+  int32_t u = f->fp[v];
+  while (u != DUMMY && is_outer(f, u)) u = f->fp[u];
+  return u;
+}
+
+static int is_flagged(const Forest* f, int32_t u, uint64_t stamp, int dflag) {
+  return u == DUMMY ? dflag : f->flag[u] == stamp;
+}
+
+/* Gabow's procedure L for the even-even edge (x, y) of one tree: walks up
+ * from FIRST(x) and FIRST(y) alternately to their join, then makes every
+ * non-outer vertex below the join outer with the edge label of its side
+ * (x side: (x, y); y side: (y, x)), FIRST = join, level next_level, appended
+ * to next[].  Returns the number of vertices made outer (0: x and y already
+ * share a blossom). */
+static int64_t blossom_step(Forest* f, int32_t x, int32_t y, uint64_t stamp, int32_t next_level, int32_t* next,
+                            int32_t* nn) {
+  int32_t r = first_of(f, x), s = first_of(f, y);
+  if (r == s) return 0;
+  int dflag = 0;
+  if (r == DUMMY) dflag = 1; else f->flag[r] = stamp;
+  if (s == DUMMY) dflag = 1; else f->flag[s] = stamp;
+  int32_t join;
+  for (;;) {
+    if (s != DUMMY) {
+      const int32_t t = r;
+      r = s;
+      s = t;
+    }
+    r = first_of(f, f->par[r]);
+    if (is_flagged(f, r, stamp, dflag)) {
+      join = r;
+      break;
+    }
+    if (r == DUMMY) dflag = 1; else f->flag[r] = stamp;
+  }
+  int64_t made = 0;
+  for (int side = 0; side < 2; ++side) {
+    const int32_t p = side ? y : x, q = side ? x : y;
+    const uint64_t label = ((uint64_t)(uint32_t)p << 32) | (uint32_t)q;
+    int32_t v = first_of(f, p);
+    while (v != join) {
+      const int32_t nv = first_of(f, f->par[v]);
+      f->elab[v] = label;
+      f->fp[v] = join;
+      f->lvl[v] = next_level;
+      next[(*nn)++] = v;
+      ++made;
+      v = nv;
+    }
+  }
+  return made;
+}
+
+/* Rematch(v, w): match v to w and flip P(v) behind it (Gabow's procedure R,
+ * H. N. Gabow, JACM 23(2), 1976).  Paths: a root's is itself; a grown
+ * vertex v with mate t has P(v) = v, t, P(par[t]); a vertex made outer by
+ * the edge label (a, b) has P(v) = reverse(P(a) from a to v), P(b).  The
+ * walk stops at a vertex whose mate has already changed. */
+static void rematch(const Forest* f, int32_t* mate, int32_t v, int32_t w, int32_t* stk, int64_t cap) {
+  int64_t top = 0;
+  for (;;) {
+    const int32_t t = mate[v];
+    mate[v] = w;
+    if (t >= 0 && mate[t] == v) {
+      if (f->elab[v] == NO_KEY) {
+        const int32_t c = f->par[t];
+        mate[t] = c;
+        v = c;
+        w = t;
+      } else {
+        const int32_t a = (int32_t)(f->elab[v] >> 32), b = (int32_t)(f->elab[v] & 0xFFFFFFFFu);
+        if (top + 2 > cap) abort();
+        stk[top++] = b;
+        stk[top++] = a;
+        v = a;
+        w = b;
+      }
+      continue;
+    }
+    if (top == 0) return;
+    w = stk[--top];
+    v = stk[--top];
+  }
+}
+
+static int cmp_u64(const void* a, const void* b) {
+  const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : x > y;
+}
+
 /* max_phases > 0 stops after that many phases (a bounded sample for timing). */
 int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate,
                             oracle_match_stats* st, int32_t max_phases) {
@@ -116,6 +222,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   int32_t* root = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* lvl = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* par = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* fp = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* best = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* acc = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* roots = xcalloc((size_t)n, sizeof(int32_t));
@@ -123,20 +230,30 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   int32_t* next = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* claimed = xcalloc((size_t)n, sizeof(int32_t));
   int32_t* pairs = xcalloc((size_t)n, sizeof(int32_t));
+  int32_t* cursor = xcalloc((size_t)n, sizeof(int32_t));
   uint64_t* claim = xcalloc((size_t)n, sizeof(uint64_t));
   uint64_t* bridge = xcalloc((size_t)n, sizeof(uint64_t));
+  uint64_t* elab = xcalloc((size_t)n, sizeof(uint64_t));
+  uint64_t* flag = xcalloc((size_t)n, sizeof(uint64_t));
   uint8_t* used = xcalloc((size_t)n, 1);
-  int32_t* cursor = xcalloc((size_t)n, sizeof(int32_t));
+  const int64_t stk_cap = 2 * (int64_t)n + 64;
+  int32_t* stk = xcalloc((size_t)stk_cap, sizeof(int32_t));
+  int64_t cand_cap = 1024;
+  uint64_t* cand = xcalloc((size_t)cand_cap, sizeof(uint64_t));
+  uint64_t stamp_base = 0;
+  Forest F = {off, col, lvl, par, fp, elab, flag};
   for (;;) {
     if (max_phases > 0 && s.phases == max_phases) break;
     ++s.phases;
-    int64_t blossoms = 0;
+    int64_t same_tree = 0;
     int32_t nroots = 0;
     for (int32_t v = 0; v < n; ++v) {
       const int ex = mate[v] < 0;
       root[v] = ex ? v : -1;
       lvl[v] = ex ? 0 : -1;
       par[v] = -1;
+      fp[v] = DUMMY;
+      elab[v] = NO_KEY;
       claim[v] = NO_KEY;
       best[v] = INT32_MAX;
       acc[v] = -1;
@@ -151,6 +268,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
     int64_t found = 0;
     while (fn > 0) {
       int64_t fe = frontier_edges(off, front, fn);
+      int64_t level_same_tree = 0;
       for (;;) {
         if (level == 0) {
           /* Every tree is one root: its lowest neighbouring tree is its first
@@ -172,14 +290,15 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         } else {
           s.edges_scanned += fe;
           for (int32_t i = 0; i < nroots; ++i) best[roots[i]] = INT32_MAX;
+          level_same_tree = 0;
           for (int32_t i = 0; i < fn; ++i) {
             const int32_t v = front[i];
             for (int32_t j = off[v]; j < off[v + 1]; ++j) {
               const int32_t w = col[j];
-              if (lvl[w] < 0 || (lvl[w] & 1) || w == v) continue;
+              if (!is_outer(&F, w) || w == v) continue;
               const int32_t rv = root[v], rw = root[w];
               if (rv == rw) {
-                ++blossoms;
+                ++level_same_tree;
                 continue;
               }
               if (used[rv] || used[rw]) continue;
@@ -187,6 +306,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
               if (rv < best[rw]) best[rw] = rv;
             }
           }
+          same_tree += level_same_tree;
         }
         int32_t np = 0;
         for (int32_t i = 0; i < nroots; ++i) {
@@ -204,7 +324,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
             const int32_t v = front[i];
             for (int32_t j = off[v]; j < off[v + 1]; ++j) {
               const int32_t w = col[j];
-              if (lvl[w] < 0 || (lvl[w] & 1)) continue;
+              if (!is_outer(&F, w)) continue;
               const int32_t rv = root[v], rw = root[w];
               if (rv == rw) continue;
               const int32_t lo = rv < rw ? rv : rw, hi = rv < rw ? rw : rv;
@@ -218,17 +338,8 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         for (int32_t i = 0; i < np; ++i) {
           const int32_t t = pairs[i];
           const int32_t a = (int32_t)(bridge[t] >> 32), c = (int32_t)(bridge[t] & 0xFFFFFFFFu);
-          for (int side = 0; side < 2; ++side) {
-            int32_t cur = side ? c : a;
-            while (par[cur] >= 0) {
-              const int32_t odd = par[cur], up = par[odd];
-              mate[odd] = up;
-              mate[up] = odd;
-              cur = up;
-            }
-          }
-          mate[a] = c;
-          mate[c] = a;
+          rematch(&F, mate, a, c, stk, stk_cap);
+          rematch(&F, mate, c, a, stk, stk_cap);
           used[t] = 1;
           used[acc[t]] = 1;
         }
@@ -245,6 +356,25 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         fe = frontier_edges(off, front, fn);
       }
       if (found) break;
+      /* Blossom candidates: the same-tree even-even frontier edges, keyed
+       * (root << 32 | edge position), taken before the claims. */
+      int64_t C = 0;
+      if (level_same_tree > 0) {
+        if (level_same_tree > cand_cap) {
+          free(cand);
+          cand_cap = level_same_tree;
+          cand = xcalloc((size_t)cand_cap, sizeof(uint64_t));
+        }
+        s.edges_scanned += fe;
+        for (int32_t i = 0; i < fn; ++i) {
+          const int32_t v = front[i];
+          for (int32_t j = off[v]; j < off[v + 1]; ++j) {
+            const int32_t w = col[j];
+            if (!is_outer(&F, w) || w == v || root[w] != root[v]) continue;
+            cand[C++] = ((uint64_t)(uint32_t)root[v] << 32) | (uint32_t)j;
+          }
+        }
+      }
       s.edges_scanned += fe;
       int32_t tn = 0;
       for (int32_t i = 0; i < fn; ++i) {
@@ -268,9 +398,28 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           lvl[w] = level + 1;
           root[x] = rv;
           par[x] = w;
+          fp[x] = w;
           lvl[x] = level + 2;
           next[nn++] = x;
         }
+      }
+      if (C > 0) {
+        /* Trees are independent; within a tree, candidates in edge order. */
+        qsort(cand, (size_t)C, sizeof(uint64_t), cmp_u64);
+        for (int64_t i = 0; i < C; ++i) {
+          const int32_t j = (int32_t)(cand[i] & 0xFFFFFFFFu);
+          int32_t lo = 0, hi = n;
+          while (lo < hi) { /* owner of entry j: the last vertex whose offset is <= j */
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= j) lo = mid; else hi = mid - 1;
+          }
+          const int64_t made = blossom_step(&F, lo, col[j], stamp_base + (uint64_t)i + 1, level + 2, next, &nn);
+          if (made > 0) {
+            ++s.blossoms;
+            s.blossom_vertices += made;
+          }
+        }
+        stamp_base += (uint64_t)C;
       }
       int32_t* t = front;
       front = next;
@@ -281,7 +430,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
     }
     s.augmentations += found;
     if (!found) {
-      s.blossom_edges = blossoms;
+      s.blossom_edges = same_tree;
       break;
     }
   }
@@ -289,6 +438,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   free(root);
   free(lvl);
   free(par);
+  free(fp);
   free(best);
   free(acc);
   free(roots);
@@ -296,10 +446,14 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   free(next);
   free(claimed);
   free(pairs);
+  free(cursor);
   free(claim);
   free(bridge);
+  free(elab);
+  free(flag);
   free(used);
-  free(cursor);
+  free(stk);
+  free(cand);
   if (st) *st = s;
   return s.matched_vertices;
 }

@@ -21,6 +21,7 @@ _lib = None
 class MatchStats(ctypes.Structure):
     _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
                 ("blossom_edges", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
+                ("blossoms", ctypes.c_int64), ("blossom_vertices", ctypes.c_int64),
                 ("phases", ctypes.c_int32), ("levels", ctypes.c_int32)]
 
 

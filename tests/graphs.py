@@ -115,6 +115,36 @@ def random_bipartite(nl, nr, avg_deg, seed):
     return from_edges(nl + nr, rng.integers(0, nl, m), nl + rng.integers(0, nr, m))
 
 
+def nested_odd_cycles(k, depth, seed):
+    """Odd cycles of odd cycles (k parts per level, `depth` levels) joined at
+    random vertices, plus pendant vertices and random chords: blossoms nested
+    inside blossoms."""
+    rng = np.random.default_rng(seed)
+    edges = []
+    count = [0]
+
+    def build(d):
+        if d == 0:
+            count[0] += 1
+            return [count[0] - 1]
+        parts = [build(d - 1) for _ in range(k)]
+        for i in range(k):
+            a, b = parts[i], parts[(i + 1) % k]
+            edges.append((a[rng.integers(len(a))], b[rng.integers(len(b))]))
+        return [v for part in parts for v in part]
+
+    build(depth)
+    n = count[0]
+    for _ in range(n // 4):
+        edges.append((int(rng.integers(n)), count[0]))
+        count[0] += 1
+    n = count[0]
+    for _ in range(n // 5):
+        edges.append((int(rng.integers(n)), int(rng.integers(n))))
+    e = np.array(edges, dtype=np.int64)
+    return from_edges(n, e[:, 0], e[:, 1])
+
+
 def matching_size(mate):
     mate = np.asarray(mate)
     return int(np.count_nonzero(mate >= 0)) // 2

@@ -1,12 +1,13 @@
 """Forest mode on the GPU (xb_match_run): bit-exact against the oracle.
 
-The engine's matching, and its phase / level / augmentation / blossom-edge
+The engine's matching, and its phase / level / augmentation / blossom
 counts, must equal the deterministic CPU restatement's entry for entry, on the
-graphs the oracle is pinned on and on larger random graphs; at full size the
-size-independent properties hold (a valid matching, maximum on bipartite
-graphs by Hopcroft-Karp, identical on repeated runs).  The reference's own
-matching test asserts validity and the maximum size
-(apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:56-74).
+graphs the oracle is pinned on and on larger random graphs, bipartite and
+general (blossoms, nested blossoms); the size must be the maximum (Edmonds or
+Hopcroft-Karp).  At full size the size-independent properties hold (a valid
+matching, maximum on bipartite graphs by Hopcroft-Karp, identical on repeated
+runs).  The reference's own matching test asserts validity and the maximum
+size (apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:56-74).
 """
 import ctypes
 
@@ -19,7 +20,8 @@ import xbmsbfs as xb
 
 pytestmark = pytest.mark.gpu
 
-STATS = ("matched_vertices", "augmentations", "blossom_edges", "edges_scanned", "phases", "levels")
+STATS = ("matched_vertices", "augmentations", "blossom_edges", "edges_scanned", "blossoms", "blossom_vertices",
+         "phases", "levels")
 
 
 def _same(ro, ci, init=None, label=""):
@@ -66,15 +68,43 @@ def test_random_bipartite_equal_oracle_and_maximum(seed):
 
 
 @pytest.mark.parametrize("seed", range(6))
-def test_random_general_equal_oracle(seed):
-    n = 2_000 + 3_000 * seed
-    ro, ci = tg.gnp(n, (1.2 + 0.6 * seed) / n, 70 + seed)
+def test_random_general_equal_oracle_and_maximum(seed):
+    n = 400 + 700 * seed
+    ro, ci = tg.gnp(n, (1.2 + 0.8 * seed) / n, 70 + seed)
     got, st = _same(ro, ci, label=f"gnp seed {seed}")
-    if n <= 5_000:
-        best = tg.matching_size(oracle.match_edmonds(ro, ci))
-        assert tg.matching_size(got) <= best
-        if st.blossom_edges == 0:
-            assert tg.matching_size(got) == best
+    assert tg.matching_size(got) == tg.matching_size(oracle.match_edmonds(ro, ci))
+
+
+def test_many_small_general_graphs_maximum():
+    """300 small graphs (sparse to dense, and nested odd cycles): GPU equals
+    the oracle and Edmonds' maximum; the blossom step runs in many of them."""
+    rng = np.random.default_rng(5)
+    with_blossoms = 0
+    for seed in range(300):
+        if seed % 3 == 2:
+            ro, ci = tg.nested_odd_cycles(int(rng.choice([3, 5, 7])), int(rng.integers(1, 4)), seed)
+        else:
+            n = int(rng.integers(3, 200))
+            ro, ci = tg.gnp(n, min(1.0, float(rng.uniform(0.5, 10.0)) / n), 500 + seed)
+        got, st = _same(ro, ci, label=f"small graph {seed}")
+        assert tg.matching_size(got) == tg.matching_size(oracle.match_edmonds(ro, ci)), f"small graph {seed}"
+        with_blossoms += st.blossoms > 0
+    assert with_blossoms > 30
+
+
+def test_blossom_step_is_needed():
+    ro, ci = tg.csr_from_adj([[1, 2], [0, 2, 3], [0, 1], [1]])
+    got, st = _same(ro, ci, np.array([-1, 2, 1, -1], dtype=np.int32), label="triangle with pendant")
+    assert list(got) == [2, 3, 0, 1] and st.blossoms == 1
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_large_general_equal_oracle(seed):
+    """200k-vertex random graphs, average degree 3-5: many blossoms per phase."""
+    n = 200_000
+    ro, ci = tg.gnp(n, (3.0 + seed) / n, 900 + seed)
+    got, st = _same(ro, ci, label=f"large gnp {seed}")
+    assert st.blossoms > 0
 
 
 def test_rmat_equal_oracle():

@@ -5,10 +5,10 @@
   cardinality matching on random graphs.
 * Hopcroft-Karp: closed forms, scipy's maximum_bipartite_matching, and the
   refusal of non-bipartite input.
-* The deterministic forest restatement (the engine's checker): valid
-  matchings everywhere; maximum on every bipartite graph; maximum whenever its
-  final phase met no same-tree even-even edge (the Hungarian-forest
-  certificate), checked against Edmonds; never larger than maximum.
+* The deterministic forest restatement with the blossom step (the engine's
+  checker): valid matchings everywhere, and maximum everywhere -- against
+  closed forms, Hopcroft-Karp on bipartite graphs, and Edmonds on random,
+  dense and nested-blossom general graphs, from empty and greedy starts.
 The reference's own matching tests assert validity and the maximum size on
 datasets that are not shipped (apps/xb/test/xblsm_gpu/xblsm_gpu_10_test.cu:
 56-74); the closed forms and library results stand in for those sizes.
@@ -66,9 +66,9 @@ def test_forest_closed_forms(name):
     tg.check_matching(ro, ci, mate)
     got = tg.matching_size(mate)
     assert st.matched_vertices == 2 * got
-    assert got <= size
-    if bip or st.blossom_edges == 0:
-        assert got == size
+    assert got == size
+    if bip:
+        assert st.blossoms == 0 and st.blossom_edges == 0
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -111,39 +111,64 @@ def test_forest_maximum_on_bipartite(seed):
     assert st.augmentations == tg.matching_size(mate)
 
 
-def test_forest_certificate_on_general_graphs():
-    """Maximum whenever the final phase met no blossom edge; both outcomes occur."""
-    certified = uncertified = short = 0
-    for seed in range(60):
-        n = 30 + 7 * (seed % 10)
-        ro, ci = tg.gnp(n, (1.5 + (seed % 4)) / n, 1000 + seed)
+def _greedy(ro, ci, order):
+    n = ro.size - 1
+    init = np.full(n, -1, dtype=np.int32)
+    for v in order:
+        if init[v] >= 0:
+            continue
+        for w in ci[ro[v]:ro[v + 1]]:
+            if init[w] < 0 and w != v:
+                init[v], init[w] = w, v
+                break
+    return init
+
+
+def test_forest_maximum_on_general_graphs():
+    """Equal to Edmonds' maximum on sparse, dense and nested-blossom graphs,
+    from the empty matching and from a greedy start in random order."""
+    rng = np.random.default_rng(0)
+    with_blossoms = 0
+    for seed in range(400):
+        if seed % 4 == 3:
+            ro, ci = tg.nested_odd_cycles(int(rng.choice([3, 5, 7])), int(rng.integers(1, 4)), seed)
+        else:
+            n = int(rng.integers(3, 160))
+            ro, ci = tg.gnp(n, min(1.0, float(rng.uniform(0.5, 8.0)) / n), 1000 + seed)
+        best = tg.matching_size(oracle.match_edmonds(ro, ci))
         mate, st = oracle.match_forest(ro, ci)
         tg.check_matching(ro, ci, mate)
-        best = tg.matching_size(oracle.match_edmonds(ro, ci))
-        got = tg.matching_size(mate)
-        assert got <= best
-        if st.blossom_edges == 0:
-            assert got == best, f"seed {seed}: certified but not maximum"
-            certified += 1
-        else:
-            uncertified += 1
-            short += got < best
-    assert certified > 0 and uncertified > 0
-    assert short > 0  # some graphs need the blossom transform
+        assert tg.matching_size(mate) == best, f"graph {seed}"
+        with_blossoms += st.blossoms > 0
+        if seed % 2 == 0:
+            init = _greedy(ro, ci, rng.permutation(ro.size - 1))
+            mate2, st2 = oracle.match_forest(ro, ci, init)
+            tg.check_matching(ro, ci, mate2)
+            assert tg.matching_size(mate2) == best, f"graph {seed}, greedy start"
+            assert st2.augmentations == best - tg.matching_size(init)
+    assert with_blossoms > 50  # the blossom step is exercised, not just bipartite-like luck
+
+
+def test_blossom_step_is_needed():
+    """Triangle {0, 1, 2} with 1-2 matched, pendant 3 on vertex 1.  The forest
+    from roots 0 and 3 makes 1 odd and 2 even (tree 0); 2's only even
+    neighbour is 0 in its own tree, so without the blossom step the phase
+    ends with one pair.  The blossom {0, 1, 2} makes 1 outer, and its edge to
+    root 3 closes the augmenting path 3-1=2-0: two pairs."""
+    ro, ci = tg.csr_from_adj([[1, 2], [0, 2, 3], [0, 1], [1]])
+    init = np.array([-1, 2, 1, -1], dtype=np.int32)
+    mate, st = oracle.match_forest(ro, ci, init)
+    tg.check_matching(ro, ci, mate)
+    assert tg.matching_size(mate) == 2
+    assert st.blossoms == 1 and st.blossom_vertices == 1 and st.augmentations == 1
+    assert list(mate) == [2, 3, 0, 1]
 
 
 def test_forest_from_an_initial_matching():
     ro, ci = tg.random_bipartite(500, 450, 2.0, 7)
     n = ro.size - 1
     # Greedy initial matching in vertex order.
-    init = np.full(n, -1, dtype=np.int32)
-    for v in range(n):
-        if init[v] >= 0:
-            continue
-        for w in ci[ro[v]:ro[v + 1]]:
-            if init[w] < 0:
-                init[v], init[w] = w, v
-                break
+    init = _greedy(ro, ci, range(n))
     tg.check_matching(ro, ci, init)
     mate, st = oracle.match_forest(ro, ci, init)
     tg.check_matching(ro, ci, mate)

@@ -1798,6 +1798,7 @@ namespace {
 
 // Device arrays of one matching run (one entry per vertex unless noted).
 struct MatchState {
+  int32_t n = 0;
   int32_t* mate = nullptr;
   int32_t* root = nullptr;
   int32_t* lvl = nullptr;
@@ -1809,16 +1810,30 @@ struct MatchState {
   int32_t* pairs = nullptr;  // the round's taken tree pairs, by lower root
   int32_t* tlist = nullptr;  // first claims of the level
   int32_t* cursor = nullptr; // level-0 scan position in each root's adjacency
+  int32_t* fp = nullptr;     // outer vertex -> a vertex above it on its path (FIRST by following outer ones)
   unsigned long long* claim = nullptr;
   unsigned long long* bridge = nullptr;
+  unsigned long long* elab = nullptr;  // blossom edge label (x << 32 | y) of a vertex made outer, else ~0
+  unsigned long long* flag = nullptr;  // join-walk stamps (unique per candidate over the run)
   uint8_t* used = nullptr;
-  // [0] blossom edges, [1] matched count / validation failures,
-  // [2] low word: taken pairs, [3] low word: first claims, [4] low word:
-  // roots kept by a compaction (0-4 cleared per phase); [6] adjacency
-  // entries read by the level-0 scans of the run.
+  int4* pool = nullptr;                // the flip's pending second calls
+  uint32_t pool_cap = 0;
+  // Blossom candidates (grown on demand).
+  unsigned long long* cand = nullptr;
+  unsigned long long* cand_sorted = nullptr;
+  int32_t* seg = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  int32_t cand_cap = 0;
+  // Counters.  Per phase: [0] same-tree even-even edges met by the scans,
+  // [2] low word: taken pairs, [3] low word: first claims, [4] low word: roots
+  // kept by a compaction, [5] low word: candidates, [8] low word: tree runs,
+  // [9] low word: flip pool entries.  Per run: [1] matched count /
+  // validation failures, [6] entries read by the level-0 scans, [7] error
+  // bits, [10] blossoms formed, [11] vertices made outer by them.
   unsigned long long* ctl = nullptr;
 
-  explicit MatchState(int32_t n) {
+  explicit MatchState(int32_t n_) : n(n_) {
     const size_t N = (size_t)n;
     mate = dalloc<int32_t>(N);
     root = dalloc<int32_t>(N);
@@ -1831,21 +1846,54 @@ struct MatchState {
     pairs = dalloc<int32_t>(N);
     tlist = dalloc<int32_t>(N);
     cursor = dalloc<int32_t>(N);
+    fp = dalloc<int32_t>(N);
     claim = dalloc<unsigned long long>(N);
     bridge = dalloc<unsigned long long>(N);
+    elab = dalloc<unsigned long long>(N);
+    flag = dalloc<unsigned long long>(N);
     used = dalloc<uint8_t>(N);
-    ctl = dalloc<unsigned long long>(8);
+    // Each blossom label expands at most once per flip: n pending calls cover a round.
+    pool_cap = (uint32_t)N + 64;
+    pool = dalloc<int4>(pool_cap);
+    ctl = dalloc<unsigned long long>(16);
   }
   ~MatchState() {
     for (void* p : {(void*)mate, (void*)root, (void*)lvl, (void*)par, (void*)best, (void*)acc, (void*)roots,
-                    (void*)roots2, (void*)cursor,
-                    (void*)pairs, (void*)tlist, (void*)claim, (void*)bridge, (void*)used, (void*)ctl})
+                    (void*)roots2, (void*)cursor, (void*)fp, (void*)pairs, (void*)tlist, (void*)claim,
+                    (void*)bridge, (void*)elab, (void*)flag, (void*)used, (void*)pool, (void*)ctl, (void*)cand,
+                    (void*)cand_sorted, (void*)seg, sort_tmp})
       dfree(p);
   }
   uint32_t* npairs() { return (uint32_t*)(ctl + 2); }
   uint32_t* tcount() { return (uint32_t*)(ctl + 3); }
   uint32_t* kept() { return (uint32_t*)(ctl + 4); }
+  uint32_t* ncand() { return (uint32_t*)(ctl + 5); }
+  unsigned* err() { return (unsigned*)(ctl + 7); }
+  uint32_t* nseg() { return (uint32_t*)(ctl + 8); }
+  uint32_t* pool_ctr() { return (uint32_t*)(ctl + 9); }
+
+  void ensure_cand(int32_t C) {
+    if (C <= cand_cap) return;
+    const int32_t cap = std::max(C, std::max(2 * cand_cap, 4096));
+    for (void* p : {(void*)cand, (void*)cand_sorted, (void*)seg, sort_tmp}) dfree(p);
+    cand = cand_sorted = nullptr;
+    seg = nullptr;
+    sort_tmp = nullptr;
+    cand = dalloc<unsigned long long>((size_t)cap);
+    cand_sorted = dalloc<unsigned long long>((size_t)cap);
+    seg = dalloc<int32_t>((size_t)cap);
+    sort_bytes = xb::sort_u64_temp_bytes(cap);
+    sort_tmp = dalloc<uint8_t>(sort_bytes);
+    cand_cap = cap;
+  }
 };
+
+// Copies the first `count` (<= 8) counters into the pinned slot.
+const unsigned long long* read_ctl(Workspace& w, const unsigned long long* d, int count) {
+  HIPCHK(hipMemcpyAsync(w.hctr, d, sizeof(unsigned long long) * (size_t)count, hipMemcpyDeviceToHost, w.st));
+  HIPCHK(hipStreamSynchronize(w.st));
+  return w.hctr;
+}
 
 unsigned long long read_u64(Workspace& w, const unsigned long long* d) {
   HIPCHK(hipMemcpyAsync(w.hctr, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, w.st));
@@ -1872,16 +1920,19 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   // the 41 bytes of forest state the reset writes.
   constexpr double kScanEdge = 12, kClaimEdge = 16, kFrontVertex = 16, kCommitPair = 64, kResetVertex = 41,
                    kRootItem = 12;
-  HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 8, st));
+  HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 16, st));
+  HIPCHK(hipMemsetAsync(ms.flag, 0, sizeof(unsigned long long) * (size_t)n, st));
+  unsigned long long stamp_base = 0;  // join-walk stamps: one per candidate over the run
+  const int32_t max_steps = 4 * n + 16;
   for (;;) {
     ++s.phases;
-    HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 5, st));
+    HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 6, st));
     HIPCHK(hipMemsetAsync(w.q[0].ctr, 0, sizeof(unsigned long long), st));
     prof.begin(XB_K_MT_TREE);
     xb::launch_mt_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.claim, ms.best, ms.acc, ms.bridge,
-                       ms.used, ms.cursor, w.q[0]);
+                       ms.used, ms.cursor, ms.fp, ms.elab, w.q[0]);
     prof.end();
-    prof.add(XB_K_MT_TREE, (kResetVertex + 8) * n);
+    prof.add(XB_K_MT_TREE, (kResetVertex + 20) * n);
     int32_t qn = 0, E = 0;
     read_ctr(w, w.q[0].ctr, &qn, &E);
     int32_t nroots = qn;
@@ -1890,7 +1941,9 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
     int cur = 0;
     int32_t level = 0;
     int64_t found = 0;
+    unsigned long long same_tree = 0;  // the phase's count of same-tree even-even edges so far
     while (qn > 0) {
+      int64_t level_same_tree = 0;
       for (;;) {
         // Level 0: the frontier is the root list, every tree one root.
         if (level == 0) {
@@ -1917,7 +1970,10 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
                              level == 0 ? ms.bridge : nullptr);
         prof.end();
         prof.add(XB_K_MT_TREE, 2 * kRootItem * nroots);
-        const int32_t np = (int32_t)(read_u64(w, ms.ctl + 2) & 0xFFFFFFFFull);
+        const unsigned long long* c = read_ctl(w, ms.ctl, 4);
+        const int32_t np = (int32_t)(c[2] & 0xFFFFFFFFull);
+        level_same_tree = (int64_t)(c[0] - same_tree);
+        same_tree = c[0];
         if (np == 0) break;
         if (level != 0) {
           prof.begin(XB_K_MT_SCAN);
@@ -1927,7 +1983,9 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
           s.edges_scanned += E;
         }
         prof.begin(XB_K_MT_TREE);
-        xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.mate, ms.used);
+        HIPCHK(hipMemsetAsync(ms.pool_ctr(), 0, sizeof(uint32_t), st));
+        xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.elab, ms.mate, ms.used,
+                           xb::FlipStack{ms.pool, ms.pool_ctr(), ms.pool_cap}, max_steps, ms.err());
         // The next round sees only the frontier and roots of trees not taken.
         HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
         HIPCHK(hipMemsetAsync(ms.kept(), 0, sizeof(uint32_t), st));
@@ -1943,6 +2001,19 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         nroots = (int32_t)(read_u64(w, ms.ctl + 4) & 0xFFFFFFFFull);
       }
       if (found) break;
+      // Blossom candidates are the same-tree even-even edges the scan met;
+      // take them before the claims grow the forest.
+      const int32_t C = (int32_t)level_same_tree;
+      if (C > 0) {
+        ms.ensure_cand(C);
+        HIPCHK(hipMemsetAsync(ms.ncand(), 0, sizeof(uint32_t), st));
+        prof.begin(XB_K_MT_BLOSSOM);
+        xb::launch_mt_collect(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.cand, ms.ncand(),
+                              (uint32_t)C, ms.err());
+        prof.end();
+        prof.add(XB_K_MT_BLOSSOM, kScanEdge * E + kFrontVertex * qn + 8.0 * C);
+        s.edges_scanned += E;
+      }
       HIPCHK(hipMemsetAsync(ms.tcount(), 0, sizeof(uint32_t), st));
       HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
       prof.begin(XB_K_MT_EXPAND);
@@ -1950,10 +2021,24 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
                           ms.tcount());
       // First claims come one per frontier edge at most, and one per vertex.
       xb::launch_mt_commit(st, ms.tlist, ms.tcount(), std::min(E, n), level + 2, ms.mate, g->d_off, ms.claim,
-                           ms.root, ms.lvl, ms.par, w.q[cur ^ 1]);
+                           ms.root, ms.lvl, ms.par, ms.fp, w.q[cur ^ 1]);
       prof.end();
       prof.add(XB_K_MT_EXPAND, kClaimEdge * E + kFrontVertex * qn);
       s.edges_scanned += E;
+      if (C > 0) {
+        // Per tree, in (root, edge position) order: odd vertices on each new
+        // blossom become outer and join the next frontier.
+        prof.begin(XB_K_MT_BLOSSOM);
+        xb::sort_u64(ms.sort_tmp, ms.sort_bytes, ms.cand, ms.cand_sorted, C, st);
+        HIPCHK(hipMemsetAsync(ms.nseg(), 0, sizeof(uint32_t), st));
+        xb::launch_mt_segments(st, ms.cand_sorted, C, ms.seg, ms.nseg());
+        xb::launch_mt_blossom(st, g->d_off, g->d_col, n, ms.par, ms.lvl, ms.fp, ms.elab, ms.flag, ms.cand_sorted,
+                              C, ms.seg, ms.nseg(), C, stamp_base, level + 2, w.q[cur ^ 1], ms.ctl + 10,
+                              ms.err());
+        prof.end();
+        prof.add(XB_K_MT_BLOSSOM, 40.0 * C);
+        stamp_base += (unsigned long long)C;
+      }
       read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
       prof.add(XB_K_MT_EXPAND, kCommitPair * qn);
       cur ^= 1;
@@ -1969,7 +2054,13 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   HIPCHK(hipMemsetAsync(ms.ctl + 1, 0, sizeof(unsigned long long), st));
   xb::launch_mt_count(st, n, ms.mate, ms.ctl + 1);
   s.matched_vertices = (int64_t)read_u64(w, ms.ctl + 1);
-  const int64_t examined = (int64_t)read_u64(w, ms.ctl + 6);
+  const unsigned long long* c = read_ctl(w, ms.ctl + 6, 6);  // [6] .. [11]
+  const int64_t examined = (int64_t)c[0];
+  const unsigned err = (unsigned)(c[1] & 0xFFFFFFFFull);
+  s.blossoms = (int64_t)c[4];
+  s.blossom_vertices = (int64_t)c[5];
+  if (err)
+    fail(XB_ERR_DEVICE, "matching: a device-side bound was exceeded (error bits " + std::to_string(err) + ")");
   s.edges_scanned += examined;
   prof.add(XB_K_MT_SCAN, 9.0 * examined);
   *out = s;

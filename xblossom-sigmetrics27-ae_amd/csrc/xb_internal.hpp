@@ -181,9 +181,16 @@ void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, in
                     int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map);
 
 // ---- forest mode (alternating-forest search of maximum matching) ----
+// Pool of pending second calls of the path flip's recursion (x, y, next
+// entry); ctr counts the entries taken this round.
+struct FlipStack {
+  int4* e;
+  uint32_t* ctr;
+  uint32_t cap;
+};
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
-                    uint8_t* used, int32_t* cursor, Queue q);
+                    uint8_t* used, int32_t* cursor, int32_t* fp, unsigned long long* elab, Queue q);
 // Level 0: each root's lowest neighbouring untaken root, by a forward-only
 // cursor over its sorted adjacency; adds the entries read to `examined`.
 void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
@@ -197,8 +204,24 @@ void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, cons
 void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                       const int32_t* root, const int32_t* lvl, const int32_t* acc, unsigned long long* bridge);
 void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs, int32_t max_pairs,
-                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par, int32_t* mate,
-                    uint8_t* used);
+                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par,
+                    const unsigned long long* elab, int32_t* mate, uint8_t* used, FlipStack stk, int32_t max_steps,
+                    unsigned* err);
+// Blossom step: candidates (same-tree even-even frontier edges), their
+// per-tree runs after sorting, and the per-tree labelling.
+void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                       const int32_t* root, const int32_t* lvl, unsigned long long* cand, uint32_t* ncand,
+                       uint32_t cap, unsigned* err);
+void launch_mt_segments(hipStream_t st, const unsigned long long* cand, int32_t C, int32_t* seg, uint32_t* nseg);
+void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
+                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* flag,
+                       const unsigned long long* cand, int32_t C, const int32_t* seg, const uint32_t* nseg,
+                       int32_t max_segs, unsigned long long stamp_base, int32_t next_level, Queue nxt,
+                       unsigned long long* counts, unsigned* err);
+// Ascending sort of 64-bit keys (device radix sort); temp_bytes(n) sizes the scratch.
+size_t sort_u64_temp_bytes(int32_t n);
+void sort_u64(void* temp, size_t bytes, const unsigned long long* in, unsigned long long* out, int32_t n,
+              hipStream_t st);
 void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best);
 // Keeps the entries of `in` whose tree is not taken: into the queue `out`
 // (list == nullptr) or into list/ctr32.
@@ -209,7 +232,7 @@ void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Que
                      uint32_t* tcount);
 void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
                       int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
-                      int32_t* root, int32_t* lvl, int32_t* par, Queue nxt);
+                      int32_t* root, int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt);
 void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
                         unsigned long long* bad);
 void launch_mt_count(hipStream_t st, int32_t n, const int32_t* mate, unsigned long long* out);

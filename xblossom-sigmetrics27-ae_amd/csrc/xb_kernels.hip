@@ -1381,6 +1381,7 @@ __global__ void k_reset_ctl(TeamCtl* ctl, int32_t n_teams, TileJob* jobs, int32_
 // blossom transform is not part of this engine.
 
 constexpr unsigned long long kNoKey = ~0ull;
+constexpr int32_t kDummy = -1;  // the non-outer vertex above a root on every path (fp of a root)
 
 // Resets the forest and queues every exposed vertex as an even root.
 __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __restrict__ mate,
@@ -1390,6 +1391,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
                                                     int32_t* __restrict__ best, int32_t* __restrict__ acc,
                                                     unsigned long long* __restrict__ bridge,
                                                     uint8_t* __restrict__ used, int32_t* __restrict__ cursor,
+                                                    int32_t* __restrict__ fp, unsigned long long* __restrict__ elab,
                                                     Queue q) {
   AppendStage& sg = g_lds.q.sg;
   stage_init(sg);
@@ -1400,6 +1402,8 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
     if (v < n) {
       ex = mate[v] < 0;
       cursor[v] = off[v];
+      fp[v] = kDummy;
+      elab[v] = kNoKey;
       root[v] = ex ? (int32_t)v : -1;
       lvl[v] = ex ? 0 : -1;
       par[v] = -1;
@@ -1416,8 +1420,8 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
   stage_sync_flush(sg, true, q);
 }
 
-// Calls f(v, w) for every edge (v, w) out of the frontier queue (edge-balanced
-// chunks of kChunk edges per block, as in the push).
+// Calls f(v, w, j) for every edge (v, w) = col[j] out of the frontier queue
+// (edge-balanced chunks of kChunk edges per block, as in the push).
 template <class F>
 __device__ __forceinline__ void for_frontier_edges(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
                                                    Queue q, int32_t qn, int32_t E, F&& f) {
@@ -1431,7 +1435,8 @@ __device__ __forceinline__ void for_frontier_edges(const int32_t* __restrict__ o
       if (e < c1) {
         const int i = chunk_owner(co, q.eo, L.s_eo, e);
         const int32_t v = q.v[i];
-        f(v, col[off[v] + (e - q.eo[i])]);
+        const int32_t j = off[v] + (e - q.eo[i]);
+        f(v, col[j], j);
       }
     }
     __syncthreads();  // the chunk's owner staging is reused by the next chunk
@@ -1447,7 +1452,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_scan(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ lvl, const uint8_t* __restrict__ used,
                                                     int32_t* __restrict__ best,
                                                     unsigned long long* __restrict__ blossoms) {
-  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w) {
+  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w, int32_t) {
     const int32_t lw = lvl[w];
     if (lw < 0 || (lw & 1) || w == v) return;
     const int32_t rv = root[v], rw = root[w];
@@ -1583,7 +1588,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_bridge(const int32_t* __restrict_
                                                       const int32_t* __restrict__ root,
                                                       const int32_t* __restrict__ lvl, const int32_t* __restrict__ acc,
                                                       unsigned long long* __restrict__ bridge) {
-  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w) {
+  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w, int32_t) {
     const int32_t lw = lvl[w];
     if (lw < 0 || (lw & 1)) return;
     const int32_t rv = root[v], rw = root[w];
@@ -1595,34 +1600,238 @@ __global__ __launch_bounds__(kBlock) void k_mt_bridge(const int32_t* __restrict_
   });
 }
 
-// Flips each taken path (one thread per path; paths are vertex-disjoint):
-// from either end of the bridge up to its root, every (odd, even claimer)
-// pair becomes matched, then the bridge itself.
+// Flips each taken path (one thread per path; paths lie in two trees of
+// their own).  Every outer vertex v has a path P(v) to its root that starts
+// with v's matched edge:
+//   root:                     P(v) = v
+//   grown (mate t claimed by par[t]):  P(v) = v, t, P(par[t])
+//   made outer by blossom edge (a, b) with v on P(a):
+//                             P(v) = reverse(P(a) from a to v), P(b)
+// Rematch(v, w) matches v to w and flips P(v) behind it; augmenting the
+// bridge (a, c) is Rematch(a, c) then Rematch(c, a).  The recursion
+// (H. N. Gabow, "An efficient implementation of Edmonds' algorithm for
+// maximum matching on graphs", JACM 23(2), 1976, procedure R) stops where
+// the walk meets a vertex whose mate was already changed; the second call
+// of a blossom label waits on a stack linked through a shared pool.
 __global__ __launch_bounds__(kBlock) void k_mt_flip(const int32_t* __restrict__ pairs, const uint32_t* __restrict__ npairs,
                                                     const int32_t* __restrict__ acc,
                                                     const unsigned long long* __restrict__ bridge,
-                                                    const int32_t* __restrict__ par, int32_t* __restrict__ mate,
-                                                    uint8_t* __restrict__ used) {
+                                                    const int32_t* __restrict__ par,
+                                                    const unsigned long long* __restrict__ elab,
+                                                    int32_t* __restrict__ mate, uint8_t* __restrict__ used,
+                                                    FlipStack stk, int32_t max_steps, unsigned* __restrict__ err) {
   const uint32_t np = *npairs;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < np; i += (int64_t)gridDim.x * kBlock) {
-    const int32_t t = pairs[i];
-    const unsigned long long key = bridge[t];
-    if (key == kNoKey) continue;  // a taken pair always has a bridge; never walk from an empty one
+    const int32_t t0 = pairs[i];
+    const unsigned long long key = bridge[t0];
+    if (key == kNoKey) {  // a taken pair always has a bridge; never walk from an empty one
+      atomicOr(err, 1u);
+      continue;
+    }
     const int32_t a = (int32_t)(key >> 32), c = (int32_t)(key & 0xFFFFFFFFull);
-    for (int side = 0; side < 2; ++side) {
-      int32_t cur = side ? c : a;
-      while (par[cur] >= 0) {
-        const int32_t odd = par[cur];
-        const int32_t up = par[odd];
-        mate[odd] = up;
-        mate[up] = odd;
-        cur = up;
+    int32_t top = -1;
+    bool ok = true;
+    // Pending call Rematch(c, a) under the running Rematch(a, c).
+    {
+      const uint32_t slot = atomicAdd(stk.ctr, 1u);
+      if (slot >= stk.cap) {
+        ok = false;
+      } else {
+        stk.e[slot] = make_int4(c, a, top, 0);
+        top = (int32_t)slot;
       }
     }
-    mate[a] = c;
-    mate[c] = a;
-    used[t] = 1;
-    used[acc[t]] = 1;
+    int32_t v = a, w = c;
+    for (int32_t step = 0; ok; ++step) {
+      if (step > max_steps) {
+        ok = false;
+        break;
+      }
+      const int32_t t = mate[v];
+      mate[v] = w;
+      bool next = false;
+      if (t >= 0 && mate[t] == v) {
+        const unsigned long long el = elab[v];
+        if (el == kNoKey) {
+          const int32_t cl = par[t];  // the outer vertex that claimed t
+          if (cl < 0) {
+            ok = false;
+            break;
+          }
+          mate[t] = cl;
+          v = cl;
+          w = t;
+          next = true;
+        } else {
+          const int32_t la = (int32_t)(el >> 32), lb = (int32_t)(el & 0xFFFFFFFFull);
+          const uint32_t slot = atomicAdd(stk.ctr, 1u);
+          if (slot >= stk.cap) {
+            ok = false;
+            break;
+          }
+          stk.e[slot] = make_int4(lb, la, top, 0);
+          top = (int32_t)slot;
+          v = la;
+          w = lb;
+          next = true;
+        }
+      }
+      if (!next) {
+        if (top < 0) break;
+        const int4 e = stk.e[top];
+        v = e.x;
+        w = e.y;
+        top = e.z;
+      }
+    }
+    if (!ok) atomicOr(err, 2u);
+    used[t0] = 1;
+    used[acc[t0]] = 1;
+  }
+}
+
+// Blossom candidates: same-tree even-even edges out of the frontier, keyed
+// (root << 32 | edge position) so a sort groups them by tree in adjacency
+// order.
+__global__ __launch_bounds__(kBlock) void k_mt_collect(const int32_t* __restrict__ off, const int32_t* __restrict__ col,
+                                                       Queue q, int32_t qn, int32_t E,
+                                                       const int32_t* __restrict__ root,
+                                                       const int32_t* __restrict__ lvl,
+                                                       unsigned long long* __restrict__ cand,
+                                                       uint32_t* __restrict__ ncand, uint32_t cap,
+                                                       unsigned* __restrict__ err) {
+  for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w, int32_t j) {
+    const int32_t lw = lvl[w];
+    if (lw < 0 || (lw & 1) || w == v) return;
+    const int32_t rv = root[v];
+    if (rv != root[w]) return;
+    const uint32_t k = atomicAdd(ncand, 1u);
+    if (k < cap)
+      cand[k] = ((unsigned long long)(uint32_t)rv << 32) | (uint32_t)j;
+    else
+      atomicOr(err, 4u);
+  });
+}
+
+// Start of each tree's run in the sorted candidates.
+__global__ __launch_bounds__(kBlock) void k_mt_segments(const unsigned long long* __restrict__ cand, int32_t C,
+                                                        int32_t* __restrict__ seg, uint32_t* __restrict__ nseg) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < C; i += (int64_t)gridDim.x * kBlock)
+    if (i == 0 || (cand[i] >> 32) != (cand[i - 1] >> 32)) seg[atomicAdd(nseg, 1u)] = (int32_t)i;
+}
+
+// Forest state the blossom step reads and writes (one tree per thread).
+struct BlossomState {
+  const int32_t* off;
+  const int32_t* col;
+  int32_t n;
+  const int32_t* par;
+  int32_t* lvl;
+  int32_t* fp;  // outer vertex -> a vertex above it on its path; following fp over outer vertices ends at FIRST
+  unsigned long long* elab;
+  unsigned long long* flag;
+};
+
+__device__ __forceinline__ bool mt_outer(const int32_t* lvl, int32_t u) {
+  const int32_t l = lvl[u];
+  return l >= 0 && !(l & 1);
+}
+
+// FIRST(v): the first non-outer vertex on P(v) (kDummy above the root).
+__device__ __forceinline__ int32_t mt_first(const BlossomState& b, int32_t v, int32_t& budget) {
+  int32_t u = b.fp[v];
+  while (u != kDummy && mt_outer(b.lvl, u) && --budget > 0) u = b.fp[u];
+  return u;
+}
+
+// Blossom step: per tree, in sorted order, each candidate edge (x, y) whose
+// ends lie in different blossoms makes every non-outer vertex on P(x) and
+// P(y) below their join outer (Gabow's procedure L): it gets the edge label
+// (x, y) or (y, x) by its side, level next_level, FIRST = the join, and joins
+// the next frontier.  These are the reference's odd nodes in a blossom made
+// even (TransformOddNodesInBlossom, xblossom_gpu_engine_10.cu:362-566).
+__global__ __launch_bounds__(kBlock) void k_mt_blossom(BlossomState b, const unsigned long long* __restrict__ cand,
+                                                       int32_t C, const int32_t* __restrict__ seg,
+                                                       const uint32_t* __restrict__ nseg,
+                                                       unsigned long long stamp_base, int32_t next_level, Queue nxt,
+                                                       unsigned long long* __restrict__ counts,
+                                                       unsigned* __restrict__ err) {
+  const uint32_t ns = *nseg;
+  for (int64_t si = (int64_t)blockIdx.x * kBlock + threadIdx.x; si < ns; si += (int64_t)gridDim.x * kBlock) {
+    int32_t i = seg[si];
+    const unsigned long long tree = cand[i] >> 32;
+    unsigned long long formed = 0, made = 0;
+    bool ok = true;
+    for (; ok && i < C && (cand[i] >> 32) == tree; ++i) {
+      const int32_t j = (int32_t)(cand[i] & 0xFFFFFFFFull);
+      // Owner of adjacency entry j: the last vertex whose offset is <= j.
+      int32_t lo = 0, hi = b.n;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (b.off[mid] <= j)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int32_t x = lo, y = b.col[j];
+      const unsigned long long stamp = stamp_base + (unsigned long long)i + 1ull;
+      int32_t budget = 4 * b.n + 16;
+      int32_t r = mt_first(b, x, budget), s = mt_first(b, y, budget);
+      if (r == s) continue;
+      bool dflag = false;
+      if (r == kDummy) dflag = true; else b.flag[r] = stamp;
+      if (s == kDummy) dflag = true; else b.flag[s] = stamp;
+      int32_t join;
+      for (;;) {
+        if (s != kDummy) {
+          const int32_t tmp = r;
+          r = s;
+          s = tmp;
+        }
+        const int32_t pr = b.par[r];  // an inner vertex's claimer: the next outer vertex up its paths
+        if (pr < 0) {
+          ok = false;
+          break;
+        }
+        r = mt_first(b, pr, budget);
+        if (budget <= 0) {
+          ok = false;
+          break;
+        }
+        if (r == kDummy ? dflag : b.flag[r] == stamp) {
+          join = r;
+          break;
+        }
+        if (r == kDummy) dflag = true; else b.flag[r] = stamp;
+      }
+      if (!ok) break;
+      ++formed;
+      for (int side = 0; side < 2; ++side) {
+        const int32_t p = side ? y : x, q2 = side ? x : y;
+        const unsigned long long label = ((unsigned long long)(uint32_t)p << 32) | (uint32_t)q2;
+        int32_t v = mt_first(b, p, budget);
+        while (v != join) {
+          if (v == kDummy || budget <= 0 || b.par[v] < 0) {  // the walk above reached the join from this side
+            ok = false;
+            break;
+          }
+          const int32_t nextv = mt_first(b, b.par[v], budget);
+          b.elab[v] = label;
+          b.fp[v] = join;
+          b.lvl[v] = next_level;
+          const int32_t dv = b.off[v + 1] - b.off[v];
+          const unsigned long long old = atomicAdd(nxt.ctr, ((unsigned long long)(uint32_t)dv << 32) | 1ull);
+          nxt.v[(uint32_t)(old & 0xFFFFFFFFull)] = v;
+          nxt.eo[(uint32_t)(old & 0xFFFFFFFFull)] = (int32_t)(uint32_t)(old >> 32);
+          ++made;
+          v = nextv;
+        }
+        if (!ok) break;
+      }
+    }
+    if (!ok) atomicOr(err, 8u);
+    if (formed) atomicAdd(&counts[0], formed);
+    if (made) atomicAdd(&counts[1], made);
   }
 }
 
@@ -1669,7 +1878,8 @@ __global__ __launch_bounds__(kBlock) void k_mt_commit(const int32_t* __restrict_
                                                       const int32_t* __restrict__ mate, const int32_t* __restrict__ off,
                                                       const unsigned long long* __restrict__ claim,
                                                       int32_t* __restrict__ root, int32_t* __restrict__ lvl,
-                                                      int32_t* __restrict__ par, Queue nxt) {
+                                                      int32_t* __restrict__ par, int32_t* __restrict__ fp,
+                                                      Queue nxt) {
   AppendStage& sg = g_lds.q.sg;
   stage_init(sg);
   const uint32_t cnt = *tcount;
@@ -1689,6 +1899,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_commit(const int32_t* __restrict_
         lvl[w] = next_level - 1;
         root[x] = rv;
         par[x] = w;
+        fp[x] = w;
         lvl[x] = next_level;
         dx = off[x + 1] - off[x];
       }
@@ -2003,9 +2214,9 @@ static int item_grid(int64_t n) {
 
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
-                    uint8_t* used, int32_t* cursor, Queue q) {
+                    uint8_t* used, int32_t* cursor, int32_t* fp, unsigned long long* elab, Queue q) {
   hipLaunchKernelGGL(k_mt_init, dim3(item_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, claim, best,
-                     acc, bridge, used, cursor, q);
+                     acc, bridge, used, cursor, fp, elab, q);
 }
 void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
                      const int32_t* lvl, const uint8_t* used, int32_t* cursor, int32_t* best,
@@ -2034,11 +2245,33 @@ void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Qu
                      bridge);
 }
 void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs, int32_t max_pairs,
-                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par, int32_t* mate,
-                    uint8_t* used) {
+                    const int32_t* acc, const unsigned long long* bridge, const int32_t* par,
+                    const unsigned long long* elab, int32_t* mate, uint8_t* used, FlipStack stk, int32_t max_steps,
+                    unsigned* err) {
   if (max_pairs <= 0) return;
   hipLaunchKernelGGL(k_mt_flip, dim3(item_grid(max_pairs)), dim3(kBlock), 0, st, pairs, npairs, acc, bridge, par,
-                     mate, used);
+                     elab, mate, used, stk, max_steps, err);
+}
+void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
+                       const int32_t* root, const int32_t* lvl, unsigned long long* cand, uint32_t* ncand,
+                       uint32_t cap, unsigned* err) {
+  if (E <= 0) return;
+  hipLaunchKernelGGL(k_mt_collect, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, cand,
+                     ncand, cap, err);
+}
+void launch_mt_segments(hipStream_t st, const unsigned long long* cand, int32_t C, int32_t* seg, uint32_t* nseg) {
+  if (C <= 0) return;
+  hipLaunchKernelGGL(k_mt_segments, dim3(item_grid(C)), dim3(kBlock), 0, st, cand, C, seg, nseg);
+}
+void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
+                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* flag,
+                       const unsigned long long* cand, int32_t C, const int32_t* seg, const uint32_t* nseg,
+                       int32_t max_segs, unsigned long long stamp_base, int32_t next_level, Queue nxt,
+                       unsigned long long* counts, unsigned* err) {
+  if (C <= 0 || max_segs <= 0) return;
+  const BlossomState b{off, col, n, par, lvl, fp, elab, flag};
+  hipLaunchKernelGGL(k_mt_blossom, dim3(item_grid(max_segs)), dim3(kBlock), 0, st, b, cand, C, seg, nseg, stamp_base,
+                     next_level, nxt, counts, err);
 }
 void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,
                        const int32_t* off, Queue out, int32_t* list, uint32_t* ctr32) {
@@ -2059,10 +2292,10 @@ void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Que
 }
 void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
                       int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
-                      int32_t* root, int32_t* lvl, int32_t* par, Queue nxt) {
+                      int32_t* root, int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt) {
   if (max_t <= 0) return;
   hipLaunchKernelGGL(k_mt_commit, dim3(item_grid(max_t)), dim3(kBlock), 0, st, tlist, tcount, next_level, mate, off,
-                     claim, root, lvl, par, nxt);
+                     claim, root, lvl, par, fp, nxt);
 }
 void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
                         unsigned long long* bad) {

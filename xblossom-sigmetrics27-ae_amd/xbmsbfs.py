@@ -194,6 +194,7 @@ class ServerCounters(ctypes.Structure):
 class MatchStats(ctypes.Structure):
     _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
                 ("blossom_edges", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
+                ("blossoms", ctypes.c_int64), ("blossom_vertices", ctypes.c_int64),
                 ("phases", ctypes.c_int32), ("levels", ctypes.c_int32), ("ms_total", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double * K_CLASSES), ("kernel_bytes", ctypes.c_double * K_CLASSES),
                 ("kernel_launches", ctypes.c_int64 * K_CLASSES), ("reserved", ctypes.c_int32 * 8)]
