@@ -318,7 +318,31 @@ def bench_match(args) -> int:
     import oracle
     import xbmsbfs as xb
 
-    g, _src, desc = make_workload("livejournal")
+    if args.match_graph == "gnp":
+        # Erdos-Renyi G(n, p), n = 1M, mean degree 4 (seed 1): its late
+        # phases grow trees over most of the graph and form 235k blossoms.
+        n_ = 1_000_000
+        rng = np.random.default_rng(1)
+        draws = int(rng.binomial(n_ * (n_ - 1) // 2, 4.0 / n_))
+        u = rng.integers(0, n_, draws)
+        v = rng.integers(0, n_, draws)
+        keep = u != v
+        a = np.concatenate([u[keep], v[keep]])
+        b = np.concatenate([v[keep], u[keep]])
+        key = np.unique(a.astype(np.int64) * n_ + b)
+        ro = np.zeros(n_ + 1, dtype=np.int64)
+        np.add.at(ro, key // n_ + 1, 1)
+
+        class _G:
+            pass
+
+        g = _G()
+        g.row_offsets = np.cumsum(ro).astype(np.int32)
+        g.col_indices = (key % n_).astype(np.int32)
+        g.n, g.m = n_, int(key.size // 2)
+        desc = "G(n, p) n=1M mean degree 4 (seed 1)"
+    else:
+        g, _src, desc = make_workload("livejournal")
     hg = xb.HGraph(g.row_offsets, g.col_indices)
     out = xb.DeviceBuffer((g.n,))
     for _ in range(args.warmup):
@@ -370,14 +394,14 @@ def bench_match(args) -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "match: " + desc.split(",")[0] + ", maximum matching from the empty matching",
-                   "n": g.n, "m": g.m},
+        "config": {"workload": "match: " + desc.split(",")[0].replace("(gamma 2.1", "(gamma 2.1)")
+                               + ", maximum matching from the empty matching", "n": g.n, "m": g.m},
         "matched_pairs": int(want_pairs),
         "phases": st.phases,
         "levels": st.levels,
         "augmentations": int(st.augmentations),
-        "final_phase_blossom_edges": int(st.blossom_edges),
-        "certified_maximum": bool(st.blossom_edges == 0),
+        "blossoms": int(st.blossoms),
+        "blossom_vertices": int(st.blossom_vertices),
         "bit_exact_vs_cpu": bit_exact,
         "edges_scanned": int(st.edges_scanned),
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
@@ -396,6 +420,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven", "serving",
                                                                  "match"])
+    ap.add_argument("--match-graph", default="livejournal", choices=["livejournal", "gnp"],
+                    help="match: the LiveJournal-shaped graph, or a blossom-heavy G(n, p)")
     ap.add_argument("--requests", type=int, default=400, help="serving: timed queries")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (queries/s)")
     ap.add_argument("--slots", type=int, default=2,

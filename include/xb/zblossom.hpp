@@ -164,8 +164,7 @@ class XBlossomGpuEngine10 {
   // FindMaximumMatch of apps/xb/src/include/xbsm_gpu/max_match_engine.h:84-113
   // (used by apps/xb/runner/xblsm_gpu/run_xb_pp_by_dataset.cu:42-44): entry v
   // is v's mate, nnodes for an exposed vertex, as in the reference GPU engine.
-  // The forest search has no blossom transform; LastMatchStats().blossom_edges
-  // == 0 certifies the result maximum (always so on bipartite graphs).
+  // The result is a maximum matching (blossoms included).
   Matching FindMaximumMatch() { return FindMaximumMatch(Matching(n_, (int32_t)n_)); }
 
   Matching FindMaximumMatch(const Matching& initial_matching) {

@@ -1814,21 +1814,23 @@ struct MatchState {
   unsigned long long* claim = nullptr;
   unsigned long long* bridge = nullptr;
   unsigned long long* elab = nullptr;  // blossom edge label (x << 32 | y) of a vertex made outer, else ~0
-  unsigned long long* flag = nullptr;  // join-walk stamps (unique per candidate over the run)
+  unsigned long long* lab = nullptr;   // blossom step: lowest (candidate << 1 | side) marking a vertex, else ~0
+  int32_t* marked = nullptr;           // blossom step: vertices marked this step
   uint8_t* used = nullptr;
   int4* pool = nullptr;                // the flip's pending second calls
   uint32_t pool_cap = 0;
   // Blossom candidates (grown on demand).
   unsigned long long* cand = nullptr;
   unsigned long long* cand_sorted = nullptr;
-  int32_t* seg = nullptr;
+  int32_t* join_of = nullptr;  // each sorted candidate's join
+  uint32_t* formed = nullptr;  // candidates that labelled a vertex
   void* sort_tmp = nullptr;
   size_t sort_bytes = 0;
   int32_t cand_cap = 0;
   // Counters.  Per phase: [0] same-tree even-even edges met by the scans,
   // [2] low word: taken pairs, [3] low word: first claims, [4] low word: roots
-  // kept by a compaction, [5] low word: candidates, [8] low word: tree runs,
-  // [9] low word: flip pool entries.  Per run: [1] matched count /
+  // kept by a compaction, [5] low word: candidates, [8] low word: vertices
+  // marked by a blossom step, [9] low word: flip pool entries.  Per run: [1] matched count /
   // validation failures, [6] entries read by the level-0 scans, [7] error
   // bits, [10] blossoms formed, [11] vertices made outer by them.
   unsigned long long* ctl = nullptr;
@@ -1850,7 +1852,8 @@ struct MatchState {
     claim = dalloc<unsigned long long>(N);
     bridge = dalloc<unsigned long long>(N);
     elab = dalloc<unsigned long long>(N);
-    flag = dalloc<unsigned long long>(N);
+    lab = dalloc<unsigned long long>(N);
+    marked = dalloc<int32_t>(N);
     used = dalloc<uint8_t>(N);
     // Each blossom label expands at most once per flip: n pending calls cover a round.
     pool_cap = (uint32_t)N + 64;
@@ -1860,8 +1863,8 @@ struct MatchState {
   ~MatchState() {
     for (void* p : {(void*)mate, (void*)root, (void*)lvl, (void*)par, (void*)best, (void*)acc, (void*)roots,
                     (void*)roots2, (void*)cursor, (void*)fp, (void*)pairs, (void*)tlist, (void*)claim,
-                    (void*)bridge, (void*)elab, (void*)flag, (void*)used, (void*)pool, (void*)ctl, (void*)cand,
-                    (void*)cand_sorted, (void*)seg, sort_tmp})
+                    (void*)bridge, (void*)elab, (void*)lab, (void*)marked, (void*)used, (void*)pool, (void*)ctl,
+                    (void*)cand, (void*)cand_sorted, (void*)join_of, (void*)formed, sort_tmp})
       dfree(p);
   }
   uint32_t* npairs() { return (uint32_t*)(ctl + 2); }
@@ -1869,19 +1872,21 @@ struct MatchState {
   uint32_t* kept() { return (uint32_t*)(ctl + 4); }
   uint32_t* ncand() { return (uint32_t*)(ctl + 5); }
   unsigned* err() { return (unsigned*)(ctl + 7); }
-  uint32_t* nseg() { return (uint32_t*)(ctl + 8); }
+  uint32_t* nmarked() { return (uint32_t*)(ctl + 8); }
   uint32_t* pool_ctr() { return (uint32_t*)(ctl + 9); }
 
   void ensure_cand(int32_t C) {
     if (C <= cand_cap) return;
     const int32_t cap = std::max(C, std::max(2 * cand_cap, 4096));
-    for (void* p : {(void*)cand, (void*)cand_sorted, (void*)seg, sort_tmp}) dfree(p);
+    for (void* p : {(void*)cand, (void*)cand_sorted, (void*)join_of, (void*)formed, sort_tmp}) dfree(p);
     cand = cand_sorted = nullptr;
-    seg = nullptr;
+    join_of = nullptr;
+    formed = nullptr;
     sort_tmp = nullptr;
     cand = dalloc<unsigned long long>((size_t)cap);
     cand_sorted = dalloc<unsigned long long>((size_t)cap);
-    seg = dalloc<int32_t>((size_t)cap);
+    join_of = dalloc<int32_t>((size_t)cap);
+    formed = dalloc<uint32_t>((size_t)cap);
     sort_bytes = xb::sort_u64_temp_bytes(cap);
     sort_tmp = dalloc<uint8_t>(sort_bytes);
     cand_cap = cap;
@@ -1921,8 +1926,7 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   constexpr double kScanEdge = 12, kClaimEdge = 16, kFrontVertex = 16, kCommitPair = 64, kResetVertex = 41,
                    kRootItem = 12;
   HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 16, st));
-  HIPCHK(hipMemsetAsync(ms.flag, 0, sizeof(unsigned long long) * (size_t)n, st));
-  unsigned long long stamp_base = 0;  // join-walk stamps: one per candidate over the run
+  HIPCHK(hipMemsetAsync(ms.lab, 0xFF, sizeof(unsigned long long) * (size_t)n, st));
   const int32_t max_steps = 4 * n + 16;
   for (;;) {
     ++s.phases;
@@ -2030,14 +2034,13 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         // blossom become outer and join the next frontier.
         prof.begin(XB_K_MT_BLOSSOM);
         xb::sort_u64(ms.sort_tmp, ms.sort_bytes, ms.cand, ms.cand_sorted, C, st);
-        HIPCHK(hipMemsetAsync(ms.nseg(), 0, sizeof(uint32_t), st));
-        xb::launch_mt_segments(st, ms.cand_sorted, C, ms.seg, ms.nseg());
-        xb::launch_mt_blossom(st, g->d_off, g->d_col, n, ms.par, ms.lvl, ms.fp, ms.elab, ms.flag, ms.cand_sorted,
-                              C, ms.seg, ms.nseg(), C, stamp_base, level + 2, w.q[cur ^ 1], ms.ctl + 10,
-                              ms.err());
+        HIPCHK(hipMemsetAsync(ms.nmarked(), 0, sizeof(uint32_t), st));
+        HIPCHK(hipMemsetAsync(ms.formed, 0, sizeof(uint32_t) * (size_t)C, st));
+        xb::launch_mt_blossom(st, g->d_off, g->d_col, n, ms.par, ms.lvl, ms.fp, ms.elab, ms.lab, ms.cand_sorted, C,
+                              ms.join_of, ms.marked, ms.nmarked(), ms.formed, level + 2, w.q[cur ^ 1],
+                              ms.ctl + 10, ms.err());
         prof.end();
-        prof.add(XB_K_MT_BLOSSOM, 40.0 * C);
-        stamp_base += (unsigned long long)C;
+        prof.add(XB_K_MT_BLOSSOM, 48.0 * C);
       }
       read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
       prof.add(XB_K_MT_EXPAND, kCommitPair * qn);

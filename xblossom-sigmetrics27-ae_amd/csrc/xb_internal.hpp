@@ -207,16 +207,17 @@ void launch_mt_flip(hipStream_t st, const int32_t* pairs, const uint32_t* npairs
                     const int32_t* acc, const unsigned long long* bridge, const int32_t* par,
                     const unsigned long long* elab, int32_t* mate, uint8_t* used, FlipStack stk, int32_t max_steps,
                     unsigned* err);
-// Blossom step: candidates (same-tree even-even frontier edges), their
-// per-tree runs after sorting, and the per-tree labelling.
+// Blossom step: candidates (same-tree even-even frontier edges), sorted by
+// (root, edge position), then marked and applied.
 void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
                        const int32_t* root, const int32_t* lvl, unsigned long long* cand, uint32_t* ncand,
                        uint32_t cap, unsigned* err);
-void launch_mt_segments(hipStream_t st, const unsigned long long* cand, int32_t C, int32_t* seg, uint32_t* nseg);
+// Marks each candidate's blossom path (lowest sorted candidate per inner
+// vertex), then makes the marked vertices outer and appends them to nxt.
 void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
-                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* flag,
-                       const unsigned long long* cand, int32_t C, const int32_t* seg, const uint32_t* nseg,
-                       int32_t max_segs, unsigned long long stamp_base, int32_t next_level, Queue nxt,
+                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* lab,
+                       const unsigned long long* cand, int32_t C, int32_t* join_of, int32_t* marked,
+                       uint32_t* nmarked, uint32_t* formed, int32_t next_level, Queue nxt,
                        unsigned long long* counts, unsigned* err);
 // Ascending sort of 64-bit keys (device radix sort); temp_bytes(n) sizes the scratch.
 size_t sort_u64_temp_bytes(int32_t n);

@@ -1713,14 +1713,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_collect(const int32_t* __restrict
   });
 }
 
-// Start of each tree's run in the sorted candidates.
-__global__ __launch_bounds__(kBlock) void k_mt_segments(const unsigned long long* __restrict__ cand, int32_t C,
-                                                        int32_t* __restrict__ seg, uint32_t* __restrict__ nseg) {
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < C; i += (int64_t)gridDim.x * kBlock)
-    if (i == 0 || (cand[i] >> 32) != (cand[i - 1] >> 32)) seg[atomicAdd(nseg, 1u)] = (int32_t)i;
-}
-
-// Forest state the blossom step reads and writes (one tree per thread).
+// Forest state the blossom step reads and writes.
 struct BlossomState {
   const int32_t* off;
   const int32_t* col;
@@ -1729,7 +1722,7 @@ struct BlossomState {
   int32_t* lvl;
   int32_t* fp;  // outer vertex -> a vertex above it on its path; following fp over outer vertices ends at FIRST
   unsigned long long* elab;
-  unsigned long long* flag;
+  unsigned long long* lab;  // inner vertex -> lowest (candidate << 1 | side) whose path holds it, ~0 if none
 };
 
 __device__ __forceinline__ bool mt_outer(const int32_t* lvl, int32_t u) {
@@ -1744,93 +1737,119 @@ __device__ __forceinline__ int32_t mt_first(const BlossomState& b, int32_t v, in
   return u;
 }
 
-// Blossom step: per tree, in sorted order, each candidate edge (x, y) whose
-// ends lie in different blossoms makes every non-outer vertex on P(x) and
-// P(y) below their join outer (Gabow's procedure L): it gets the edge label
-// (x, y) or (y, x) by its side, level next_level, FIRST = the join, and joins
-// the next frontier.  These are the reference's odd nodes in a blossom made
-// even (TransformOddNodesInBlossom, xblossom_gpu_engine_10.cu:362-566).
-__global__ __launch_bounds__(kBlock) void k_mt_blossom(BlossomState b, const unsigned long long* __restrict__ cand,
-                                                       int32_t C, const int32_t* __restrict__ seg,
-                                                       const uint32_t* __restrict__ nseg,
-                                                       unsigned long long stamp_base, int32_t next_level, Queue nxt,
-                                                       unsigned long long* __restrict__ counts,
-                                                       unsigned* __restrict__ err) {
-  const uint32_t ns = *nseg;
-  for (int64_t si = (int64_t)blockIdx.x * kBlock + threadIdx.x; si < ns; si += (int64_t)gridDim.x * kBlock) {
-    int32_t i = seg[si];
-    const unsigned long long tree = cand[i] >> 32;
-    unsigned long long formed = 0, made = 0;
-    bool ok = true;
-    for (; ok && i < C && (cand[i] >> 32) == tree; ++i) {
-      const int32_t j = (int32_t)(cand[i] & 0xFFFFFFFFull);
-      // Owner of adjacency entry j: the last vertex whose offset is <= j.
-      int32_t lo = 0, hi = b.n;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (b.off[mid] <= j)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      const int32_t x = lo, y = b.col[j];
-      const unsigned long long stamp = stamp_base + (unsigned long long)i + 1ull;
-      int32_t budget = 4 * b.n + 16;
-      int32_t r = mt_first(b, x, budget), s = mt_first(b, y, budget);
-      if (r == s) continue;
-      bool dflag = false;
-      if (r == kDummy) dflag = true; else b.flag[r] = stamp;
-      if (s == kDummy) dflag = true; else b.flag[s] = stamp;
-      int32_t join;
-      for (;;) {
-        if (s != kDummy) {
-          const int32_t tmp = r;
-          r = s;
-          s = tmp;
-        }
-        const int32_t pr = b.par[r];  // an inner vertex's claimer: the next outer vertex up its paths
-        if (pr < 0) {
-          ok = false;
-          break;
-        }
-        r = mt_first(b, pr, budget);
-        if (budget <= 0) {
-          ok = false;
-          break;
-        }
-        if (r == kDummy ? dflag : b.flag[r] == stamp) {
-          join = r;
-          break;
-        }
-        if (r == kDummy) dflag = true; else b.flag[r] = stamp;
-      }
-      if (!ok) break;
-      ++formed;
-      for (int side = 0; side < 2; ++side) {
-        const int32_t p = side ? y : x, q2 = side ? x : y;
-        const unsigned long long label = ((unsigned long long)(uint32_t)p << 32) | (uint32_t)q2;
-        int32_t v = mt_first(b, p, budget);
-        while (v != join) {
-          if (v == kDummy || budget <= 0 || b.par[v] < 0) {  // the walk above reached the join from this side
-            ok = false;
-            break;
-          }
-          const int32_t nextv = mt_first(b, b.par[v], budget);
-          b.elab[v] = label;
-          b.fp[v] = join;
-          b.lvl[v] = next_level;
-          const int32_t dv = b.off[v + 1] - b.off[v];
-          const unsigned long long old = atomicAdd(nxt.ctr, ((unsigned long long)(uint32_t)dv << 32) | 1ull);
-          nxt.v[(uint32_t)(old & 0xFFFFFFFFull)] = v;
-          nxt.eo[(uint32_t)(old & 0xFFFFFFFFull)] = (int32_t)(uint32_t)(old >> 32);
-          ++made;
-          v = nextv;
-        }
-        if (!ok) break;
+// The next non-outer vertex above inner vertex u on every path through it:
+// FIRST of its claimer.  Levels fall strictly along this chain (the claimer's
+// path existed before u was claimed), and kDummy sits at level -1.
+__device__ __forceinline__ int32_t mt_up(const BlossomState& b, int32_t u, int32_t& budget) {
+  return mt_first(b, b.par[u], budget);
+}
+
+__device__ __forceinline__ int32_t mt_depth(const BlossomState& b, int32_t u) { return u == kDummy ? -1 : b.lvl[u]; }
+
+// Blossom step, marking pass.  Processing a tree's candidate edges one by
+// one in key order (Gabow's procedure L) makes an inner vertex outer with the
+// label of the FIRST candidate whose path, in the forest as it stood before
+// the step, runs through it: earlier candidates only merge whole subtrees, so
+// a later candidate's path keeps every inner vertex of its original path and
+// gains none.  So each candidate (x, y) -- in sorted order i -- finds the
+// join of FIRST(x) and FIRST(y) by levels, and writes min(i << 1 | side)
+// into every inner vertex strictly below it; a vertex's first mark lists it.
+__global__ __launch_bounds__(kBlock) void k_mt_blossom_mark(BlossomState b, const unsigned long long* __restrict__ cand,
+                                                            int32_t C, int32_t* __restrict__ join_of,
+                                                            int32_t* __restrict__ marked,
+                                                            uint32_t* __restrict__ nmarked,
+                                                            unsigned* __restrict__ err) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < C; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t j = (int32_t)(cand[i] & 0xFFFFFFFFull);
+    // Owner of adjacency entry j: the last vertex whose offset is <= j.
+    int32_t lo = 0, hi = b.n;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (b.off[mid] <= j)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int32_t x = lo, y = b.col[j];
+    int32_t budget = 4 * b.n + 16;
+    const int32_t r = mt_first(b, x, budget), s = mt_first(b, y, budget);
+    join_of[i] = r;
+    if (r == s) continue;  // x and y already share a blossom
+    int32_t a = r, c = s;
+    while (a != c && budget > 0) {
+      if (mt_depth(b, a) >= mt_depth(b, c)) {
+        if (a == kDummy || b.par[a] < 0) break;
+        a = mt_up(b, a, budget);
+      } else {
+        if (c == kDummy || b.par[c] < 0) break;
+        c = mt_up(b, c, budget);
       }
     }
-    if (!ok) atomicOr(err, 8u);
-    if (formed) atomicAdd(&counts[0], formed);
+    if (a != c || budget <= 0) {
+      atomicOr(err, 8u);
+      continue;
+    }
+    const int32_t join = a;
+    join_of[i] = join;
+    for (int side = 0; side < 2; ++side) {
+      int32_t u = side ? s : r;
+      const unsigned long long code = ((unsigned long long)i << 1) | (unsigned long long)side;
+      while (u != join) {
+        if (u == kDummy || b.par[u] < 0 || budget <= 0) {
+          atomicOr(err, 8u);
+          break;
+        }
+        if (atomicMin(&b.lab[u], code) == kNoKey) marked[atomicAdd(nmarked, 1u)] = u;
+        u = mt_up(b, u, budget);
+      }
+    }
+  }
+}
+
+// Blossom step, applying pass: every marked vertex becomes outer with its
+// candidate's edge label (oriented to its side), FIRST pointer = that
+// candidate's join, an even level, and a place in the next frontier.  These
+// are the reference's odd nodes in a blossom made even
+// (TransformOddNodesInBlossom, xblossom_gpu_engine_10.cu:362-566).
+__global__ __launch_bounds__(kBlock) void k_mt_blossom_apply(BlossomState b, const unsigned long long* __restrict__ cand,
+                                                             const int32_t* __restrict__ join_of,
+                                                             const int32_t* __restrict__ marked,
+                                                             const uint32_t* __restrict__ nmarked,
+                                                             uint32_t* __restrict__ formed, int32_t next_level,
+                                                             Queue nxt, unsigned long long* __restrict__ counts) {
+  const uint32_t cnt = *nmarked;
+  unsigned long long made = 0, blossoms = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * kBlock) {
+    const int32_t u = marked[k];
+    const unsigned long long code = b.lab[u];
+    b.lab[u] = kNoKey;
+    const int64_t i = (int64_t)(code >> 1);
+    const int side = (int)(code & 1ull);
+    const int32_t j = (int32_t)(cand[i] & 0xFFFFFFFFull);
+    int32_t lo = 0, hi = b.n;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (b.off[mid] <= j)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int32_t x = lo, y = b.col[j];
+    const int32_t p = side ? y : x, q = side ? x : y;
+    b.elab[u] = ((unsigned long long)(uint32_t)p << 32) | (uint32_t)q;
+    b.fp[u] = join_of[i];
+    b.lvl[u] = next_level;
+    const int32_t du = b.off[u + 1] - b.off[u];
+    const unsigned long long old = atomicAdd(nxt.ctr, ((unsigned long long)(uint32_t)du << 32) | 1ull);
+    nxt.v[(uint32_t)(old & 0xFFFFFFFFull)] = u;
+    nxt.eo[(uint32_t)(old & 0xFFFFFFFFull)] = (int32_t)(uint32_t)(old >> 32);
+    ++made;
+    if (atomicExch(&formed[i], 1u) == 0u) ++blossoms;
+  }
+  made = wave_sum64(made);
+  blossoms = wave_sum64(blossoms);
+  if (lane_id() == 0) {
+    if (blossoms) atomicAdd(&counts[0], blossoms);
     if (made) atomicAdd(&counts[1], made);
   }
 }
@@ -2259,19 +2278,18 @@ void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Q
   hipLaunchKernelGGL(k_mt_collect, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, cand,
                      ncand, cap, err);
 }
-void launch_mt_segments(hipStream_t st, const unsigned long long* cand, int32_t C, int32_t* seg, uint32_t* nseg) {
-  if (C <= 0) return;
-  hipLaunchKernelGGL(k_mt_segments, dim3(item_grid(C)), dim3(kBlock), 0, st, cand, C, seg, nseg);
-}
 void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
-                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* flag,
-                       const unsigned long long* cand, int32_t C, const int32_t* seg, const uint32_t* nseg,
-                       int32_t max_segs, unsigned long long stamp_base, int32_t next_level, Queue nxt,
+                       int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* lab,
+                       const unsigned long long* cand, int32_t C, int32_t* join_of, int32_t* marked,
+                       uint32_t* nmarked, uint32_t* formed, int32_t next_level, Queue nxt,
                        unsigned long long* counts, unsigned* err) {
-  if (C <= 0 || max_segs <= 0) return;
-  const BlossomState b{off, col, n, par, lvl, fp, elab, flag};
-  hipLaunchKernelGGL(k_mt_blossom, dim3(item_grid(max_segs)), dim3(kBlock), 0, st, b, cand, C, seg, nseg, stamp_base,
-                     next_level, nxt, counts, err);
+  if (C <= 0) return;
+  const BlossomState b{off, col, n, par, lvl, fp, elab, lab};
+  hipLaunchKernelGGL(k_mt_blossom_mark, dim3(item_grid(C)), dim3(kBlock), 0, st, b, cand, C, join_of, marked, nmarked,
+                     err);
+  // Every vertex is marked at most once per step: n bounds the list.
+  hipLaunchKernelGGL(k_mt_blossom_apply, dim3(item_grid(n)), dim3(kBlock), 0, st, b, cand, join_of, marked, nmarked,
+                     formed, next_level, nxt, counts);
 }
 void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,
                        const int32_t* off, Queue out, int32_t* list, uint32_t* ctr32) {

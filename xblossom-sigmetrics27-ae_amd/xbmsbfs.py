@@ -392,9 +392,8 @@ class XBlossomGpuEngine10:
     def FindMaximumMatch(self, initial_matching=None) -> np.ndarray:
         """FindMaximumMatch of apps/xb/src/include/xbsm_gpu/max_match_engine.h:
         84-113: entry v is v's mate, nnodes for an exposed vertex (the
-        reference GPU engine's sentinel).  The forest search has no blossom
-        transform: ``last_match_stats.blossom_edges == 0`` certifies that the
-        result is maximum (always so on bipartite graphs)."""
+        reference GPU engine's sentinel).  The result is a maximum matching,
+        blossoms included (``last_match_stats.blossoms`` counts them)."""
         n = self.graph.nnodes
         init = None
         if initial_matching is not None:
