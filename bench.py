@@ -342,7 +342,8 @@ def bench_match(args) -> int:
         g.n, g.m = n_, int(key.size // 2)
         desc = "G(n, p) n=1M mean degree 4 (seed 1)"
     else:
-        g, _src, desc = make_workload("livejournal")
+        g, _src, _ = make_workload("livejournal")
+        desc = "LiveJournal-shaped Chung-Lu (gamma 2.1, seed 4)"
     hg = xb.HGraph(g.row_offsets, g.col_indices)
     out = xb.DeviceBuffer((g.n,))
     for _ in range(args.warmup):
@@ -394,8 +395,8 @@ def bench_match(args) -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "match: " + desc.split(",")[0].replace("(gamma 2.1", "(gamma 2.1)")
-                               + ", maximum matching from the empty matching", "n": g.n, "m": g.m},
+        "config": {"workload": "match: " + desc + ", maximum matching from the empty matching",
+                   "n": g.n, "m": g.m},
         "matched_pairs": int(want_pairs),
         "phases": st.phases,
         "levels": st.levels,
