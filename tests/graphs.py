@@ -145,6 +145,26 @@ def nested_odd_cycles(k, depth, seed):
     return from_edges(n, e[:, 0], e[:, 1])
 
 
+def with_loops_and_duplicates(ro, ci, seed):
+    """The same graph with self-loops and repeated entries added (raw CSR input
+    may hold both); lists stay sorted."""
+    rng = np.random.default_rng(seed)
+    n = ro.size - 1
+    adj = [list(ci[ro[v]:ro[v + 1]]) for v in range(n)]
+    for v in rng.choice(n, size=max(1, n // 5), replace=False):
+        adj[v].append(v)
+    for v in range(n):
+        if adj[v] and rng.random() < 0.3:
+            adj[v].append(adj[v][int(rng.integers(len(adj[v])))])
+    ro2 = np.zeros(n + 1, dtype=np.int32)
+    cols = []
+    for v in range(n):
+        a = sorted(int(x) for x in adj[v])
+        cols.extend(a)
+        ro2[v + 1] = ro2[v] + len(a)
+    return ro2, np.array(cols, dtype=np.int32)
+
+
 def matching_size(mate):
     mate = np.asarray(mate)
     return int(np.count_nonzero(mate >= 0)) // 2

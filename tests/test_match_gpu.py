@@ -193,3 +193,27 @@ def test_full_size_bipartite_properties():
     want, ws = oracle.match_forest(ro, ci)
     assert np.array_equal(a, want) and st.levels == ws.levels
     hg.close()
+
+
+def test_self_loops_and_duplicate_entries():
+    for seed in range(10):
+        n = 500 + 300 * seed
+        ro, ci = tg.gnp(n, 3.0 / n, 4000 + seed)
+        ro2, ci2 = tg.with_loops_and_duplicates(ro, ci, seed)
+        got, _ = _same(ro2, ci2, label=f"loops and duplicates {seed}")
+        assert tg.matching_size(got) == tg.matching_size(oracle.match_edmonds(ro, ci))
+
+
+def test_repeated_runs_reuse_state():
+    """The graph keeps its matching state between runs: a run after a run
+    from another start, and after a blossom-heavy run, is unchanged."""
+    ro, ci = tg.gnp(20_000, 4.0 / 20_000, 77)
+    hg = xb.HGraph(ro, ci)
+    want, _ = oracle.match_forest(ro, ci)
+    first, _ = xb.match(hg)
+    init = np.full(ro.size - 1, -1, dtype=np.int32)
+    init[0:2] = [1, 0] if 1 in ci[ro[0]:ro[1]] else [-1, -1]
+    xb.match(hg, init)
+    again, _ = xb.match(hg)
+    assert np.array_equal(first, want) and np.array_equal(again, want)
+    hg.close()

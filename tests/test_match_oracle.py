@@ -195,3 +195,15 @@ def test_edge_cases():
         mate, st = oracle.match_forest(ro, ci)
         tg.check_matching(ro, ci, mate)
         assert tg.matching_size(mate) == tg.matching_size(oracle.match_edmonds(ro, ci))
+
+
+def test_self_loops_and_duplicate_entries():
+    """Loops never match and repeated entries change nothing: the size equals
+    Edmonds' maximum on the clean graph."""
+    for seed in range(40):
+        n = 20 + 5 * seed
+        ro, ci = tg.gnp(n, 3.0 / n, 3000 + seed)
+        ro2, ci2 = tg.with_loops_and_duplicates(ro, ci, seed)
+        mate, _ = oracle.match_forest(ro2, ci2)
+        assert oracle.match_violations(ro, ci, mate) == 0
+        assert tg.matching_size(mate) == tg.matching_size(oracle.match_edmonds(ro, ci))

@@ -216,6 +216,9 @@ struct xb_graph {
   int32_t* d_hidx = nullptr;   // engine id -> index in d_heavy, else -1
   int32_t n_heavy = 0;
   Workspace ws;
+  // Forest-mode state (about 90 B per vertex), kept from the first matching
+  // run until the graph is destroyed.
+  std::shared_ptr<void> match_state;
 
   int32_t outdeg(int32_t v) const { return h_off[v + 1] - h_off[v]; }
 
@@ -2001,8 +2004,13 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         found += np;
         cur ^= 1;
         std::swap(ms.roots, ms.roots2);
-        read_ctr(w, w.q[cur].ctr, &qn, &E);
-        nroots = (int32_t)(read_u64(w, ms.ctl + 4) & 0xFFFFFFFFull);
+        // Both counters in one round trip.
+        HIPCHK(hipMemcpyAsync(w.hctr, w.q[cur].ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(w.hctr + 1, ms.ctl + 4, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        qn = (int32_t)(uint32_t)(w.hctr[0] & 0xFFFFFFFFull);
+        E = (int32_t)(uint32_t)(w.hctr[0] >> 32);
+        nroots = (int32_t)(w.hctr[1] & 0xFFFFFFFFull);
       }
       if (found) break;
       // Blossom candidates are the same-tree even-even edges the scan met;
@@ -2082,7 +2090,9 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
     if (n == 0) return;
     const bool dev = (flags & XB_OUT_DEVICE) != 0;
     const hipStream_t st = g->stream;
-    MatchState ms(n);
+    if (!g->match_state)
+      g->match_state = std::shared_ptr<void>(new MatchState(n), [](void* p) { delete (MatchState*)p; });
+    MatchState& ms = *(MatchState*)g->match_state.get();
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
