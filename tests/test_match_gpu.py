@@ -217,3 +217,13 @@ def test_repeated_runs_reuse_state():
     again, _ = xb.match(hg)
     assert np.array_equal(first, want) and np.array_equal(again, want)
     hg.close()
+
+
+def test_reference_known_flip():
+    """apps/xb/test/device/path_table_test.cu:80-124 (matching after
+    AddAugmentingPath(6, 7))."""
+    from test_match_oracle import KNOWN_FLIP_ADJ, KNOWN_FLIP_INIT, KNOWN_FLIP_RESULT
+
+    ro, ci = tg.csr_from_adj(KNOWN_FLIP_ADJ)
+    got, st = _same(ro, ci, np.array(KNOWN_FLIP_INIT, dtype=np.int32), label="reference known flip")
+    assert list(got) == KNOWN_FLIP_RESULT

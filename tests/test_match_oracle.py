@@ -207,3 +207,20 @@ def test_self_loops_and_duplicate_entries():
         mate, _ = oracle.match_forest(ro2, ci2)
         assert oracle.match_violations(ro, ci, mate) == 0
         assert tg.matching_size(mate) == tg.matching_size(oracle.match_edmonds(ro, ci))
+
+
+# The reference's known answer for a path flip
+# (apps/xb/test/device/path_table_test.cu:80-124): tree 0 -(1=2), 0 -(3=4),
+# 4 -(5=6) with 7 exposed; the augmenting path 0-3=4-5=6-7 flips to the
+# matching below (1=2 untouched).  As a graph with that initial matching, the
+# forest finds the same (only) augmenting path.
+KNOWN_FLIP_ADJ = [[1, 3], [0, 2], [1], [0, 4], [3, 5], [4, 6], [5, 7], [6]]
+KNOWN_FLIP_INIT = [-1, 2, 1, 4, 3, 6, 5, -1]
+KNOWN_FLIP_RESULT = [3, 2, 1, 0, 5, 4, 7, 6]
+
+
+def test_reference_known_flip():
+    ro, ci = tg.csr_from_adj(KNOWN_FLIP_ADJ)
+    mate, st = oracle.match_forest(ro, ci, np.array(KNOWN_FLIP_INIT, dtype=np.int32))
+    assert list(mate) == KNOWN_FLIP_RESULT
+    assert st.augmentations == 1
