@@ -57,6 +57,12 @@ constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 #ifndef XB_PULL_U
 #define XB_PULL_U 4
 #endif
+// Write phase of the light emit: 16-byte row stores (4 vertices per lane)
+// where the row segment is aligned.
+#ifndef XB_EMIT_VEC_STORE
+#define XB_EMIT_VEC_STORE 1
+#endif
+
 #ifndef XB_EMIT_U
 #define XB_EMIT_U 4
 #endif
@@ -921,6 +927,51 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
     }
     __syncthreads();
     // ---- write phase ----
+#if XB_EMIT_VEC_STORE
+    if constexpr (W == 1 && K <= 4) {
+      // Lane (row group, quad c4) writes 4 consecutive vertices of a row as
+      // one 16-byte store when the row segment is aligned, else one by one.
+      const int c4 = 4 * (tid % (VG / 4));
+      const int row0 = tid / (VG / 4);
+#pragma unroll 1
+      for (int rr = row0; rr < 64; rr += 256 / (VG / 4)) {
+        const int r = 64 * grp + rr;
+        if (r >= rows) break;
+        int32_t lv[4], pv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          uint32_t c = 0;
+#pragma unroll
+          for (int k = 0; k < KL; ++k) c |= (uint32_t)((L.pl[k][c4 + a] >> rr) & 1ull) << k;
+          const int32_t va = (int32_t)(vb + c4 + a);
+          lv[a] = (int32_t)c - 1;
+          pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? src[r] : L.tile[rr * LD + c4 + a]));
+        }
+        const int64_t v0 = vb + c4;
+        const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v0;
+        const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v0;
+        if (v0 + 3 < n && ((lo | po) & 3) == 0) {
+          typedef int32_t v4i __attribute__((ext_vector_type(4)));
+          if (lvl_out) {
+            const v4i lq = {lv[0], lv[1], lv[2], lv[3]};
+            __builtin_nontemporal_store(lq, reinterpret_cast<v4i*>(lvl_out + lo));
+          }
+          const v4i pq = {pv[0], pv[1], pv[2], pv[3]};
+          __builtin_nontemporal_store(pq, reinterpret_cast<v4i*>(par_out + po));
+        } else {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            if (v0 + a < n) {
+              if (lvl_out) __builtin_nontemporal_store(lv[a], &lvl_out[lo + a]);
+              __builtin_nontemporal_store(pv[a], &par_out[po + a]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+#endif
     const int32_t v2 = (int32_t)(vb + wcol);
     if (v2 < n) {
       int wq = -1;
