@@ -931,18 +931,31 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
     if constexpr (W == 1 && K <= 4) {
       // Lane (row group, quad c4) writes 4 consecutive vertices of a row as
       // one 16-byte store when the row segment is aligned, else one by one.
+      constexpr int RSTEP = 256 / (VG / 4);  // rows per pass (16)
       const int c4 = 4 * (tid % (VG / 4));
       const int row0 = tid / (VG / 4);
+      // The level codes of this lane's 4 vertices in its 64 / RSTEP rows,
+      // one byte per (row, vertex), taken from the planes once.
+      uint32_t cw[64 / RSTEP];
+#pragma unroll
+      for (int it = 0; it < 64 / RSTEP; ++it) cw[it] = 0;
 #pragma unroll 1
-      for (int rr = row0; rr < 64; rr += 256 / (VG / 4)) {
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+          const uint64_t q = L.pl[k][c4 + a] >> row0;
+#pragma unroll
+          for (int it = 0; it < 64 / RSTEP; ++it) cw[it] |= (uint32_t)((q >> (RSTEP * it)) & 1ull) << (8 * a + k);
+        }
+#pragma unroll 1
+      for (int it = 0; it < 64 / RSTEP; ++it) {
+        const int rr = row0 + RSTEP * it;
         const int r = 64 * grp + rr;
         if (r >= rows) break;
         int32_t lv[4], pv[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          uint32_t c = 0;
-#pragma unroll
-          for (int k = 0; k < KL; ++k) c |= (uint32_t)((L.pl[k][c4 + a] >> rr) & 1ull) << k;
+          const uint32_t c = (cw[it] >> (8 * a)) & 0xFFu;
           const int32_t va = (int32_t)(vb + c4 + a);
           lv[a] = (int32_t)c - 1;
           pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? src[r] : L.tile[rr * LD + c4 + a]));
