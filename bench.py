@@ -580,11 +580,16 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        # The same launch's measured HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+        # calibrated) over its measured time, beside the algorithmic figure.
+        "traffic_GBps": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "avg_launch_ms": round(avg_ms, 4),
         "launches": agg_launch[dom],
     }
-    # Whole-step view: section 8(d) bytes of the step over the step time.
+    # Whole-step view: the step's algorithmic bytes (CSR and state streams per
+    # level, plus the output tables) over the step time.
     step_bytes = sum(agg_bytes.values()) / args.steps
     kernels = {k: {"ms_per_step": round(agg_ms[k] / args.steps, 3),
                    "launches_per_step": agg_launch[k] / args.steps,

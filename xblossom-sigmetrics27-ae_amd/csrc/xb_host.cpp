@@ -1694,7 +1694,8 @@ void run_batch(xb_batch* b, xb_run_stats* st) {
     }
     if (j.status != 0) fail(XB_ERR_DEVICE, "a batch tile did not complete");
     max_level = std::max(max_level, j.levels);
-    // Section 8(d) bytes of the tile: L CSR streams, L state streams, outputs.
+    // Algorithmic bytes of the tile: per level one CSR stream and the state
+    // streams (24 B per word per vertex), plus the output tables.
     const double L = j.levels + 1.0, W = (j.cnt + 63) / 64, n = g->n, m = (double)g->m;
     bytes += L * (4.0 * (n + 1) + 4.0 * m) + L * 24.0 * W * n + (j.lvl_out ? 8.0 : 4.0) * j.cnt * n;
   }
