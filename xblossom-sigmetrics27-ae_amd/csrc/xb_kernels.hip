@@ -1040,8 +1040,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32
   commit_phase<T>(self_grid(), s, cur, code, out_off, nxt);
 }
 
+// Minimum workgroups per CU the pull's register budget is sized for: 8 (64
+// VGPRs, 8 waves per SIMD) hides more of its dependent gathers than the 7 its
+// natural 65-69 VGPRs allow (C3: 13.2 -> 12.7 ms of pull per step).
+#ifndef XB_PULL_MIN_BLOCKS
+#define XB_PULL_MIN_BLOCKS 8
+#endif
+
 template <int T, int P, bool REC>
-__global__ __launch_bounds__(kBlock) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
+__global__ __launch_bounds__(kBlock, XB_PULL_MIN_BLOCKS) void k_bp_pull_bucket(BPState s, int cur, uint32_t code, int32_t cnt,
                                                            const int32_t* __restrict__ in_off,
                                                            const int32_t* __restrict__ in_col,
                                                            const int32_t* __restrict__ out_off,
