@@ -1222,52 +1222,89 @@ __device__ bool team_barrier(const TeamSync& ts, Grid g, unsigned* gerr, int pha
 
 // Out-of-line entry points of the phases for the persistent kernel, so each
 // phase keeps its own register allocation instead of one inlined giant.
+// Arguments of a device-function call arrive in vector registers even when
+// every lane holds the same value; the wrappers move them back to scalar
+// registers (readfirstlane) so the phase code addresses memory as it does in
+// a standalone kernel, where they are kernel arguments.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class P>
+__device__ __forceinline__ P* uni(P* p) {
+  return reinterpret_cast<P*>((uintptr_t)uni((int64_t)(uintptr_t)p));
+}
+__device__ __forceinline__ Grid uni(Grid g) { return Grid{uni(g.bid), uni(g.nb)}; }
+__device__ __forceinline__ Queue uni(Queue q) { return Queue{uni(q.v), uni(q.eo), uni(q.ctr)}; }
+__device__ __forceinline__ BPState uni(const BPState& s) {
+  BPState u;
+  u.n = uni(s.n);
+  u.seen = uni(s.seen);
+  u.front[0] = uni(s.front[0]);
+  u.front[1] = uni(s.front[1]);
+  u.nextw = uni(s.nextw);
+  u.planes = uni(s.planes);
+  u.tflag = uni(s.tflag);
+  u.tlist = uni(s.tlist);
+  u.tcount = uni(s.tcount);
+  u.hpar = uni(s.hpar);
+  u.hidx = uni(s.hidx);
+  u.new_of = uni(s.new_of);
+  u.orig_of = uni(s.orig_of);
+  u.in_col_o = uni(s.in_col_o);
+  return u;
+}
+
 template <int T>
 __device__ __noinline__ void nl_seed(Grid g, BPState s, const int32_t* src, int32_t cnt) {
-  seed_phase<T>(g, s, src, cnt);
+  seed_phase<T>(uni(g), uni(s), uni(src), uni(cnt));
 }
 template <int T>
 __device__ __noinline__ void nl_push(Grid g, BPState s, int cur, const int32_t* off, const int32_t* col, Queue q,
                                      int32_t qn, int32_t E) {
-  push_phase<T>(g, s, cur, off, col, q, qn, E);
+  push_phase<T>(uni(g), uni(s), uni(cur), uni(off), uni(col), uni(q), uni(qn), uni(E));
 }
 template <int T>
 __device__ __noinline__ void nl_commit(Grid g, BPState s, int cur, uint32_t code, const int32_t* out_off,
                                        Queue nxt) {
-  commit_phase<T>(g, s, cur, code, out_off, nxt);
+  commit_phase<T>(uni(g), uni(s), uni(cur), uni(code), uni(out_off), uni(nxt));
 }
 template <int T, int P, bool REC>
 __device__ __noinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
                                             const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
                                             const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
-  pull_bucket_phase<T, P, REC>(g, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
+  pull_bucket_phase<T, P, REC>(uni(g), uni(s), uni(cur), uni(code), uni(cnt), uni(in_off), uni(in_col), uni(out_off),
+                               uni(perm), uni(begin), uni(end), uni(nxt));
 }
 template <int T>
 __device__ __noinline__ void nl_pull_hub(Grid g, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                                          const HubChunk* chunks, int32_t n_chunks) {
-  pull_hub_phase<T>(g, s, cur, cnt, in_col, chunks, n_chunks);
+  pull_hub_phase<T>(uni(g), uni(s), uni(cur), uni(cnt), uni(in_col), uni(chunks), uni(n_chunks));
 }
 template <int T>
 __device__ __noinline__ void nl_clear(Grid g, BPState s, int cur, const int32_t* qv, int32_t qn) {
-  clear_phase<T>(g, s, cur, qv, qn);
+  clear_phase<T>(uni(g), uni(s), uni(cur), uni(qv), uni(qn));
 }
 __device__ __noinline__ void nl_fill_u64(Grid g, uint64_t* p, int64_t count, uint64_t val) {
-  fill_u64(g, p, count, val);
+  fill_u64(uni(g), uni(p), uni(count), (uint64_t)uni((int64_t)val));
 }
 __device__ __noinline__ void nl_fill_u32(Grid g, uint32_t* p, int64_t count, uint32_t val) {
-  fill_u32(g, p, count, val);
+  fill_u32(uni(g), uni(p), uni(count), uni(val));
 }
 template <int T, int K>
 __device__ __noinline__ void nl_emit_heavy(Grid g, BPState s, int32_t word, const int32_t* in_off,
                                            const int32_t* in_col, const int32_t* heavy, int32_t n_heavy) {
-  emit_heavy_phase<T, K>(g, s, word, in_off, in_col, heavy, n_heavy);
+  emit_heavy_phase<T, K>(uni(g), uni(s), uni(word), uni(in_off), uni(in_col), uni(heavy), uni(n_heavy));
 }
 template <int T, int K>
 __device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
                                            int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
-  emit_light_phase<T, K>(g, s, word, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out,
-                         RowMap{});
+  emit_light_phase<T, K>(uni(g), uni(s), uni(word), uni(rows), uni(src), uni(in_off), uni(in_col), uni(in_col_orig),
+                         uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{});
 }
 
 template <int T, int K>
