@@ -207,6 +207,10 @@ struct xb_graph {
   int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1;
   int32_t* d_perm = nullptr;          // engine ids with in-degree > 0, bucket-major, ascending per bucket
   int32_t bucket_end[3] = {0, 0, 0};  // perm ranges of the small / medium / large buckets
+  // A bucket whose perm range is consecutive ids (degree-ordered ids of a
+  // symmetric graph) is addressed as index + shift, without the list.
+  bool bucket_contig[3] = {false, false, false};
+  int32_t bucket_shift[3] = {0, 0, 0};
   int64_t bucket_edges[3] = {0, 0, 0};
   xb::HubChunk* d_hub_chunks = nullptr;
   int32_t n_hub_chunks = 0;
@@ -611,8 +615,13 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
           edges += d;
         }
       }
+      const int32_t first = c == 0 ? 0 : g->bucket_end[c - 1];
       g->bucket_end[c] = (int32_t)perm.size();
       g->bucket_edges[c] = edges;
+      bool contig = true;
+      for (int32_t i = first + 1; i < g->bucket_end[c] && contig; ++i) contig = perm[(size_t)i] == perm[(size_t)i - 1] + 1;
+      g->bucket_contig[c] = contig;
+      g->bucket_shift[c] = (first < g->bucket_end[c]) ? perm[(size_t)first] - first : 0;
     }
     std::vector<xb::HubChunk> chunks;
     int32_t hubs = 0;
@@ -937,7 +946,7 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
         if (end > begin) {
           prof.begin(XB_K_PULL);
           xb::launch_bp_pull_bucket(w.st, T, c, s, cur, code, cnt, g->d_ein_off, g->d_ein_col, g->d_eoff,
-                                    g->d_perm, begin, end, qx);
+                                    g->bucket_contig[c] ? nullptr : g->d_perm, g->bucket_shift[c], begin, end, qx);
           prof.end();
           // offsets + in-edges + (front, seen r/w) per vertex and word
           prof.add(XB_K_PULL, 4.0 * (end - begin + 1) + 4.0 * g->bucket_edges[c] + 24.0 * T * (end - begin));
@@ -1457,7 +1466,11 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     v.oin_col = g->d_oin_col;
     v.oin_colo = g->d_oin_colo;
     v.perm = g->d_perm;
-    for (int c = 0; c < 3; ++c) v.bucket_end[c] = g->bucket_end[c];
+    for (int c = 0; c < 3; ++c) {
+      v.bucket_end[c] = g->bucket_end[c];
+      v.bucket_perm[c] = g->bucket_contig[c] ? nullptr : g->d_perm;
+      v.bucket_shift[c] = g->bucket_shift[c];
+    }
     v.n_hub_chunks = g->n_hub_chunks;
     v.hubs = g->d_hub_chunks;
     v.heavy = g->d_heavy;

@@ -74,6 +74,10 @@ struct GraphView {
   const int32_t* oin_colo;  // the same entries as original ids
   const int32_t* perm;
   int32_t bucket_end[3];
+  // Per bucket: its id list, or null when the bucket is the id range
+  // [begin, end) + bucket_shift (degree-ordered ids of a symmetric graph).
+  const int32_t* bucket_perm[3];
+  int32_t bucket_shift[3];
   int32_t n_hub_chunks;
   const HubChunk* hubs;
   const int32_t* heavy;
@@ -163,10 +167,11 @@ void launch_bp_push(hipStream_t st, int T, BPState s, int cur, const int32_t* of
                     Queue q, int32_t qn, int32_t E);
 void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, const int32_t* out_off,
                       Queue nxt, int32_t max_touched);
-// cls 0: T lanes per vertex, 1: 4T lanes, 2: one wave per vertex.
+// cls 0: T lanes per vertex, 1: 4T lanes, 2: one wave per vertex.  perm null:
+// vertex = index + shift.
 void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
                            const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
-                           const int32_t* perm, int32_t begin, int32_t end, Queue nxt);
+                           const int32_t* perm, int32_t shift, int32_t begin, int32_t end, Queue nxt);
 void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);

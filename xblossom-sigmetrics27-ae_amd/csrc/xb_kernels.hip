@@ -570,7 +570,7 @@ template <int T, int P, bool REC>
 __device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
                                   const int32_t* __restrict__ in_off, const int32_t* __restrict__ in_col,
                                   const int32_t* __restrict__ out_off, const int32_t* __restrict__ perm,
-                                  int32_t begin, int32_t end, Queue nxt) {
+                                  int32_t shift, int32_t begin, int32_t end, Queue nxt) {
   constexpr int G = T * P;
   constexpr int VPW = kWave / G;
   constexpr int VPB = (kBlock / kWave) * VPW;
@@ -591,7 +591,7 @@ __device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int
     uint64_t old = 0, need = 0;
     uint32_t* hp = nullptr;
     if (act) {
-      v = perm[idx];
+      v = perm ? perm[idx] : (int32_t)(idx + shift);  // no list: the bucket is the id range [begin, end) + shift
       b = in_off[v];
       e = in_off[v + 1];
       old = s.seen[(int64_t)v * T + t];
@@ -1052,9 +1052,9 @@ __global__ __launch_bounds__(kBlock, XB_PULL_MIN_BLOCKS) void k_bp_pull_bucket(B
                                                            const int32_t* __restrict__ in_off,
                                                            const int32_t* __restrict__ in_col,
                                                            const int32_t* __restrict__ out_off,
-                                                           const int32_t* __restrict__ perm, int32_t begin,
-                                                           int32_t end, Queue nxt) {
-  pull_bucket_phase<T, P, REC>(self_grid(), s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end,
+                                                           const int32_t* __restrict__ perm, int32_t shift,
+                                                           int32_t begin, int32_t end, Queue nxt) {
+  pull_bucket_phase<T, P, REC>(self_grid(), s, cur, code, cnt, in_off, in_col, out_off, perm, shift, begin, end,
                                nxt);
 }
 
@@ -1275,9 +1275,10 @@ __device__ __noinline__ void nl_commit(Grid g, BPState s, int cur, uint32_t code
 template <int T, int P, bool REC>
 __device__ __noinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
                                             const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
-                                            const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
+                                            const int32_t* perm, int32_t shift, int32_t begin, int32_t end,
+                                            Queue nxt) {
   pull_bucket_phase<T, P, REC>(uni(g), uni(s), uni(cur), uni(code), uni(cnt), uni(in_off), uni(in_col), uni(out_off),
-                               uni(perm), uni(begin), uni(end), uni(nxt));
+                               uni(perm), uni(shift), uni(begin), uni(end), uni(nxt));
 }
 template <int T>
 __device__ __noinline__ void nl_pull_hub(Grid g, BPState s, int cur, int32_t cnt, const int32_t* in_col,
@@ -1371,17 +1372,18 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
     if (pull) {
       int32_t begin = 0;
       if (gv.bucket_end[0] > begin)
-        nl_pull_bucket<T, 1, false>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.perm, begin,
-                                       gv.bucket_end[0], qx);
+        nl_pull_bucket<T, 1, false>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.bucket_perm[0],
+                                       gv.bucket_shift[0], begin, gv.bucket_end[0], qx);
       begin = gv.bucket_end[0];
       if (gv.bucket_end[1] > begin)
         nl_pull_bucket<T, (T * 4 <= kWave ? 4 : kWave / T), false>(g, s, cur, code, job.cnt, gv.in_off,
-                                                                      gv.in_col, gv.off, gv.perm, begin,
+                                                                      gv.in_col, gv.off, gv.bucket_perm[1],
+                                                                      gv.bucket_shift[1], begin,
                                                                       gv.bucket_end[1], qx);
       begin = gv.bucket_end[1];
       if (gv.bucket_end[2] > begin)
-        nl_pull_bucket<T, kWave / T, true>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off, gv.perm,
-                                              begin, gv.bucket_end[2], qx);
+        nl_pull_bucket<T, kWave / T, true>(g, s, cur, code, job.cnt, gv.in_off, gv.in_col, gv.off,
+                                              gv.bucket_perm[2], gv.bucket_shift[2], begin, gv.bucket_end[2], qx);
       if (gv.n_hub_chunks > 0) nl_pull_hub<T>(g, s, cur, job.cnt, gv.in_col, gv.hubs, gv.n_hub_chunks);
     } else {
       nl_push<T>(g, s, cur, gv.off, gv.col, qc, qn, E);
@@ -2248,26 +2250,26 @@ void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, c
 
 template <int T, int P, bool REC>
 static void pull_bucket_launch(hipStream_t st, BPState s, int cur, int32_t code, int32_t cnt, const int32_t* in_off,
-                               const int32_t* in_col, const int32_t* out_off, const int32_t* perm, int32_t begin,
-                               int32_t end, Queue nxt) {
+                               const int32_t* in_col, const int32_t* out_off, const int32_t* perm, int32_t shift,
+                               int32_t begin, int32_t end, Queue nxt) {
   constexpr int VPB = (kBlock / kWave) * (kWave / (T * P));
   hipLaunchKernelGGL((k_bp_pull_bucket<T, P, REC>), dim3(grid_for((int64_t)(end - begin), VPB)), dim3(kBlock), 0,
-                     st, s, cur, (uint32_t)code, cnt, in_off, in_col, out_off, perm, begin, end, nxt);
+                     st, s, cur, (uint32_t)code, cnt, in_off, in_col, out_off, perm, shift, begin, end, nxt);
 }
 
 void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
                            const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
-                           const int32_t* perm, int32_t begin, int32_t end, Queue nxt) {
+                           const int32_t* perm, int32_t shift, int32_t begin, int32_t end, Queue nxt) {
   if (end <= begin) return;
   if (cls == 0) {
-    XB_DISPATCH_T(T, (pull_bucket_launch<TT, 1, false>(st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin,
-                                                       end, nxt)));
+    XB_DISPATCH_T(T, (pull_bucket_launch<TT, 1, false>(st, s, cur, code, cnt, in_off, in_col, out_off, perm, shift,
+                                                       begin, end, nxt)));
   } else if (cls == 1) {
     XB_DISPATCH_T(T, (pull_bucket_launch<TT, (TT * 4 <= kWave ? 4 : kWave / TT), false>(
-                         st, s, cur, code, cnt, in_off, in_col, out_off, perm, begin, end, nxt)));
+                         st, s, cur, code, cnt, in_off, in_col, out_off, perm, shift, begin, end, nxt)));
   } else {
     XB_DISPATCH_T(T, (pull_bucket_launch<TT, kWave / TT, true>(st, s, cur, code, cnt, in_off, in_col, out_off, perm,
-                                                               begin, end, nxt)));
+                                                               shift, begin, end, nxt)));
   }
 }
 
