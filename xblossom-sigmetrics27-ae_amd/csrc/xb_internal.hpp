@@ -23,12 +23,13 @@ namespace xb {
 // 64-bit source words per vertex ([v*T + t], engine ids).  planes hold 8
 // bit-sliced planes of the level code (code = level + 1, 0 = unreached), bit
 // b of plane k = bit k of code(source 64t+b, v), in word groups of
-// W = min(T, 4) words and two halves (W is the emit's words per pass):
+// W = min(T, XB_PLANE_GROUP_MAX) words (1 by default: word-major) and two
+// halves:
 //   plane k < 4:   planes[((t/W * n + v) * W + t%W) * 4 + k]
 //   plane k >= 4:  planes[T*n*4 + ((t/W * n + v) * W + t%W) * 4 + k - 4]
-// so a vertex's first 4 planes of a word group are one 128-byte line (four
-// words' codes up to 15, i.e. levels <= 14): the emit's random in-neighbour
-// reads use every byte they fetch.
+// so the first 4 planes of a (vertex, word) are 32 contiguous bytes (codes up
+// to 15, i.e. levels <= 14), one gather per in-neighbour in the emit, and one
+// emit pass (one word) reads a single n*32-byte region.
 struct BPState {
   int64_t n;            // vertices (plane addressing)
   uint64_t* seen;
