@@ -132,6 +132,8 @@ struct alignas(256) TeamCtl {
   XcdSlot xcd[kMaxXcd];
   unsigned long long ctr[3];  // packed (edges << 32 | count) queue counters, level d uses ctr[d % 3]
   uint32_t tcount[2];         // touched-list counts, level d uses tcount[d & 1]
+  uint32_t emit_chunk;        // the light emit's shared (word group, chunk) counter, reset per tile
+  uint32_t pad1_;
   // Phase clocks (wall_clock64 ticks) kept by workgroup 0 of the team: the
   // span between two team barriers is charged to the phase that ended at the
   // second one, indexed by the XB_K_* class of that phase.

@@ -173,6 +173,7 @@ constexpr int kEmitTileInts = 256 * 17;
 struct EmitLds {
   int32_t tile[kEmitTileInts];
   uint64_t pl[4][64];  // plane k of (vertex, word) of the block, compute -> write phase (lane-contiguous)
+  int64_t next;        // dynamic schedule: the block's next (word group, chunk)
 };
 
 union Lds {
@@ -828,7 +829,8 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
                                  const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
                                  int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out,
-                                 int32_t* __restrict__ par_out, RowMap map) {
+                                 int32_t* __restrict__ par_out, RowMap map, unsigned* dyn = nullptr,
+                                 int32_t ngrp = 0) {
   constexpr int W = kGroupWords<T>;
   constexpr int PART = 4;
   constexpr int VG = 64 / W;
@@ -841,12 +843,29 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
   const int w = tid % W;
   const int p = (tid / W) % PART;
   const int vi = tid / (W * PART);
-  const int t = grp * W + w;          // this lane's word of the tile
+  int t = grp * W + w;                // this lane's word of the tile
   const int rb = 64 * w;              // its rows in the block tile
   const int wcol = tid % VG;          // write phase: vertex column
   const int wrow = tid / VG;          // write phase: first row
   __syncthreads();
-  for (int64_t vb = (int64_t)g.bid * VG; vb < n; vb += (int64_t)g.nb * VG) {
+  // Static schedule: chunk bid, bid + nb, ... of word group grp.  Dynamic
+  // (dyn non-null): the team takes (word group, chunk) pairs of ngrp groups
+  // from a shared counter, so no workgroup idles behind a slower one.
+  const int64_t nchunks = ((int64_t)n + VG - 1) / VG;
+  for (int64_t it = g.bid;; it += g.nb) {
+    int64_t vb;
+    if (dyn) {
+      if (tid == 0) L.next = (int64_t)atomicAdd(dyn, 1u);
+      __syncthreads();
+      const int64_t c = L.next;
+      if (c >= (int64_t)ngrp * nchunks) break;
+      grp = (int32_t)(c / nchunks);
+      vb = (c % nchunks) * VG;
+      t = grp * W + w;
+    } else {
+      vb = it * VG;
+      if (vb >= n) break;
+    }
     // ---- compute phase ----
     const int32_t v = (int32_t)(vb + vi);
     const bool act = v < n;
@@ -1303,9 +1322,10 @@ __device__ __noinline__ void nl_emit_heavy(Grid g, BPState s, int32_t word, cons
 template <int T, int K>
 __device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
-                                           int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out) {
+                                           int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out,
+                                           unsigned* dyn, int32_t ngrp) {
   emit_light_phase<T, K>(uni(g), uni(s), uni(word), uni(rows), uni(src), uni(in_off), uni(in_col), uni(in_col_orig),
-                         uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{});
+                         uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{}, uni(dyn), uni(ngrp));
 }
 
 template <int T, int K>
@@ -1314,11 +1334,13 @@ __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const Til
   const int words = (job.cnt + 63) / 64;
   if (gv.n_heavy > 0)
     for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
+  // The light emit of every word group shares one counter, reset before the
+  // barrier that precedes it (the reset is visible to the team after it).
+  if (g.bid == 0 && threadIdx.x == 0) atomicExch(&ts.ctl->emit_chunk, 0u);
   if (!team_barrier(ts, g, gerr, XB_K_EMIT_HEAVY)) return false;
-  for (int grp = 0; grp * kGroupWords<T> < words; ++grp)
-    nl_emit_light<T, K>(g, s, grp, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.oin_colo, gv.n, gv.cut_medium,
-                        job.lvl_out,
-                        job.par_out);
+  const int32_t ngrp = (words + kGroupWords<T> - 1) / kGroupWords<T>;
+  nl_emit_light<T, K>(g, s, 0, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.oin_colo, gv.n, gv.cut_medium,
+                      job.lvl_out, job.par_out, &ts.ctl->emit_chunk, ngrp);
   return true;
 }
 
