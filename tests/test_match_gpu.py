@@ -227,3 +227,28 @@ def test_reference_known_flip():
     ro, ci = tg.csr_from_adj(KNOWN_FLIP_ADJ)
     got, st = _same(ro, ci, np.array(KNOWN_FLIP_INIT, dtype=np.int32), label="reference known flip")
     assert list(got) == KNOWN_FLIP_RESULT
+
+
+def test_general_graphs_from_greedy_starts():
+    """Random greedy initial matchings on blossom-forming graphs: GPU equals
+    the oracle from the same start, and reaches Edmonds' maximum."""
+    rng = np.random.default_rng(12)
+    for seed in range(60):
+        if seed % 2:
+            ro, ci = tg.nested_odd_cycles(int(rng.choice([3, 5])), int(rng.integers(1, 4)), 700 + seed)
+        else:
+            n = int(rng.integers(20, 400))
+            ro, ci = tg.gnp(n, min(1.0, float(rng.uniform(1.0, 6.0)) / n), 800 + seed)
+        n = ro.size - 1
+        init = np.full(n, -1, dtype=np.int32)
+        for v in rng.permutation(n):
+            if init[v] >= 0:
+                continue
+            for w in ci[ro[v]:ro[v + 1]]:
+                if init[w] < 0 and w != v:
+                    init[v], init[w] = w, v
+                    break
+        got, st = _same(ro, ci, init, label=f"greedy start {seed}")
+        best = tg.matching_size(oracle.match_edmonds(ro, ci))
+        assert tg.matching_size(got) == best, f"greedy start {seed}"
+        assert st.augmentations == best - tg.matching_size(init)
