@@ -501,6 +501,11 @@ __device__ void push_phase(Grid g, BPState s, int cur, const int32_t* __restrict
         const int i = chunk_owner(co, q.eo, L.s_eo, e);
         const int32_t v = q.v[i];
         u = col[off[v] + (e - q.eo[i])];
+        // A heavy vertex takes its parents here as in the pull: every pusher
+        // offers itself for each bit it brings, and the lowest original id
+        // wins, so the emit need not scan its long in-list for push levels.
+        const int32_t h = s.hidx[u];
+        const uint32_t vo = h >= 0 ? (uint32_t)s.orig_of[v] : 0u;
         bool any = false;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
@@ -509,6 +514,10 @@ __device__ void push_phase(Grid g, BPState s, int cur, const int32_t* __restrict
           uint64_t nb = f & ~s.seen[(int64_t)u * T + t];
           if (!nb) continue;
           any = true;
+          if (h >= 0) {
+            uint32_t* hp = s.hpar + ((int64_t)h * T + t) * 64;
+            for (uint64_t r = nb; r; r &= r - 1) atomicMin(&hp[__builtin_ctzll(r)], vo);
+          }
           nb &= ~s.nextw[(int64_t)u * T + t];
           if (nb) atomicOr((unsigned long long*)&s.nextw[(int64_t)u * T + t], (unsigned long long)nb);
         }
@@ -769,10 +778,11 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
   return m;
 }
 
-// Heavy vertices (in-degree above the medium pull cut): the pull recorded the
-// parents of every bit it discovered; bits discovered by push levels are
-// resolved here, one wave per vertex, lane i testing in-neighbour j0+i, the
-// lowest lane offering a bit (exclusive prefix-OR) owning that bit's parent.
+// Heavy vertices (in-degree above the medium pull cut): the pull and the push
+// record the parent of every bit they discover (lowest original id by
+// atomicMin); a bit still unrecorded is resolved here, one wave per vertex,
+// lane i testing in-neighbour j0+i, the lowest lane offering a bit (exclusive
+// prefix-OR) owning that bit's parent.
 template <int T, int K>
 __device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t* __restrict__ in_off,
                                  const int32_t* __restrict__ in_col, const int32_t* __restrict__ heavy,
