@@ -24,12 +24,25 @@ def test_autotune_report_and_parity(graph):
     assert len(d["trials"]) >= 20 and 1 <= d["passes"] <= 2
     assert 0 < d["best_ms_per_source"] <= d["default_ms_per_source"]
     t = eng.tunables
-    assert t["words_per_tile"] in (1, 2, 4, 8)
+    assert t["words_per_tile"] in (1, 2, 4, 8, 16)
     assert t["pull_cut_small"] <= t["pull_cut_medium"] <= t["pull_cut_hub"]
     # The tuned options give the same tables.
     src = xb.sample_sources(g, 150, seed=23)
     lv, pa = eng.MultiSourceBFS(src, mode=xb.XB_MODE_BATCHED)
     lr, pr, _ = oracle.msbfs(g.row_offsets, g.col_indices, src, threads=8)
+    assert np.array_equal(lv, lr) and np.array_equal(pa, pr)
+
+
+def test_autotune_sample_wider_than_eight_words(graph):
+    """A 700-source sample: the 16-word trial runs a 700-row tile (the
+    sample buffer holds one tile of the widest trial)."""
+    g, hg = graph
+    sample = xb.sample_sources(g, 700, seed=24)
+    best, rep = xb.autotune(hg, sample)
+    widths = {t["words_per_tile"] for t in rep.as_dict()["trials"]}
+    assert {8, 16} <= widths
+    lv, pa = xb.XBlossomGpuEngine10(hg).MultiSourceBFS(sample, mode=xb.XB_MODE_BATCHED, **best)
+    lr, pr, _ = oracle.msbfs(g.row_offsets, g.col_indices, sample, threads=8)
     assert np.array_equal(lv, lr) and np.array_equal(pa, pr)
 
 

@@ -39,7 +39,7 @@ def _check(g, hg, src, **kw):
     return eng.last_stats
 
 
-@pytest.mark.parametrize("T", [1, 2, 4, 8])
+@pytest.mark.parametrize("T", [1, 2, 4, 8, 16])
 @pytest.mark.parametrize("alpha", [1, 20, 1_000_000_000])
 def test_batched_rmat(rmat12, T, alpha):
     g, hg = rmat12
@@ -48,7 +48,7 @@ def test_batched_rmat(rmat12, T, alpha):
     assert st.tiles == -(-150 // (64 * T))
 
 
-@pytest.mark.parametrize("T", [1, 8])
+@pytest.mark.parametrize("T", [1, 8, 16])
 def test_batched_heavy_kernels(rmat12, T):
     g, hg = rmat12
     src = xb.sample_sources(g, 200, seed=4)
@@ -107,7 +107,7 @@ def test_medium_path_no_fallback():
     assert st.fallback_sources == 0 and st.max_level == 199
 
 
-@pytest.mark.parametrize("T", [1, 2, 4, 8])
+@pytest.mark.parametrize("T", [1, 2, 4, 8, 16])
 def test_bucket_mix(T):
     """Every pull bucket class plus hub pieces on one graph (cut-offs chosen so all are populated)."""
     g = xb.gen_chung_lu(30_000, 2.1, 12.0, 3_000, seed=21)

@@ -58,7 +58,7 @@ def _alone_checksums(hg, src, **tunables):
     return out
 
 
-@pytest.mark.parametrize("T", [1, 8])
+@pytest.mark.parametrize("T", [1, 8, 16])
 def test_mixed_equals_alone(web, T):
     g, hg = web
     reqs = _requests(g, 40, seed=11 + T)

@@ -1020,6 +1020,9 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
 // Words per vertex for a tile of cnt sources: the smallest power of two with
 // 64 * words >= cnt, at most Tmax.  State traffic scales with the words, so a
 // short tile does not pay for the empty ones.
+// Widest tile: 16 words = 1,024 sources (words_per_tile 1, 2, 4, 8 or 16).
+constexpr int kMaxTileWords = 16;
+
 int tile_words(int32_t cnt, int Tmax) {
   int t = 1;
   while (t * 64 < cnt && t < Tmax) t *= 2;
@@ -1078,7 +1081,8 @@ xb_msbfs_opts parse_opts(const xb_msbfs_opts* opts) {
   }
   if (o.mode < XB_MODE_AUTO || o.mode > XB_MODE_BATCHED) fail(XB_ERR_INVALID_ARG, "unknown mode");
   const int T = o.words_per_tile;
-  if (T != 1 && T != 2 && T != 4 && T != 8) fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4 or 8");
+  if (T < 1 || T > kMaxTileWords || (T & (T - 1)) != 0)
+    fail(XB_ERR_INVALID_ARG, "words_per_tile must be 1, 2, 4, 8 or 16");
   if (o.alpha < 1 || o.pull_cut_small < 1 || o.pull_cut_medium < o.pull_cut_small ||
       o.pull_cut_hub < o.pull_cut_medium || o.hub_chunk < 1)
     fail(XB_ERR_INVALID_ARG, "bad tunable");
@@ -1369,6 +1373,10 @@ int32_t xb_last_error(char* buf, size_t len) {
 // ===========================================================================
 // Batched multi-graph runs: one persistent, hipGraph-captured launch
 // ===========================================================================
+// The persistent kernel instantiates its tile loop for 1, 2, 4 and 8 words;
+// a batch asked for 16-word tiles runs 512-source tiles.
+constexpr int kBatchMaxWords = 8;
+
 struct xb_batch {
   hipStream_t stream = nullptr;
   int T = 8;
@@ -1437,7 +1445,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   }
   std::unique_ptr<xb_batch> b(new xb_batch());
   const BPOpts bo = bp_opts(o);
-  b->T = bo.T;
+  b->T = std::min(bo.T, kBatchMaxWords);
   b->alpha = bo.alpha;
   b->cut_small = bo.cut_small;
   b->cut_medium = bo.cut_medium;
@@ -2217,7 +2225,7 @@ extern "C" int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, co
     cur.flags = XB_OUT_DEVICE;
     if (report) std::memset(report, 0, sizeof(*report));
     TuneRun tr{g, sample, S, nullptr, nullptr, report};
-    const int32_t rows = std::min(S, 512);
+    const int32_t rows = std::min(S, 64 * kMaxTileWords);  // one tile of the widest trial
     struct Free {
       TuneRun& t;
       ~Free() {
@@ -2235,7 +2243,7 @@ extern "C" int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, co
       std::vector<int32_t> values;
     };
     const std::vector<Axis> axes = {
-        {&xb_msbfs_opts::words_per_tile, {1, 2, 4, 8}},
+        {&xb_msbfs_opts::words_per_tile, {1, 2, 4, 8, 16}},
         {&xb_msbfs_opts::alpha, {6, 10, 14, 20, 28, 40, 64}},
         {&xb_msbfs_opts::pull_cut_small, {4, 8, 16}},
         {&xb_msbfs_opts::pull_cut_medium, {32, 64, 128, 256}},

@@ -2222,6 +2222,10 @@ void launch_plain_push(hipStream_t st, const int32_t* off, const int32_t* col, Q
       constexpr int TT = 8;    \
       CALL;                    \
     } break;                   \
+    case 16: {                 \
+      constexpr int TT = 16;   \
+      CALL;                    \
+    } break;                   \
     default:                   \
       break;                   \
   }
