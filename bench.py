@@ -221,7 +221,8 @@ def bench_serving(args) -> int:
         S = max(1, min(256, int(np.floor(np.exp(rng_s.uniform(0.0, np.log(257.0)))))))
         reqs.append(rng_s.choice(cand, size=S, replace=True).astype(np.int32))
     warm, reqs = reqs[: args.warmup], reqs[args.warmup:]
-    srv = xb.Server(hg, slots=args.slots)
+    tun = {"words_per_tile": args.words_per_tile} if args.words_per_tile else {}
+    srv = xb.Server(hg, slots=args.slots, **tun)
     for s_ in warm:
         rid = srv.submit(s_)
         srv.wait(rid)
@@ -285,7 +286,7 @@ def bench_serving(args) -> int:
         "data": "synthetic",
         "config": {"workload": "serving: directed R-MAT n=20M (a=.6 b=.15 c=.15, seed 7), Poisson arrivals "
                                f"{args.rate:g}/s (seed 7), 1-256 sources per query (seed 8)",
-                   "n": g.n, "m": g.m, "queries": N, "slots": args.slots,
+                   "n": g.n, "m": g.m, "queries": N, "slots": args.slots, "words_per_tile": args.words_per_tile or 8,
                    "sources": int(sum(x[5] for x in infos))},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
