@@ -896,12 +896,16 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
     prof.add(XB_K_MEMSET, 24.0 * cap + 4.0 * n);
     w.bp_dirty = false;
   }
+  // Planes 0-3 (codes below 16) are cleared here; planes 4-7 only before the
+  // first level whose code needs them (level 15), which most tiles never reach.
+  const size_t half = (size_t)T * (size_t)n * 4;
   HIPCHK(hipMemsetAsync(s.seen, 0, sizeof(uint64_t) * nw, w.st));
-  HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * nw * 8, w.st));
+  HIPCHK(hipMemsetAsync(s.planes, 0, sizeof(uint64_t) * half, w.st));
   if (g->n_heavy > 0)
     HIPCHK(hipMemsetAsync(s.hpar, 0xFF, sizeof(uint32_t) * (size_t)g->n_heavy * T * 64, w.st));
   prof.end();
-  prof.add(XB_K_MEMSET, 72.0 * nw + 256.0 * T * g->n_heavy);
+  prof.add(XB_K_MEMSET, 8.0 * nw + 8.0 * half + 256.0 * T * g->n_heavy);
+  bool hi_planes_clear = false;
   // First frontier: the distinct source vertices with their degree prefix.
   std::vector<int32_t> uniq(src, src + cnt);
   std::sort(uniq.begin(), uniq.end());
@@ -934,6 +938,13 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
     if (code > 255) {
       overflow = true;
       break;
+    }
+    if (code >= 16 && !hi_planes_clear) {
+      prof.begin(XB_K_MEMSET);
+      HIPCHK(hipMemsetAsync(s.planes + half, 0, sizeof(uint64_t) * half, w.st));
+      prof.end();
+      prof.add(XB_K_MEMSET, 8.0 * half);
+      hi_planes_clear = true;
     }
     xb::Queue& qc = w.q[cur];
     xb::Queue& qx = w.q[cur ^ 1];
