@@ -275,11 +275,14 @@ int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, u
  * No counterpart in the reference, whose thresholds are hand-set constants
  * (the |E|/20 push/pull switch of apps/ligra/ligra/ligra.h:236-277 and the
  * XB engine's fixed launch shapes).  Coordinate descent over words per
- * tile, the push/pull switch alpha and the pull bucket cut-offs: each trial
- * runs one tile of the caller's sample sources on the device and is scored
- * in milliseconds per source (best of two runs); a value replaces the
- * current one only if it is at least 2% faster.  The sample's results are
- * discarded.  *best receives the tuned options (mode XB_MODE_BATCHED). */
+ * tile (1, 2, 4, 8, 16), the push/pull switch alpha and the pull bucket
+ * cut-offs: each trial runs one tile of the caller's sample sources (the
+ * first 64*words_per_tile of them) on the device and is scored in
+ * milliseconds per source (best of two runs); a value replaces the current
+ * one only if it is at least 2% faster.  The trials write into two scratch
+ * tables of min(S, 1024) x n int32 that are freed on return; the sample's
+ * results are discarded.  *best receives the tuned options (mode
+ * XB_MODE_BATCHED). */
 typedef struct xb_tune_trial {
   int32_t words_per_tile, alpha, pull_cut_small, pull_cut_medium, pull_cut_hub, hub_chunk;
   double ms_per_source;
