@@ -16,7 +16,7 @@
  * 256-271: paths_[off..off+1] = [w, v], root_distance[x] = root_distance[v]+2).
  * The CPU engine's expand step does the same with a std::thread pool
  * (apps/xb/src/xblsm_cpu/par_pro_pernode.cu:178-234).  The reference lets the
- * first atomic winner pick the anchor; this build fixes the choice to the
+ * first atomic winner pick the anchor; this restatement fixes the choice to the
  * LOWEST-ID in-neighbour one level closer, so every run is bit-identical.
  *
  *   level[s][v]  = hop distance from sources[s] to v, -1 if unreached
