@@ -446,7 +446,7 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: the multi-source BFS path shards by source tile with no "
-              "per-level exchange, and this build runs on a single MI355X; use --gpus 1.", file=sys.stderr)
+              "per-level exchange, and the engine runs on a single MI355X; use --gpus 1.", file=sys.stderr)
         return 2
 
     import numpy as np
