@@ -107,6 +107,29 @@ def test_medium_path_no_fallback():
     assert st.fallback_sources == 0 and st.max_level == 199
 
 
+def test_deep_and_shallow_tiles_share_planes():
+    """Level codes 16 and above use planes 4-7, which a tile clears only when
+    it first reaches level 15.  A 120-vertex path (levels to 119) beside a
+    40-leaf star (levels <= 2): 16-word tiles alternate shallow / deep /
+    shallow / deep in one run and across runs on the same workspace."""
+    import graphs as tg
+
+    ro, ci = tg.two_components(120, 1)
+    star_ro, star_ci = tg.star(41)
+    n1 = ro.size - 1
+    ro = np.concatenate([ro, star_ro[1:] + ro[-1]]).astype(np.int32)
+    ci = np.concatenate([ci, star_ci + n1]).astype(np.int32)
+    g = xb.Csr(ro, ci)
+    hg = xb.HGraph(ro, ci)
+    rng = np.random.default_rng(31)
+    shallow = rng.integers(n1, n1 + 41, size=1024).astype(np.int32)
+    deep = rng.integers(0, 120, size=1024).astype(np.int32)
+    for order in ((shallow, deep, shallow, deep[:300]), (deep, shallow[:700], deep[:100])):
+        src = np.concatenate(order)
+        st = _check(g, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=16)
+        assert st.fallback_sources == 0 and st.max_level == 119
+
+
 @pytest.mark.parametrize("T", [1, 2, 4, 8, 16])
 def test_bucket_mix(T):
     """Every pull bucket class plus hub pieces on one graph (cut-offs chosen so all are populated)."""
