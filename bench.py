@@ -62,21 +62,30 @@ def make_workload(name: str):
     return g, src, desc
 
 
-def traffic_from_profiles(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary, if present.
-
-    The summary file profiles/pmc_<kernel>_<workload>.txt holds a line
-    'traffic_bytes_per_launch=<float>' (FETCH_SIZE + WRITE_SIZE per dispatch,
-    scaled by the copy-kernel calibration factor)."""
+def profile_values(workload: str, kernel: str) -> dict:
+    """The key=value lines of a committed rocprofv3 PMC summary,
+    profiles/pmc_<kernel>_<workload>.txt, if present: traffic_bytes_per_launch
+    (FETCH_SIZE + WRITE_SIZE per dispatch of `kernel`, scaled by the
+    copy-kernel calibration factors), and for the whole step
+    step_traffic_bytes (the same over every engine kernel of one step) with
+    the configuration it was profiled in (profiled_engine,
+    profiled_words_per_tile)."""
     sym = KERNEL_SYMBOL.get(kernel, kernel)
     path = os.path.join(PROFILES, f"pmc_{sym}_{workload}.txt")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
-            if line.startswith("traffic_bytes_per_launch="):
-                return float(line.split("=", 1)[1])
-    return None
+    vals = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                key, eq, val = line.strip().partition("=")
+                if eq and key.isidentifier():
+                    vals[key] = val
+    return vals
+
+
+def traffic_from_profiles(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, or None."""
+    v = profile_values(workload, kernel).get("traffic_bytes_per_launch")
+    return float(v) if v else None
 
 
 def cpu_baseline(g, src, seconds: float, threads: int):
@@ -592,6 +601,17 @@ def main() -> int:
     # Whole-step view: the step's algorithmic bytes (CSR and state streams per
     # level, plus the output tables) over the step time.
     step_bytes = sum(agg_bytes.values()) / args.steps
+    # Whole-step measured traffic, when this run is in the profiled
+    # configuration: the engine kernels' calibrated PMC bytes of one step plus
+    # the fills' bytes (a fill writes exactly its size).
+    prof_vals = profile_values(args.workload, dom)
+    step_traffic = None
+    wpt = tun.get("words_per_tile", args.words_per_tile or 8)
+    if ("step_traffic_bytes" in prof_vals and prof_vals.get("profiled_engine") == engine
+            and prof_vals.get("profiled_words_per_tile") == str(wpt)):
+        tb = float(prof_vals["step_traffic_bytes"]) + agg_bytes.get("memset", 0.0) / args.steps
+        step_traffic = {"bytes": tb, "GBps": round(tb / (ms_per_step * 1e-3) / 1e9, 1),
+                        "frac": round(tb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     kernels = {k: {"ms_per_step": round(agg_ms[k] / args.steps, 3),
                    "launches_per_step": agg_launch[k] / args.steps,
                    "GBps": round(agg_bytes[k] / (agg_ms[k] * 1e-3) / 1e9, 1) if agg_ms[k] > 0 else None}
@@ -633,6 +653,7 @@ def main() -> int:
         "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,
         "roofline": roofline,
         "step_algorithmic_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "step_traffic": step_traffic,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
