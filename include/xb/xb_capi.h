@@ -280,8 +280,9 @@ int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, u
  * first 64*words_per_tile of them) on the device and is scored in
  * milliseconds per source (best of two runs); a value replaces the current
  * one only if it is at least 2% faster.  The trials write into two scratch
- * tables of min(S, 1024) x n int32 that are freed on return; the sample's
- * results are discarded.  *best receives the tuned options (mode
+ * tables of min(S, 1024) x n int32 (fewer rows if those would exceed half
+ * the free device memory) that are freed on return; the sample's results
+ * are discarded.  *best receives the tuned options (mode
  * XB_MODE_BATCHED). */
 typedef struct xb_tune_trial {
   int32_t words_per_tile, alpha, pull_cut_small, pull_cut_medium, pull_cut_hub, hub_chunk;

@@ -2236,7 +2236,17 @@ extern "C" int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, co
     cur.flags = XB_OUT_DEVICE;
     if (report) std::memset(report, 0, sizeof(*report));
     TuneRun tr{g, sample, S, nullptr, nullptr, report};
-    const int32_t rows = std::min(S, 64 * kMaxTileWords);  // one tile of the widest trial
+    // One tile of the widest trial, unless its two scratch tables (8 bytes
+    // per row and vertex) would take more than half the free device memory:
+    // then fewer rows, in whole 64-source words, and wider trials run with
+    // the rows there are (tile_words narrows them, so they cannot win).
+    int32_t rows = std::min(S, 64 * kMaxTileWords);
+    {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      while (rows > 64 && 8.0 * rows * (double)g->n > 0.5 * (double)fr) rows = std::max(64, rows / 2 / 64 * 64);
+    }
+    tr.S = rows;
     struct Free {
       TuneRun& t;
       ~Free() {
