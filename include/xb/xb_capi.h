@@ -68,8 +68,8 @@ typedef struct xb_graph xb_graph;
  * vertex (one per source word), deg <= pull_cut_medium 4T lanes, deg <=
  * pull_cut_hub a whole wave, larger ones are split into hub_chunk-edge pieces
  * with one wave each.  Vertices above pull_cut_medium are "heavy": the pull
- * records their parents as it finds them, and the emit pass resolves only
- * what push levels discovered. */
+ * and the push record their parents as they find them (lowest original id),
+ * and the emit copies them out instead of rescanning long in-lists. */
 typedef struct xb_msbfs_opts {
   int32_t mode;            /* XB_MODE_*                                        */
   uint32_t flags;          /* XB_OUT_DEVICE | XB_PROFILE | ...                 */
