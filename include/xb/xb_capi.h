@@ -60,6 +60,8 @@ extern "C" {
 #define XB_OUT_DEVICE 1u  /* levels/parents are device pointers (else host)  */
 #define XB_PROFILE 2u     /* time every kernel class with HIP events         */
 #define XB_NO_LEVELS 4u   /* levels == NULL allowed: only parents written     */
+#define XB_FIXED_TEAMS 8u /* xb_batch_*: keep the planned team sizes (no
+                             re-split of the grid from each run's clocks)    */
 
 typedef struct xb_graph xb_graph;
 

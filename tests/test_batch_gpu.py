@@ -53,12 +53,15 @@ def test_batch_mixed_vs_oracle(mixed, T):
     assert st.kernel_launches[11] == 1
 
 
-def test_batch_replay_and_single_graph_agree(mixed):
+@pytest.mark.parametrize("fixed_teams", [False, True])
+def test_batch_replay_and_single_graph_agree(mixed, fixed_teams):
+    """Replays of one batch give the oracle's tables every time, whether the
+    grid is re-split between the teams after each run (the default) or not."""
     gs, hgs, srcs, refs = mixed
     lv = [xb.DeviceBuffer((s.size, g.nnodes), fill=7) for s, g in zip(srcs, hgs)]
     pa = [xb.DeviceBuffer((s.size, g.nnodes), fill=7) for s, g in zip(srcs, hgs)]
-    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa], words_per_tile=4)
-    for rep in range(3):
+    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa], words_per_tile=4, fixed_teams=fixed_teams)
+    for rep in range(5):
         for t in lv + pa:
             t.upload(np.full(t.count, 7, dtype=np.int32))
         st = b.run()
@@ -109,4 +112,10 @@ def test_uneven_batch_full_size():
         got_l = np.concatenate([lv[i].rows(r, r + 1) for r in rows])
         got_p = np.concatenate([pa[i].rows(r, r + 1) for r in rows])
         assert np.array_equal(got_l, lr) and np.array_equal(got_p, pr), f"graph {i}: oracle rows"
+    # Later replays run on grid splits re-balanced from the team clocks: the
+    # tables stay the same.
+    sums = [(t.checksum(), u.checksum()) for t, u in zip(lv, pa)]
+    for _ in range(3):
+        b.run()
+    assert sums == [(t.checksum(), u.checksum()) for t, u in zip(lv, pa)]
     b.close()

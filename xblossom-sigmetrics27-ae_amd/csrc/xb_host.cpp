@@ -1384,6 +1384,12 @@ int32_t xb_last_error(char* buf, size_t len) {
 // ===========================================================================
 // Batched multi-graph runs: one persistent, hipGraph-captured launch
 // ===========================================================================
+// Fewest workgroups a re-split of a batch's grid leaves a team (the first
+// plan gives each team at least 8, before anything is measured).
+#ifndef XB_REBALANCE_FLOOR
+#define XB_REBALANCE_FLOOR 4
+#endif
+
 // The persistent kernel instantiates its tile loop for 1, 2, 4 and 8 words;
 // a batch asked for 16-word tiles runs 512-source tiles.
 constexpr int kBatchMaxWords = 8;
@@ -1403,6 +1409,8 @@ struct xb_batch {
   std::vector<xb::TileJob> h_jobs;
   std::vector<xb::Team> h_teams;
   std::vector<int> team_words;  // words per vertex the team's workspace holds
+  std::vector<int> team_floor, team_cap;  // workgroups a team keeps at least / can use at most
+  bool rebalance = true;                  // re-split the grid from each run's team clocks
   std::vector<std::vector<int32_t>> job_sources;  // host copy of each tile's sources (plain re-runs)
   int32_t n_teams = 0;
   int grid = 0;
@@ -1463,6 +1471,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   b->cut_hub = bo.cut_hub;
   b->hub_chunk = bo.hub_chunk;
   b->profile = (o.flags & XB_PROFILE) != 0;
+  b->rebalance = (o.flags & XB_FIXED_TEAMS) == 0;
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   const int T = b->T;
   const int32_t tile = 64 * T;
@@ -1579,6 +1588,8 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   for (int t = 0; t < nt; ++t) {
     blocks[t] = std::min(useful(t), std::max(1, std::min(8, budget / nt)));
     used += blocks[t];
+    b->team_floor.push_back(std::min(useful(t), XB_REBALANCE_FLOOR));
+    b->team_cap.push_back(useful(t));
   }
   for (int round = 0; round < 4 && used < budget; ++round) {
     double open_cost = 0;
@@ -1686,6 +1697,80 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   return b.release();
 }
 
+// Re-splits the grid between the teams for the next replay.  A team's
+// measured work is its span (the sum of its phase clocks) times its
+// workgroups; the target gives every team workgroups in proportion to its
+// work, within its floor and cap, and the new split goes half-way from the
+// current one towards it (team spans are not linear in the workgroup count,
+// so a few replays converge instead of one jumping).  The grid total stays
+// the captured launch's; results do not depend on the split.
+void rebalance_teams(xb_batch* b, const std::vector<xb::TeamCtl>& ctl) {
+  const int nt = b->n_teams;
+  if (nt < 2) return;
+  std::vector<double> w((size_t)nt);
+  double W = 0;
+  for (int t = 0; t < nt; ++t) {
+    double ticks = 0;
+    for (int c = 0; c < XB_K_CLASSES; ++c) ticks += (double)ctl[t].tphase[c];
+    w[t] = ticks * b->h_teams[t].nblocks;
+    W += w[t];
+  }
+  if (W <= 0) return;
+  const int total = b->grid;
+  auto lo = [&](int t) { return b->team_floor[t]; };
+  auto hi = [&](int t) { return b->team_cap[t]; };
+  // Water-filling of the target.
+  std::vector<int> tgt((size_t)nt);
+  int used = 0;
+  for (int t = 0; t < nt; ++t) used += (tgt[t] = lo(t));
+  for (int round = 0; round < 8 && used < total; ++round) {
+    double open = 0;
+    for (int t = 0; t < nt; ++t)
+      if (tgt[t] < hi(t)) open += w[t];
+    if (open <= 0) break;
+    const int spare = total - used;
+    int added = 0;
+    for (int t = 0; t < nt; ++t) {
+      if (tgt[t] >= hi(t)) continue;
+      const int add = std::min(hi(t) - tgt[t], (int)(spare * w[t] / open));
+      tgt[t] += add;
+      added += add;
+    }
+    used += added;
+    if (added == 0) break;
+  }
+  // Half-way, then exact total: extra workgroups go to the highest work per
+  // workgroup, surplus ones leave the lowest.
+  std::vector<int> nb((size_t)nt);
+  int sum = 0;
+  for (int t = 0; t < nt; ++t) {
+    nb[t] = std::min(hi(t), std::max(lo(t), (b->h_teams[t].nblocks + tgt[t] + 1) / 2));
+    sum += nb[t];
+  }
+  while (sum != total) {
+    int pick = -1;
+    double best = 0;
+    for (int t = 0; t < nt; ++t) {
+      const double load = w[t] / nb[t];
+      if (sum < total && nb[t] < hi(t) && (pick < 0 || load > best)) pick = t, best = load;
+      if (sum > total && nb[t] > lo(t) && (pick < 0 || load < best)) pick = t, best = load;
+    }
+    if (pick < 0) return;  // no admissible move: keep the current split
+    nb[pick] += sum < total ? 1 : -1;
+    sum += sum < total ? 1 : -1;
+  }
+  bool same = true;
+  for (int t = 0; t < nt; ++t) same = same && nb[t] == b->h_teams[t].nblocks;
+  if (same) return;
+  int first = 0;
+  for (int t = 0; t < nt; ++t) {
+    b->h_teams[t].first_block = first;
+    b->h_teams[t].nblocks = nb[t];
+    first += nb[t];
+  }
+  h2d(b->d_teams, b->h_teams.data(), sizeof(xb::Team) * nt, b->stream);
+}
+
 void run_batch(xb_batch* b, xb_run_stats* st) {
   for (xb_graph* g : b->graphs)
     if (g->cut_small != b->cut_small || g->cut_medium != b->cut_medium || g->cut_hub != b->cut_hub ||
@@ -1731,11 +1816,15 @@ void run_batch(xb_batch* b, xb_run_stats* st) {
     const double L = j.levels + 1.0, W = (j.cnt + 63) / 64, n = g->n, m = (double)g->m;
     bytes += L * (4.0 * (n + 1) + 4.0 * m) + L * 24.0 * W * n + (j.lvl_out ? 8.0 : 4.0) * j.cnt * n;
   }
+  std::vector<xb::TeamCtl> ctl;
+  if ((st && b->profile) || b->rebalance) {
+    ctl.resize((size_t)b->n_teams);
+    d2h(ctl.data(), b->d_ctl, sizeof(xb::TeamCtl) * ctl.size(), b->stream);
+  }
   if (st && b->profile) {
     // Phase clocks: each team's critical-path time per phase, weighted by the
-    // team's share of the grid, so the classes add up to about the wall time.
-    std::vector<xb::TeamCtl> ctl((size_t)b->n_teams);
-    d2h(ctl.data(), b->d_ctl, sizeof(xb::TeamCtl) * ctl.size(), b->stream);
+    // team's share of the grid, so the classes add up to about the wall time
+    // (with the split this run used).
     int dev = 0, khz = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -1746,6 +1835,7 @@ void run_batch(xb_batch* b, xb_run_stats* st) {
           if (c != XB_K_PERSISTENT) st->kernel_ms[c] += (double)ctl[t].tphase[c] / khz * share;
       }
   }
+  if (b->rebalance) rebalance_teams(b, ctl);
   if (st) {
     st->ms_total = ms;
     st->kernel_ms[XB_K_PERSISTENT] = ms;
