@@ -185,7 +185,10 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
  * each) run on that team with the level loop on the device, and the whole
  * batch is one persistent kernel launch captured in a hipGraph.
  * xb_batch_run replays the graph; tiles deeper than the 8-bit level code are
- * re-run on the plain path afterwards.  The graphs must stay alive, and their
+ * re-run on the plain path afterwards.  After each run the workgroups are
+ * re-split between the teams from their measured spans (unless the options
+ * carry XB_FIXED_TEAMS), so the first few replays get faster; the split never
+ * changes results.  The graphs must stay alive, and their
  * tuning (bucket cuts) unchanged, while the batch exists.  Results are those
  * of xb_msbfs_run on each graph. */
 typedef struct xb_batch xb_batch;
