@@ -36,7 +36,6 @@ KERNEL_SYMBOL = {
     "pull": "k_bp_pull_bucket",
     "pull_hub": "k_bp_pull_hub",
     "push": "k_bp_push",
-    "emit_heavy": "k_bp_emit_heavy",
     "plain_push": "k_plain_push",
 }
 

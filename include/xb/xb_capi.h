@@ -93,7 +93,7 @@ typedef struct xb_msbfs_opts {
 #define XB_K_PULL 4       /* bit-parallel bottom-up pull, degree buckets          */
 #define XB_K_PULL_HUB 5   /* bottom-up pull, hub adjacency chunks               */
 #define XB_K_CLEAR 6      /* frontier clear                                     */
-#define XB_K_EMIT_HEAVY 7 /* parent resolution for heavy vertices               */
+#define XB_K_EMIT_HEAVY 7 /* reserved: heavy parents are recorded by pull/push */
 #define XB_K_EMIT 8       /* level/parent table write-out                       */
 #define XB_K_PLAIN_INIT 9 /* plain engine: row initialisation                   */
 #define XB_K_PLAIN_PUSH 10 /* plain engine: top-down push level                 */

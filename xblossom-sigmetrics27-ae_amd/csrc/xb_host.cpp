@@ -216,8 +216,7 @@ struct xb_graph {
   int32_t n_hub_chunks = 0;
   int32_t n_hubs = 0;
   int64_t hub_edges = 0;
-  int32_t* d_heavy = nullptr;  // engine ids with in-degree > cut_medium, ascending (the emit heavy list)
-  int32_t* d_hidx = nullptr;   // engine id -> index in d_heavy, else -1
+  int32_t* d_hidx = nullptr;   // engine id -> row of a heavy vertex (in-degree > cut_medium) in hpar, else -1
   int32_t n_heavy = 0;
   Workspace ws;
   // Forest-mode state (about 90 B per vertex), kept from the first matching
@@ -242,7 +241,6 @@ struct xb_graph {
     dfree(d_col);
     dfree(d_perm);
     dfree(d_hub_chunks);
-    dfree(d_heavy);
     dfree(d_hidx);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -648,23 +646,17 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
     g->n_hub_chunks = (int32_t)chunks.size();
     g->n_hubs = hubs;
     g->hub_edges = hub_edges;
-    // Heavy vertices (bucket 2 and hubs): their parents are recorded by the pull.
-    std::vector<int32_t> hv, hidx((size_t)n, -1);
+    // Heavy vertices (bucket 2 and hubs): the pull and the push record their
+    // parents in the heavy-parent table, row hidx[v].
+    std::vector<int32_t> hidx((size_t)n, -1);
+    int32_t n_heavy = 0;
     for (int32_t v = 0; v < n; ++v)
-      if (io[v + 1] - io[v] > o.cut_medium) {
-        hidx[v] = (int32_t)hv.size();
-        hv.push_back(v);
-      }
-    dfree(g->d_heavy);
+      if (io[v + 1] - io[v] > o.cut_medium) hidx[v] = n_heavy++;
     dfree(g->d_hidx);
-    g->d_heavy = nullptr;
     g->d_hidx = nullptr;
-    g->d_heavy = dalloc<int32_t>(hv.size());
     g->d_hidx = dalloc<int32_t>((size_t)n);
-    if (!hv.empty())
-      h2d(g->d_heavy, hv.data(), sizeof(int32_t) * hv.size(), g->stream);
     h2d(g->d_hidx, hidx.data(), sizeof(int32_t) * (size_t)n, g->stream);
-    g->n_heavy = (int32_t)hv.size();
+    g->n_heavy = n_heavy;
     g->cut_small = o.cut_small;
     g->cut_medium = o.cut_medium;
     g->cut_hub = o.cut_hub;
@@ -1001,12 +993,8 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   int K = 0;
   while ((1 << K) <= d + 1) ++K;
   const int words = (cnt + 63) / 64;
-  if (g->n_heavy > 0)
-    for (int t = 0; t < words; ++t) {
-      prof.begin(XB_K_EMIT_HEAVY);
-      xb::launch_bp_emit_heavy(w.st, T, s, t, K, g->d_ein_off, g->d_ein_col, g->d_heavy, g->n_heavy);
-      prof.end();
-    }
+  // Heavy vertices need no pass of their own: the pull and the push recorded
+  // the parent of every bit they discovered in hpar, and the emit copies it.
   // Light vertices, one launch per word group (up to 4 words, 256 rows).
   const int W = xb::emit_group_words(T);
   for (int grp = 0; grp * W < words; ++grp) {
@@ -1501,7 +1489,6 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     }
     v.n_hub_chunks = g->n_hub_chunks;
     v.hubs = g->d_hub_chunks;
-    v.heavy = g->d_heavy;
     v.hidx = g->d_hidx;
     v.n_heavy = g->n_heavy;
     v.n = g->n;

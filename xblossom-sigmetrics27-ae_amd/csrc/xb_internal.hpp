@@ -81,7 +81,6 @@ struct GraphView {
   int32_t bucket_shift[3];
   int32_t n_hub_chunks;
   const HubChunk* hubs;
-  const int32_t* heavy;
   const int32_t* hidx;
   int32_t n_heavy;
   int32_t n;
@@ -179,8 +178,6 @@ void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, 
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
 // Heavy vertices: resolve the parents the pull did not record (push levels).
-void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
-                          const int32_t* in_col, const int32_t* heavy, int32_t n_heavy);
 // src: the tile's source ids (level-1 parents are the sources themselves).
 // Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W = min(T, 4)).
 int emit_group_words(int T);

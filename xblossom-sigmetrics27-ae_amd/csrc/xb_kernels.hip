@@ -112,18 +112,6 @@ __device__ __forceinline__ uint64_t quad_perm64(uint64_t x) {
   return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
 }
 
-__device__ __forceinline__ uint64_t wave_excl_or(uint64_t x) {
-  const int l = lane_id();
-  uint64_t incl = x;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    uint64_t y = __shfl_up(incl, d, kWave);
-    if (l >= d) incl |= y;
-  }
-  uint64_t ex = __shfl_up(incl, 1, kWave);
-  return l == 0 ? 0ull : ex;
-}
-
 // Wave-aggregated append of v to a plain list with a 32-bit counter.
 __device__ __forceinline__ void wave_push_list(bool pred, int32_t v, int32_t* list, uint32_t* ctr) {
   const uint64_t mask = __ballot(pred);
@@ -778,52 +766,6 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
   return m;
 }
 
-// Heavy vertices (in-degree above the medium pull cut): the pull and the push
-// record the parent of every bit they discover (lowest original id by
-// atomicMin); a bit still unrecorded is resolved here, one wave per vertex,
-// lane i testing in-neighbour j0+i, the lowest lane offering a bit (exclusive
-// prefix-OR) owning that bit's parent.
-template <int T, int K>
-__device__ void emit_heavy_phase(Grid g, BPState s, int32_t word, const int32_t* __restrict__ in_off,
-                                 const int32_t* __restrict__ in_col, const int32_t* __restrict__ heavy,
-                                 int32_t n_heavy) {
-  const int l = lane_id();
-  for (int64_t h = gwave(g); h < n_heavy; h += gnwaves(g)) {
-    const int32_t v = heavy[h];
-    uint64_t pk[K];
-    load_planes<T, K>(s, v, word, pk);
-    const CodeMasks<K> cm = code_masks<K>(pk);
-    uint32_t* hp = s.hpar + ((int64_t)h * T + word) * 64;
-    const uint64_t unset = __ballot(hp[l] == 0xFFFFFFFFu);
-    uint64_t need = cm.reached & ~cm.lvl0 & ~cm.lvl1 & unset;
-    const int32_t b = in_off[v], e = in_off[v + 1];
-    for (int32_t j0 = b; j0 < e && need; j0 += kWave) {
-      const int32_t j = j0 + l;
-      uint64_t eq = 0;
-      if (j < e) {
-        const int32_t u = in_col[j];
-        uint64_t pu[K];
-        load_planes<T, K>(s, u, word, pu);
-        eq = need;
-#pragma unroll
-        for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
-      }
-      const uint64_t ex = wave_excl_or(eq);
-      uint64_t first = eq & ~ex;
-      if (first) {
-        const uint32_t uo = (uint32_t)s.in_col_o[j];
-        while (first) {
-          const int bit = __builtin_ctzll(first);
-          first &= first - 1;
-          hp[bit] = uo;
-        }
-      }
-      const uint64_t all = __shfl(ex | eq, kWave - 1, kWave);
-      need &= ~all;
-    }
-  }
-}
-
 // Light vertices, one word group (W = min(T, 4) words, 64*W rows) per pass.
 // A block takes VG = 64/W consecutive vertices (output order); lane (vertex,
 // partition p, word w) takes every 4th in-neighbour of the vertex (kEmitU per
@@ -1100,14 +1042,6 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
   clear_phase<T>(self_grid(), s, cur, qv, qn);
 }
 
-template <int T, int K>
-__global__ __launch_bounds__(kBlock) void k_bp_emit_heavy(BPState s, int32_t word,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ heavy, int32_t n_heavy) {
-  emit_heavy_phase<T, K>(self_grid(), s, word, in_off, in_col, heavy, n_heavy);
-}
-
 // Minimum workgroups per CU the standalone emit's register budget is sized
 // for: 8 (64 VGPRs) hides more of its dependent gathers than the 6 waves per
 // SIMD its natural 76 VGPRs allow (C3: 27.5 -> 24.8 ms of emit per step).
@@ -1325,11 +1259,6 @@ __device__ __noinline__ void nl_fill_u32(Grid g, uint32_t* p, int64_t count, uin
   fill_u32(uni(g), uni(p), uni(count), uni(val));
 }
 template <int T, int K>
-__device__ __noinline__ void nl_emit_heavy(Grid g, BPState s, int32_t word, const int32_t* in_off,
-                                           const int32_t* in_col, const int32_t* heavy, int32_t n_heavy) {
-  emit_heavy_phase<T, K>(uni(g), uni(s), uni(word), uni(in_off), uni(in_col), uni(heavy), uni(n_heavy));
-}
-template <int T, int K>
 __device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
                                            int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out,
@@ -1342,12 +1271,10 @@ template <int T, int K>
 __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const TileJob& job, const TeamSync& ts,
                                unsigned* gerr) {
   const int words = (job.cnt + 63) / 64;
-  if (gv.n_heavy > 0)
-    for (int t = 0; t < words; ++t) nl_emit_heavy<T, K>(g, s, t, gv.in_off, gv.in_col, gv.heavy, gv.n_heavy);
   // The light emit of every word group shares one counter, reset before the
   // barrier that precedes it (the reset is visible to the team after it).
   if (g.bid == 0 && threadIdx.x == 0) atomicExch(&ts.ctl->emit_chunk, 0u);
-  if (!team_barrier(ts, g, gerr, XB_K_EMIT_HEAVY)) return false;
+  if (!team_barrier(ts, g, gerr, XB_K_EMIT)) return false;
   const int32_t ngrp = (words + kGroupWords<T> - 1) / kGroupWords<T>;
   nl_emit_light<T, K>(g, s, 0, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.oin_colo, gv.n, gv.cut_medium,
                       job.lvl_out, job.par_out, &ts.ctl->emit_chunk, ngrp);
@@ -2323,24 +2250,11 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 }
 
 template <int T, int K>
-static void emit_heavy_launch(hipStream_t st, BPState s, int32_t word, const int32_t* in_off, const int32_t* in_col,
-                              const int32_t* heavy, int32_t n_heavy) {
-  hipLaunchKernelGGL((k_bp_emit_heavy<T, K>), dim3(grid_for((int64_t)n_heavy * kWave, kBlock)), dim3(kBlock), 0, st,
-                     s, word, in_off, in_col, heavy, n_heavy);
-}
-
-template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                               int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
   hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64 / kGroupWords<T>)), dim3(kBlock), 0, st, s, word,
                      rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
-}
-
-void launch_bp_emit_heavy(hipStream_t st, int T, BPState s, int32_t word, int32_t K, const int32_t* in_off,
-                          const int32_t* in_col, const int32_t* heavy, int32_t n_heavy) {
-  if (n_heavy <= 0) return;
-  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_heavy_launch<TT, KK>(st, s, word, in_off, in_col, heavy, n_heavy))));
 }
 
 int emit_group_words(int T) { return T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX; }
