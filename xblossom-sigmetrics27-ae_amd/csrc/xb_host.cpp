@@ -1575,7 +1575,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   for (int t = 0; t < nt; ++t) {
     blocks[t] = std::min(useful(t), std::max(1, std::min(8, budget / nt)));
     used += blocks[t];
-    b->team_floor.push_back(std::min(useful(t), XB_REBALANCE_FLOOR));
+    b->team_floor.push_back(std::min(blocks[t], XB_REBALANCE_FLOOR));
     b->team_cap.push_back(useful(t));
   }
   for (int round = 0; round < 4 && used < budget; ++round) {
