@@ -168,6 +168,8 @@ def bench_uneven(args) -> int:  # noqa: C901
         log(f"plain: {t_plain * 1e3:.1f} ms")
     st = batch.last_stats
     achieved = st.kernel_bytes[11] / (st.kernel_ms[11] * 1e-3) / 1e9
+    avg_launch_ms = float(np.mean(ms_graph))
+    traffic = traffic_from_profiles("uneven", "k_bp_persistent")
     line = {
         "metric": METRIC,
         "value": round(gteps, 3),
@@ -192,8 +194,12 @@ def bench_uneven(args) -> int:  # noqa: C901
         "bit_exact_sums_vs_host_driven_and_plain": exact,
         "roofline": {"bound": "hbm", "kernel": "k_bp_persistent", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes_per_launch": st.kernel_bytes[11],
-                     "avg_launch_ms": round(float(np.mean(ms_graph)), 3), "launches": args.steps},
+                     "traffic": traffic,
+                     "traffic_GBps": round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": (round(traffic / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                      if traffic else None),
+                     "algorithmic_bytes_per_launch": st.kernel_bytes[11],
+                     "avg_launch_ms": round(avg_launch_ms, 3), "launches": args.steps},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
