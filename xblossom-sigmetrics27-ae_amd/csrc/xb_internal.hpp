@@ -163,7 +163,7 @@ void launch_plain_set_source(hipStream_t st, int32_t* level, int32_t* parent, in
 void launch_plain_push(hipStream_t st, const int32_t* off, const int32_t* col, Queue cur, int32_t qn,
                        int32_t E, int32_t* level, int32_t* parent, int32_t d, Queue nxt);
 
-// ---- batched bit-parallel engine (T = words per tile: 1, 2, 4 or 8) ----
+// ---- batched bit-parallel engine (T = words per tile: 1, 2, 4, 8 or 16) ----
 void launch_bp_seed(hipStream_t st, int T, BPState s, const int32_t* src, int32_t cnt);
 void launch_bp_push(hipStream_t st, int T, BPState s, int cur, const int32_t* off, const int32_t* col,
                     Queue q, int32_t qn, int32_t E);
@@ -177,7 +177,6 @@ void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, i
 void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
-// Heavy vertices: resolve the parents the pull did not record (push levels).
 // src: the tile's source ids (level-1 parents are the sources themselves).
 // Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W = min(T, 4)).
 int emit_group_words(int T);
