@@ -242,6 +242,11 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
   uint64_t* cand = xcalloc((size_t)cand_cap, sizeof(uint64_t));
   uint64_t stamp_base = 0;
   Forest F = {off, col, lvl, par, fp, elab, flag};
+  /* Once a phase's level 0 has run to a round with no pair, no two exposed
+   * vertices are adjacent, and that stays so: an augmentation matches both of
+   * its exposed ends and exposes nothing.  Later phases skip the level-0
+   * rounds, which could only scan the roots' lists and find nothing. */
+  int roots_independent = 0;
   for (;;) {
     if (max_phases > 0 && s.phases == max_phases) break;
     ++s.phases;
@@ -270,6 +275,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
       int64_t fe = frontier_edges(off, front, fn);
       int64_t level_same_tree = 0;
       for (;;) {
+        if (level == 0 && roots_independent) break;
         if (level == 0) {
           /* Every tree is one root: its lowest neighbouring tree is its first
            * adjacency entry that is an untaken root, found by a cursor that
@@ -355,6 +361,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         nroots = k;
         fe = frontier_edges(off, front, fn);
       }
+      if (level == 0) roots_independent = 1;
       if (found) break;
       /* Blossom candidates: the same-tree even-even frontier edges, keyed
        * (root << 32 | edge position), taken before the claims. */

@@ -2041,6 +2041,11 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 16, st));
   HIPCHK(hipMemsetAsync(ms.lab, 0xFF, sizeof(unsigned long long) * (size_t)n, st));
   const int32_t max_steps = 4 * n + 16;
+  // Once a phase's level 0 has run to a round with no pair, no two exposed
+  // vertices are adjacent, and that stays so: an augmentation matches both of
+  // its exposed ends and exposes nothing.  Later phases skip the level-0
+  // rounds, which could only scan the roots' lists and find nothing.
+  bool roots_independent = false;
   for (;;) {
     ++s.phases;
     HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 6, st));
@@ -2062,6 +2067,7 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
     while (qn > 0) {
       int64_t level_same_tree = 0;
       for (;;) {
+        if (level == 0 && roots_independent) break;
         // Level 0: the frontier is the root list, every tree one root.
         if (level == 0) {
           prof.begin(XB_K_MT_SCAN);
@@ -2122,6 +2128,7 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         E = (int32_t)(uint32_t)(w.hctr[0] >> 32);
         nroots = (int32_t)(w.hctr[1] & 0xFFFFFFFFull);
       }
+      if (level == 0) roots_independent = true;
       if (found) break;
       // Blossom candidates are the same-tree even-even edges the scan met;
       // take them before the claims grow the forest.
