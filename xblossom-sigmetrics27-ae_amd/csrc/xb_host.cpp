@@ -2087,7 +2087,9 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
           prof.add(XB_K_MT_SCAN, kScanEdge * E + kFrontVertex * qn);
           s.edges_scanned += E;
         }
-        HIPCHK(hipMemsetAsync(ms.npairs(), 0, sizeof(uint32_t), st));
+        // The round's counters: pairs, flip-stack pool, kept roots and the
+        // next frontier queue (free since the last round's compaction).
+        xb::launch_mt_zero_ctrs(st, ms.npairs(), ms.pool_ctr(), ms.kept(), w.q[cur ^ 1].ctr);
         prof.begin(XB_K_MT_TREE);
         xb::launch_mt_accept(st, ms.roots, nroots, ms.best, ms.acc, ms.pairs, ms.npairs(),
                              level == 0 ? ms.bridge : nullptr);
@@ -2106,12 +2108,9 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
           s.edges_scanned += E;
         }
         prof.begin(XB_K_MT_TREE);
-        HIPCHK(hipMemsetAsync(ms.pool_ctr(), 0, sizeof(uint32_t), st));
         xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.elab, ms.mate, ms.used,
                            xb::FlipStack{ms.pool, ms.pool_ctr(), ms.pool_cap}, max_steps, ms.err());
         // The next round sees only the frontier and roots of trees not taken.
-        HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
-        HIPCHK(hipMemsetAsync(ms.kept(), 0, sizeof(uint32_t), st));
         xb::launch_mt_compact(st, w.q[cur].v, qn, ms.root, ms.used, g->d_off, w.q[cur ^ 1], nullptr, nullptr);
         xb::launch_mt_compact(st, ms.roots, nroots, ms.root, ms.used, g->d_off, xb::Queue{}, ms.roots2,
                               ms.kept());

@@ -228,6 +228,8 @@ size_t sort_u64_temp_bytes(int32_t n);
 void sort_u64(void* temp, size_t bytes, const unsigned long long* in, unsigned long long* out, int32_t n,
               hipStream_t st);
 void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best);
+// One launch zeroing a round's four counters.
+void launch_mt_zero_ctrs(hipStream_t st, uint32_t* a, uint32_t* b, uint32_t* c, unsigned long long* d);
 // Keeps the entries of `in` whose tree is not taken: into the queue `out`
 // (list == nullptr) or into list/ctr32.
 void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,

@@ -1644,6 +1644,17 @@ __global__ __launch_bounds__(kBlock) void k_mt_compact(const int32_t* __restrict
 }
 
 // Clears the trees' lowest-neighbour records before the next round.
+// Zeroes a round's counters in one launch (pairs, flip-stack pool, kept
+// roots, the next frontier queue's packed counter).
+__global__ void k_mt_zero_ctrs(uint32_t* a, uint32_t* b, uint32_t* c, unsigned long long* d) {
+  if (threadIdx.x == 0) {
+    *a = 0u;
+    *b = 0u;
+    *c = 0u;
+    *d = 0ull;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_mt_reset_best(const int32_t* __restrict__ roots, int32_t nroots,
                                                           int32_t* __restrict__ best) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nroots; i += (int64_t)gridDim.x * kBlock)
@@ -2356,6 +2367,9 @@ void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int
   if (cnt <= 0) return;
   hipLaunchKernelGGL(k_mt_compact, dim3(item_grid(cnt)), dim3(kBlock), 0, st, in, cnt, root, used, off, out, list,
                      ctr32);
+}
+void launch_mt_zero_ctrs(hipStream_t st, uint32_t* a, uint32_t* b, uint32_t* c, unsigned long long* d) {
+  hipLaunchKernelGGL(k_mt_zero_ctrs, dim3(1), dim3(64), 0, st, a, b, c, d);
 }
 void launch_mt_reset_best(hipStream_t st, const int32_t* roots, int32_t nroots, int32_t* best) {
   if (nroots <= 0) return;
