@@ -3,11 +3,13 @@
 Bar: bit-exact levels and parents (integer work).  Cases cover both modes,
 every words-per-tile value, forced push-only / pull-only direction choices,
 heavy-vertex kernels (tiny cuts), directed graphs, partial tiles, duplicate
-and isolated sources, and depth beyond the 8-bit level code (plain fallback).
+and isolated sources, raw input with self-loops and repeated entries, zero
+sources, and depth beyond the 8-bit level code (plain fallback).
 """
 import numpy as np
 import pytest
 
+import graphs
 import oracle
 import xbmsbfs as xb
 
@@ -250,6 +252,38 @@ def test_unsorted_adjacency_is_sorted_on_ingest(rmat12):
         ci[a:b] = ci[a:b][::-1]
     hg = xb.HGraph(g.row_offsets, ci)
     _check(g, hg, xb.sample_sources(g, 100, seed=2), mode=xb.XB_MODE_BATCHED)
+
+
+@pytest.mark.parametrize("T", [1, 4, 16])
+def test_self_loops_and_repeated_entries(rmat12, T):
+    """Raw CSR input may hold self-loops and repeated entries: HGraph takes
+    the files as they are (apps/xb/src/include/graph/hgraph_gpu.h:37-48).
+    Neither changes a level or a lowest-id parent: the tables equal the
+    oracle's on the raw input and the oracle's on the clean graph.  Tiny
+    bucket cuts send the repeated entries through the hub-piece pull too."""
+    g, _ = rmat12
+    ro, ci = graphs.with_loops_and_duplicates(g.row_offsets, g.col_indices, seed=9)
+    assert ci.size > g.m
+    raw = xb.Csr(ro, ci)
+    hg = xb.HGraph(ro, ci)
+    src = xb.sample_sources(g, 130, seed=6)
+    lv_clean, pa_clean, _ = oracle.msbfs(g.row_offsets, g.col_indices, src, threads=8)
+    for kw in ({}, dict(pull_cut_small=1, pull_cut_medium=3, pull_cut_hub=6, hub_chunk=5)):
+        _check(raw, hg, src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, **kw)
+        lv, pa = xb.XBlossomGpuEngine10(hg).MultiSourceBFS(src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, **kw)
+        assert np.array_equal(lv, lv_clean) and np.array_equal(pa, pa_clean)
+    _check(raw, hg, src[:20], mode=xb.XB_MODE_PLAIN)
+
+
+def test_zero_sources(rmat12):
+    """S = 0 is a valid call: nothing is written and the stats read zero."""
+    _, hg = rmat12
+    eng = xb.XBlossomGpuEngine10(hg)
+    eng.MultiSourceBFS(xb.sample_sources(rmat12[0], 10, seed=1), mode=xb.XB_MODE_BATCHED)
+    assert eng.last_stats.ms_total > 0
+    lv, pa = eng.MultiSourceBFS(np.zeros(0, dtype=np.int32), mode=xb.XB_MODE_BATCHED)
+    assert lv.shape == (0, hg.nnodes) and pa.shape == (0, hg.nnodes)
+    assert eng.last_stats.ms_total == 0 and eng.last_stats.tiles == 0
 
 
 def test_cpp_shim_call_site(tmp_path):

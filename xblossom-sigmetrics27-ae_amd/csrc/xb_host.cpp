@@ -1226,13 +1226,13 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
   return guarded([&] {
     if (!g) fail(XB_ERR_INVALID_ARG, "null graph");
     if (S < 0) fail(XB_ERR_INVALID_ARG, "S < 0");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
     if (S == 0) return;
     if (!sources || !parents) fail(XB_ERR_INVALID_ARG, "null sources or parents");
     const xb_msbfs_opts o = parse_opts(opts);
     if (!levels && !(o.flags & XB_NO_LEVELS)) fail(XB_ERR_INVALID_ARG, "null levels (set XB_NO_LEVELS)");
     check_sources(g, sources, S);
     const bool dev_out = (o.flags & XB_OUT_DEVICE) != 0;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
     Profiler prof;
     prof.on = (o.flags & XB_PROFILE) != 0;
     prof.st = g->stream;
