@@ -76,6 +76,26 @@ def test_batch_replay_and_single_graph_agree(mixed, fixed_teams):
         assert np.array_equal(l1, refs[i][0]) and np.array_equal(p1, refs[i][1])
 
 
+def test_batch_graph_without_sources(mixed):
+    """A graph of a batch may have no sources: it gets no tile and no team,
+    and the other graphs' tables stay the oracle's.  A batch with no source
+    at all is refused."""
+    gs, hgs, srcs, refs = mixed
+    srcs2 = list(srcs)
+    srcs2[2] = np.zeros(0, dtype=np.int32)
+    res, st = xb.run_batched(hgs, srcs2, words_per_tile=4)
+    assert st.kernel_launches[11] == 1
+    for i, ((lv, pa), (lr, pr)) in enumerate(zip(res, refs)):
+        if i == 2:
+            assert lv.shape == (0, hgs[2].nnodes) and pa.shape == (0, hgs[2].nnodes)
+            continue
+        assert np.array_equal(lv, lr), f"graph {i}: levels"
+        assert np.array_equal(pa, pr), f"graph {i}: parents"
+    with pytest.raises(xb.XbError) as ei:
+        xb.run_batched(hgs[:2], [np.zeros(0, dtype=np.int32)] * 2)
+    assert ei.value.code == xb.XB_ERR_INVALID_ARG
+
+
 def test_batch_rejects_bad_sources(mixed):
     gs, hgs, srcs, refs = mixed
     bad = [s.copy() for s in srcs]
