@@ -1561,38 +1561,51 @@ __global__ __launch_bounds__(kBlock) void k_mt_scan0(const int32_t* __restrict__
     const bool lng = v >= 0 && e - j > kShortWalk;
     if (v >= 0 && !lng) {
       const int32_t j0 = j;
+      int32_t hit = INT32_MAX;  // the entry found, kept rather than read again
       while (j < e) {
+        // Both flags are loaded at once (w < n always): one dependent round
+        // trip after col[j] instead of two.
         const int32_t w = col[j];
-        if (w != v && lvl[w] == 0 && !used[w]) break;
+        const int32_t lw = lvl[w];
+        const uint8_t uw = used[w];
+        if (w != v && lw == 0 && !uw) {
+          hit = w;
+          break;
+        }
         ++j;
       }
       cnt += (unsigned long long)(j - j0) + (j < e ? 1ull : 0ull);
       cursor[v] = j;
-      best[v] = j < e ? col[j] : INT32_MAX;
+      best[v] = hit;
     }
     uint64_t m = __ballot(lng);
     while (m) {
       const int src = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
       const int32_t vv = __shfl(v, src, kWave), j0 = __shfl(j, src, kWave), ee = __shfl(e, src, kWave);
-      int32_t found = ee;
+      int32_t found = ee, fw = INT32_MAX;
       for (int32_t k = j0; k < ee; k += kWave) {
         const int32_t idx = k + lane_id();
         bool ok = false;
+        int32_t w = 0;
         if (idx < ee) {
-          const int32_t w = col[idx];
-          ok = w != vv && lvl[w] == 0 && !used[w];
+          w = col[idx];
+          const int32_t lw = lvl[w];
+          const uint8_t uw = used[w];
+          ok = w != vv && lw == 0 && !uw;
         }
         const uint64_t hit = __ballot(ok);
         if (hit) {
-          found = k + __ffsll((unsigned long long)hit) - 1;
+          const int first = __ffsll((unsigned long long)hit) - 1;
+          found = k + first;
+          fw = __shfl(w, first, kWave);
           break;
         }
       }
       if (lane_id() == src) {
         cnt += (unsigned long long)(found - j0) + (found < ee ? 1ull : 0ull);
         cursor[vv] = found;
-        best[vv] = found < ee ? col[found] : INT32_MAX;
+        best[vv] = fw;
       }
     }
   }
