@@ -188,9 +188,12 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
  * re-run on the plain path afterwards.  After each run the workgroups are
  * re-split between the teams from their measured spans (unless the options
  * carry XB_FIXED_TEAMS), so the first few replays get faster; the split never
- * changes results.  The graphs must stay alive, and their
- * tuning (bucket cuts) unchanged, while the batch exists.  Results are those
- * of xb_msbfs_run on each graph. */
+ * changes results.  The graphs must stay alive while the batch exists.  A
+ * run or autotune on one of them with other bucket cut-offs rebuilds its
+ * layout arrays, which the captured launch reads: after any rebuild (even one
+ * that restores the original cut-offs) xb_batch_run returns
+ * XB_ERR_INVALID_ARG without launching; create a new batch.  Results are
+ * those of xb_msbfs_run on each graph. */
 typedef struct xb_batch xb_batch;
 
 int32_t xb_batch_create(xb_graph* const* graphs, int32_t G,
@@ -320,7 +323,9 @@ int32_t xb_autotune(xb_graph* g, const int32_t* sample, int32_t S, const xb_msbf
  * different slots overlap on the device.  A row's result does not depend on
  * the tile it ran in: every table equals the one xb_msbfs_run writes for
  * the same sources.  The server owns the graph's layout while it lives: do
- * not run other work on the same graph handle meanwhile. */
+ * not run other work on the same graph handle meanwhile (a rebuild of the
+ * layout by a run with other cut-offs makes every later tile fail with
+ * XB_ERR_INVALID_ARG). */
 typedef struct xb_server xb_server;
 
 #define XB_SERVE_COUNT_EDGES 1u /* count traversed edges per request (GTEPS) */

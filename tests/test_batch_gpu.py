@@ -139,3 +139,30 @@ def test_uneven_batch_full_size():
         b.run()
     assert sums == [(t.checksum(), u.checksum()) for t, u in zip(lv, pa)]
     b.close()
+
+
+def test_batch_refuses_run_after_layout_rebuild(mixed):
+    """A batch holds its graphs' pull-layout arrays.  A run with other
+    cut-offs rebuilds (frees and reallocates) them; a run with the original
+    cut-offs afterwards rebuilds them again.  Equal cut values then do not mean
+    the same arrays, so the batch refuses to run before launching anything,
+    and a new batch runs bit-exact."""
+    gs, hgs, srcs, refs = mixed
+    i = 1
+    lv = xb.DeviceBuffer((srcs[i].size, hgs[i].nnodes))
+    pa = xb.DeviceBuffer((srcs[i].size, hgs[i].nnodes))
+    b = xb.Batch([hgs[i]], [srcs[i]], [lv.ptr], [pa.ptr], words_per_tile=4)
+    b.run()
+    assert np.array_equal(pa.numpy(), refs[i][1])
+    eng = xb.XBlossomGpuEngine10(hgs[i])
+    eng.MultiSourceBFS(srcs[i][:5], mode=xb.XB_MODE_BATCHED, pull_cut_small=4, pull_cut_medium=16)
+    eng.MultiSourceBFS(srcs[i][:5], mode=xb.XB_MODE_BATCHED)  # default cut-offs again
+    with pytest.raises(xb.XbError) as ei:
+        b.run()
+    assert ei.value.code == xb.XB_ERR_INVALID_ARG
+    b.close()
+    b2 = xb.Batch([hgs[i]], [srcs[i]], [lv.ptr], [pa.ptr], words_per_tile=4)
+    lv.upload(np.full(lv.count, 7, dtype=np.int32))
+    b2.run()
+    assert np.array_equal(lv.numpy(), refs[i][0]) and np.array_equal(pa.numpy(), refs[i][1])
+    b2.close()

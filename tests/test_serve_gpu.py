@@ -172,3 +172,25 @@ def test_rejections(web):
     with pytest.raises(xb.XbError) as ei:
         xb.Server(hg, mode=xb.XB_MODE_PLAIN)
     assert ei.value.code == xb.XB_ERR_UNSUPPORTED
+
+
+def test_server_refuses_tiles_after_layout_rebuild(web):
+    """The server's slots read the graph's pull-layout arrays.  Re-tuning the
+    graph while the server lives (here while it is idle: other cut-offs, then
+    the original ones back) rebuilds them, so later tiles fail with
+    XB_ERR_INVALID_ARG instead of reading freed memory."""
+    g, hg = web
+    reqs = _requests(g, 2, seed=21, smax=8)
+    srv = xb.Server(hg, slots=1, budget_bytes=256 << 20)
+    rid = srv.submit(reqs[0])
+    assert srv.wait(rid, timeout_ms=60_000).status == xb.XB_OK
+    srv.release(rid)
+    eng = xb.XBlossomGpuEngine10(hg)
+    eng.MultiSourceBFS(reqs[1], mode=xb.XB_MODE_BATCHED, pull_cut_small=4, pull_cut_medium=16)
+    eng.MultiSourceBFS(reqs[1], mode=xb.XB_MODE_BATCHED)
+    rid = srv.submit(reqs[1])
+    with pytest.raises(xb.XbError) as ei:
+        srv.wait(rid, timeout_ms=60_000)
+    assert ei.value.code == xb.XB_ERR_INVALID_ARG
+    srv.release(rid)
+    srv.close()

@@ -205,6 +205,12 @@ struct xb_graph {
   Pinned pinned;
   // In-degree layout for the pull and emit, rebuilt when a cut changes.
   int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1;
+  // Bumped on every rebuild of the layout (d_perm, d_hub_chunks and d_hidx are
+  // freed and reallocated then).  A batch or server holding those pointers
+  // records the generation it was built with and refuses to run on another:
+  // equal cut values after a change and a change back do not mean the same
+  // device arrays.
+  uint64_t layout_gen = 0;
   int32_t* d_perm = nullptr;          // engine ids with in-degree > 0, bucket-major, ascending per bucket
   int32_t bucket_end[3] = {0, 0, 0};  // perm ranges of the small / medium / large buckets
   // A bucket whose perm range is consecutive ids (degree-ordered ids of a
@@ -661,6 +667,7 @@ void ensure_layout(xb_graph* g, const BPOpts& o) {
     g->cut_medium = o.cut_medium;
     g->cut_hub = o.cut_hub;
     g->hub_chunk = o.hub_chunk;
+    g->layout_gen++;
   }
 }
 
@@ -1386,8 +1393,8 @@ struct xb_batch {
   hipStream_t stream = nullptr;
   int T = 8;
   int alpha = 20;
-  int cut_small = 0, cut_medium = 0, cut_hub = 0, hub_chunk = 0;
   std::vector<xb_graph*> graphs;
+  std::vector<uint64_t> layout_gen;  // each graph's layout generation when the batch was built
   std::vector<void*> allocs;  // device memory owned by the batch
   xb::Team* d_teams = nullptr;
   xb::GraphView* d_views = nullptr;
@@ -1454,10 +1461,6 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   const BPOpts bo = bp_opts(o);
   b->T = std::min(bo.T, kBatchMaxWords);
   b->alpha = bo.alpha;
-  b->cut_small = bo.cut_small;
-  b->cut_medium = bo.cut_medium;
-  b->cut_hub = bo.cut_hub;
-  b->hub_chunk = bo.hub_chunk;
   b->profile = (o.flags & XB_PROFILE) != 0;
   b->rebalance = (o.flags & XB_FIXED_TEAMS) == 0;
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -1470,6 +1473,7 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     xb_graph* g = graphs[i];
     b->graphs.push_back(g);
     ensure_layout(g, bo);
+    b->layout_gen.push_back(g->layout_gen);
     xb::GraphView& v = views[i];
     v.off = g->d_eoff;
     v.col = g->d_ecol;
@@ -1759,9 +1763,10 @@ void rebalance_teams(xb_batch* b, const std::vector<xb::TeamCtl>& ctl) {
 }
 
 void run_batch(xb_batch* b, xb_run_stats* st) {
-  for (xb_graph* g : b->graphs)
-    if (g->cut_small != b->cut_small || g->cut_medium != b->cut_medium || g->cut_hub != b->cut_hub ||
-        g->hub_chunk != b->hub_chunk)
+  // The captured launch holds the graphs' layout arrays: any rebuild since the
+  // batch was built (even one that restored the same cut-offs) freed them.
+  for (size_t i = 0; i < b->graphs.size(); ++i)
+    if (b->graphs[i]->layout_gen != b->layout_gen[i])
       fail(XB_ERR_INVALID_ARG, "a graph of the batch was re-tuned after the batch was created");
   if (st) std::memset(st, 0, sizeof(*st));
   hipEvent_t e0, e1;
@@ -2468,6 +2473,7 @@ struct ServeSlot {
 struct xb_server {
   xb_graph* g = nullptr;
   BPOpts o{};
+  uint64_t layout_gen = 0;  // the graph's layout generation the slots were set up with
   int dev = 0;
   int32_t tile = 0;
   bool count_edges = true;
@@ -2539,6 +2545,9 @@ bool form_tile(xb_server* sv, std::vector<TileSeg>& segs, std::vector<int32_t>& 
 void run_serve_tile(xb_server* sv, ServeSlot* sl, const std::vector<TileSeg>& segs,
                     const std::vector<int32_t>& src, Profiler& prof, std::vector<int64_t>& edges) {
   const xb_graph* g = sv->g;
+  // The slots' tiles read the graph's layout arrays; a rebuild freed them.
+  if (g->layout_gen != sv->layout_gen)
+    fail(XB_ERR_INVALID_ARG, "the served graph was re-tuned while the server was running");
   const int64_t n = g->n;
   const int32_t cnt = (int32_t)src.size();
   int32_t* base = reinterpret_cast<int32_t*>(sv->arena.base);
@@ -2680,6 +2689,7 @@ int32_t xb_server_create(xb_graph* g, const xb_server_opts* opts, const xb_msbfs
     sv->max_queued = so.max_queued;
     HIPCHK(hipGetDevice(&sv->dev));
     ensure_layout(g, sv->o);
+    sv->layout_gen = g->layout_gen;
     // Slots: every buffer a tile needs is allocated here (freeing device
     // memory synchronizes the device, which would stall the other slots).
     for (int i = 0; i < so.slots; ++i) {
