@@ -177,6 +177,31 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S,
                      int32_t* levels, int32_t* parents,
                      const xb_msbfs_opts* opts, xb_run_stats* stats);
 
+/* ---- allocation-free run -----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: serving loops with a fixed memory footprint and harnesses that
+ * forbid hipMalloc.  Same tables as xb_msbfs_run with XB_OUT_DEVICE (levels
+ * and parents are caller-owned device S x n int32 arrays, both required),
+ * but the call makes no device or pinned allocation and frees nothing: its
+ * queues and tile state live in a static device array of
+ * XB_NOALLOC_SCRATCH_BYTES per GPU, which each GPU the library runs on holds
+ * until the process ends.  The batched engine takes the widest tile (at most
+ * opts->words_per_tile words) whose state fits that array.  Limits:
+ *   - XB_ERR_BUDGET when the graph is too large: the plain engine needs
+ *     about 16 bytes per vertex, a 64-source bit-parallel tile about
+ *     128 bytes per vertex (XB_MODE_AUTO then runs the plain engine);
+ *   - XB_ERR_UNSUPPORTED for XB_NO_LEVELS or XB_PROFILE, and when the graph's
+ *     pull layout was built for other bucket cut-offs than opts' (a graph is
+ *     created with the default cut-offs; xb_msbfs_run with other cut-offs
+ *     rebuilds the layout, which allocates).
+ * Calls on one GPU share its array and are serialized by a lock inside the
+ * library; every call clears the state it reads, so nothing carries over from
+ * the previous call. */
+#define XB_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int32_t S,
+                             int32_t* levels_dev, int32_t* parents_dev,
+                             const xb_msbfs_opts* opts, xb_run_stats* stats);
+
 /* ---- batched multi-graph runs: one persistent, hipGraph-captured launch ----
  *
  * A batch is G graphs, each with its own source set and device output

@@ -178,3 +178,11 @@ def test_text_loader_parses_large_files_in_parallel(tmp_path):
     (tmp_path / "bad.txt").write_text(txt[:cut] + "\n12x3" + txt[cut:])
     rc = lib.xb_graph_load_text(str(rp).encode(), str(tmp_path / "bad.txt").encode(), 0, ctypes.byref(h))
     assert rc == xb.XB_ERR_IO
+
+
+def test_noalloc_null_arguments():
+    """The allocation-free entry point validates before touching a device."""
+    lib = xb.load_engine()
+    src = np.zeros(1, dtype=np.int32)
+    st = xb.RunStats()
+    assert lib.xb_msbfs_run_noalloc(None, xb._ptr(src), 1, None, None, None, ctypes.byref(st)) == xb.XB_ERR_INVALID_ARG

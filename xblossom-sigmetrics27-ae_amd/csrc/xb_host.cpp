@@ -203,6 +203,10 @@ struct xb_graph {
   int32_t* d_ein_col_o = nullptr;   // the same in-neighbours as original ids
   hipStream_t stream = nullptr;
   Pinned pinned;
+  // Pinned upload buffer of the allocation-free entry point: a tile's first
+  // frontier (distinct sources and their degree prefix) and its source ids,
+  // at most 3 x 1,024 entries.  Allocated with the graph.
+  int32_t* q0_pinned = nullptr;
   // In-degree layout for the pull and emit, rebuilt when a cut changes.
   int32_t cut_small = -1, cut_medium = -1, cut_hub = -1, hub_chunk = -1;
   // Bumped on every rebuild of the layout (d_perm, d_hub_chunks and d_hidx are
@@ -248,11 +252,16 @@ struct xb_graph {
     dfree(d_perm);
     dfree(d_hub_chunks);
     dfree(d_hidx);
+    if (q0_pinned) (void)hipHostFree(q0_pinned);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
+
+// Builds the pull layout for the default cut-offs (defined with the layout
+// code below), so that a fresh graph can run the allocation-free entry point.
+void init_default_layout(xb_graph* g);
 
 void ensure_device() {
   int cnt = 0;
@@ -425,6 +434,8 @@ xb_graph* build_graph(const int32_t* off, int64_t n_plus_1, const int32_t* col, 
     g->d_oin_colo = g->d_col;
   }
   build_engine_graph(g.get(), off, colv.data(), in_off, in_col, (flags & XB_GRAPH_DIRECTED) != 0);
+  init_default_layout(g.get());
+  HIPCHK(hipHostMalloc((void**)&g->q0_pinned, sizeof(int32_t) * 3 * 1024, hipHostMallocDefault));
   return g.release();
 }
 
@@ -1099,6 +1110,12 @@ BPOpts bp_opts(const xb_msbfs_opts& o) {
   return BPOpts{o.words_per_tile, o.alpha, o.pull_cut_small, o.pull_cut_medium, o.pull_cut_hub, o.hub_chunk};
 }
 
+void init_default_layout(xb_graph* g) {
+  xb_msbfs_opts o;
+  xb_opts_default(&o);
+  ensure_layout(g, bp_opts(o));
+}
+
 void check_sources(const xb_graph* g, const int32_t* sources, int32_t S) {
   for (int32_t i = 0; i < S; ++i)
     if (sources[i] < 0 || sources[i] >= g->n)
@@ -1375,6 +1392,147 @@ int32_t xb_last_error(char* buf, size_t len) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Allocation-free runs: the workspace lives in a static device array
+// ===========================================================================
+// One copy of `scratch` per GPU (each device the library runs on holds its own
+// at its own address until the process ends).  Host code never uses the
+// array's name as an address: it is a host-side stand-in.  The device address
+// comes from hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[XB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+  std::mutex mu;         // serializes the calls on one device (they share its array)
+  void* ptr = nullptr;   // device address of that device's copy of `scratch`
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// The current device's scratch address and lock, set up on first use.
+NoAllocDevice& noalloc_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) fail(XB_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  NoAllocDevice& d = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.ptr) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(XB_ERR_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+    d.ptr = p;
+  }
+  return d;
+}
+
+// Carves a workspace out of `base` (null: only sizes it).  T = 0: the plain
+// engine's queues only; T > 0: also the bit-parallel tile state for tiles of
+// up to T words.  Every piece is 256-byte aligned.  Returns the bytes used.
+size_t carve_workspace(const xb_graph* g, int T, char* base, Workspace* w) {
+  size_t used = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = base ? base + used : nullptr;
+    used += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const size_t n1 = (size_t)g->n + 1;
+  Workspace ws;
+  ws.ctrs = (unsigned long long*)take(2 * sizeof(unsigned long long));
+  ws.tcount = (uint32_t*)take(sizeof(uint32_t));
+  for (int i = 0; i < 2; ++i) {
+    ws.q[i].v = (int32_t*)take(sizeof(int32_t) * n1);
+    ws.q[i].eo = (int32_t*)take(sizeof(int32_t) * n1);
+    ws.q[i].ctr = ws.ctrs ? ws.ctrs + i : nullptr;
+  }
+  if (T > 0) {
+    const size_t nw = (size_t)g->n * (size_t)T + 1;
+    ws.bp.seen = (uint64_t*)take(sizeof(uint64_t) * nw);
+    ws.bp.front[0] = (uint64_t*)take(sizeof(uint64_t) * nw);
+    ws.bp.front[1] = (uint64_t*)take(sizeof(uint64_t) * nw);
+    ws.bp.nextw = (uint64_t*)take(sizeof(uint64_t) * nw);
+    ws.bp.planes = (uint64_t*)take(sizeof(uint64_t) * nw * 8);
+    ws.bp.tflag = (uint32_t*)take(sizeof(uint32_t) * n1);
+    ws.bp.tlist = (int32_t*)take(sizeof(int32_t) * n1);
+    ws.bp.tcount = ws.tcount;
+    ws.bp.n = g->n;
+    ws.side_cap = (int64_t)std::max(g->n_heavy, 1) * T * 64;
+    ws.side = (int32_t*)take(sizeof(int32_t) * (size_t)ws.side_cap);
+    ws.src_cap = 64 * (int64_t)T;
+    ws.d_src = (int32_t*)take(sizeof(int32_t) * (size_t)ws.src_cap);
+    ws.T = T;
+    ws.bp_dirty = true;  // the array keeps the last call's contents: clear all tile state first
+  }
+  if (w) {
+    ws.n = g->n;
+    ws.st = g->stream;
+    ws.hctr = g->pinned.p;
+    ws.q0_host = g->q0_pinned;
+    ws.q0_cap = 3 * 1024;
+    *w = ws;
+  }
+  return used;
+}
+
+void run_noalloc(xb_graph* g, const int32_t* sources, int32_t S, int32_t* levels, int32_t* parents,
+                 const xb_msbfs_opts* opts, xb_run_stats* stats) {
+  if (!g) fail(XB_ERR_INVALID_ARG, "null graph");
+  if (S < 0) fail(XB_ERR_INVALID_ARG, "S < 0");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (S == 0) return;
+  if (!sources || !levels || !parents) fail(XB_ERR_INVALID_ARG, "null sources, levels or parents");
+  const xb_msbfs_opts o = parse_opts(opts);
+  if (o.flags & (XB_NO_LEVELS | XB_PROFILE))
+    fail(XB_ERR_UNSUPPORTED, "the allocation-free run writes both device tables and takes no XB_PROFILE");
+  check_sources(g, sources, S);
+  BPOpts bo = bp_opts(o);
+  if (o.mode != XB_MODE_PLAIN &&
+      (bo.cut_small != g->cut_small || bo.cut_medium != g->cut_medium || bo.cut_hub != g->cut_hub ||
+       bo.hub_chunk != g->hub_chunk))
+    fail(XB_ERR_UNSUPPORTED, "the graph's pull layout was built for other cut-offs; run xb_msbfs_run once with "
+                             "these options to rebuild it");
+  // Widest tile (at most the requested words) whose state fits the scratch.
+  int T = 0;
+  if (o.mode != XB_MODE_PLAIN)
+    for (int t = bo.T; t >= 1 && T == 0; t /= 2)
+      if (carve_workspace(g, t, nullptr, nullptr) <= XB_NOALLOC_SCRATCH_BYTES) T = t;
+  const bool plain = o.mode == XB_MODE_PLAIN || (T == 0 && o.mode == XB_MODE_AUTO);
+  if (!plain && T == 0)
+    fail(XB_ERR_BUDGET, "graph too large for the allocation-free scratch: one 64-source tile needs " +
+                            std::to_string(carve_workspace(g, 1, nullptr, nullptr)) + " bytes, the scratch holds " +
+                            std::to_string((size_t)XB_NOALLOC_SCRATCH_BYTES));
+  if (plain && carve_workspace(g, 0, nullptr, nullptr) > XB_NOALLOC_SCRATCH_BYTES)
+    fail(XB_ERR_BUDGET, "graph too large for the allocation-free scratch: the plain engine's queues need " +
+                            std::to_string(carve_workspace(g, 0, nullptr, nullptr)) + " bytes, the scratch holds " +
+                            std::to_string((size_t)XB_NOALLOC_SCRATCH_BYTES));
+  NoAllocDevice& dev = noalloc_device();
+  std::lock_guard<std::mutex> lk(dev.mu);
+  Workspace w;
+  carve_workspace(g, plain ? 0 : T, (char*)dev.ptr, &w);
+  Profiler prof;  // off: byte and launch counts only, no events
+  prof.st = g->stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (plain) {
+    run_plain(g, w, sources, S, levels, parents, true, prof, stats);
+  } else {
+    bo.T = T;
+    run_batched(g, w, bo, sources, S, levels, parents, true, prof, stats);
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (stats) {
+    prof.collect(stats);
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
+
+}  // namespace
+
+extern "C" int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int32_t S, int32_t* levels_dev,
+                                        int32_t* parents_dev, const xb_msbfs_opts* opts, xb_run_stats* stats) {
+  return guarded([&] { run_noalloc(g, sources, S, levels_dev, parents_dev, opts, stats); });
+}
 
 // ===========================================================================
 // Batched multi-graph runs: one persistent, hipGraph-captured launch

@@ -64,6 +64,7 @@ EXPORTED_SYMBOLS = (
     "xb_graph_info",
     "xb_graph_destroy",
     "xb_msbfs_run",
+    "xb_msbfs_run_noalloc",
     "xb_count_traversed",
     "xb_device_checksum",
     "xb_test_edge_map",
@@ -88,6 +89,8 @@ EXPORTED_SYMBOLS = (
 )
 
 XB_SERVE_COUNT_EDGES = 1
+# Bytes of the static device array behind xb_msbfs_run_noalloc (per GPU).
+XB_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class XbError(RuntimeError):
@@ -245,6 +248,8 @@ def load_engine():
     lib.xb_graph_save_binary.argtypes = [VP, ctypes.c_char_p]
     lib.xb_graph_destroy.argtypes = [VP]
     lib.xb_msbfs_run.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts), ctypes.POINTER(RunStats)]
+    lib.xb_msbfs_run_noalloc.argtypes = [VP, _P32, I32, _P32, _P32, ctypes.POINTER(Opts),
+                                         ctypes.POINTER(RunStats)]
     lib.xb_count_traversed.argtypes = [VP, _P32, I32, ctypes.POINTER(I64)]
     lib.xb_device_checksum.argtypes = [_P32, I64, ctypes.POINTER(ctypes.c_uint64)]
     lib.xb_test_edge_map.argtypes = [_P32, I32, _P32, _P32, ctypes.POINTER(I64)]
@@ -601,6 +606,18 @@ def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False,
     out = np.empty(max(n, 1), dtype=np.int32)
     _check(lib.xb_match_run(graph.handle, _ptr(init), _ptr(out), prof, ctypes.byref(st)))
     return out[:n], st
+
+
+def msbfs_run_noalloc(graph: HGraph, sources, levels_dev: int, parents_dev: int, *, mode: int = XB_MODE_AUTO,
+                      **tunables) -> RunStats:
+    """xb_msbfs_run_noalloc: the S x n tables into caller-owned device buffers
+    (addresses as ints), with no allocation inside the call."""
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    o = default_opts(mode=mode, flags=XB_OUT_DEVICE, **tunables)
+    st = RunStats()
+    _check(load_engine().xb_msbfs_run_noalloc(graph.handle, _ptr(src), src.size, _ptr(levels_dev),
+                                              _ptr(parents_dev), ctypes.byref(o), ctypes.byref(st)))
+    return st
 
 
 def device_checksum(ptr: int, count: int) -> int:
