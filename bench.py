@@ -361,12 +361,13 @@ def bench_match(args) -> int:
         desc = "LiveJournal-shaped Chung-Lu (gamma 2.1, seed 4)"
     hg = xb.HGraph(g.row_offsets, g.col_indices)
     out = xb.DeviceBuffer((g.n,))
+    regrow = args.match_regrow
     for _ in range(args.warmup):
-        xb.match(hg, None, out.ptr, device=True)
+        xb.match(hg, None, out.ptr, device=True, regrow=regrow)
     t0 = time.perf_counter()
     stats = []
     for _ in range(args.steps):
-        _, st = xb.match(hg, None, out.ptr, device=True)
+        _, st = xb.match(hg, None, out.ptr, device=True, regrow=regrow)
         stats.append(st)
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
@@ -375,7 +376,7 @@ def bench_match(args) -> int:
     want_pairs = st.matched_vertices // 2
     assert int(np.count_nonzero(mate >= 0)) == st.matched_vertices
     # Per-class kernel times on one more run (HIP events on the engine's stream).
-    _, pst = xb.match(hg, None, out.ptr, device=True, profile=True)
+    _, pst = xb.match(hg, None, out.ptr, device=True, profile=True, regrow=regrow)
     kern = pst.kernels()
     dom = max(kern, key=lambda k: kern[k]["ms"])
     achieved = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
@@ -389,7 +390,7 @@ def bench_match(args) -> int:
         # The whole run on one core (about 20 s): the same algorithm, so its
         # matching must equal the GPU's entry for entry.
         t1 = time.perf_counter()
-        cmate, cst = oracle.match_forest(g.row_offsets, g.col_indices)
+        cmate, cst = oracle.match_forest(g.row_offsets, g.col_indices, reuse=not regrow)
         ct = time.perf_counter() - t1
         bit_exact = bool(np.array_equal(cmate, mate)) and cst.edges_scanned == st.edges_scanned
         cpu = {"value": round(cst.edges_scanned / ct / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
@@ -411,7 +412,12 @@ def bench_match(args) -> int:
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": "match: " + desc + ", maximum matching from the empty matching",
-                   "n": g.n, "m": g.m},
+                   "n": g.n, "m": g.m, "forest": "regrow every phase" if regrow else "tree reuse"},
+        "reused_tree_vertices": int(st.reused_tree_vertices),
+        "reset_tree_vertices": int(st.reset_tree_vertices),
+        "aug_path_edges": int(st.aug_path_edges),
+        "expand_edges": int(st.expand_edges),
+        "blossom_scan_edges": int(st.blossom_scan_edges),
         "matched_pairs": int(want_pairs),
         "phases": st.phases,
         "levels": st.levels,
@@ -438,6 +444,8 @@ def main() -> int:
                                                                  "match"])
     ap.add_argument("--match-graph", default="livejournal", choices=["livejournal", "gnp"],
                     help="match: the LiveJournal-shaped graph, or a blossom-heavy G(n, p)")
+    ap.add_argument("--match-regrow", action="store_true",
+                    help="match: regrow the forest every phase instead of keeping the trees of exposed roots")
     ap.add_argument("--requests", type=int, default=400, help="serving: timed queries")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (queries/s)")
     ap.add_argument("--slots", type=int, default=2,

@@ -62,6 +62,9 @@ extern "C" {
 #define XB_NO_LEVELS 4u   /* levels == NULL allowed: only parents written     */
 #define XB_FIXED_TEAMS 8u /* xb_batch_*: keep the planned team sizes (no
                              re-split of the grid from each run's clocks)    */
+#define XB_MATCH_REGROW 16u /* xb_match_run: regrow the whole forest every
+                               phase instead of keeping the trees of roots
+                               that are still exposed                       */
 
 typedef struct xb_graph xb_graph;
 
@@ -100,9 +103,10 @@ typedef struct xb_msbfs_opts {
 #define XB_K_PERSISTENT 11 /* persistent batch kernel (whole batch, one launch) */
 #define XB_K_MT_SCAN 12    /* matching: augmenting-edge scan and bridge passes  */
 #define XB_K_MT_EXPAND 13  /* matching: claim and commit (forest growth)        */
-#define XB_K_MT_TREE 14    /* matching: forest reset, tree pairing, path flips  */
+#define XB_K_MT_TREE 14    /* matching: tree pairing, path flips, compaction    */
 #define XB_K_MT_BLOSSOM 15 /* matching: blossom labelling                       */
-#define XB_K_CLASSES 16
+#define XB_K_MT_INIT 16    /* matching: phase set-up (forest reset or reuse)   */
+#define XB_K_CLASSES 17
 
 /* Per-run statistics (filled when the pointer is non-null).  kernel_ms and
  * kernel_launches are filled only with XB_PROFILE; kernel_bytes always holds
@@ -276,6 +280,10 @@ int32_t xb_last_error(char* buf, size_t len);
  * become even; an even-even edge between two trees closes an augmenting path.
  * A phase flips a maximal set of tree-disjoint augmenting paths; phases repeat
  * until one finds none, and the result is a maximum matching on every graph.
+ * Between phases the trees whose root is still exposed are kept (the
+ * reference's lazy reset, xblossom_gpu_engine_10.cu:119-175) and the next
+ * phase grows from all their outer vertices; XB_MATCH_REGROW resets the
+ * whole forest every phase instead.
  * All choices are deterministic (lowest root id, then lowest vertex or edge
  * position), so a run's matching is reproducible.  Symmetric graphs only;
  * unmatched vertices are -1. */
@@ -294,12 +302,24 @@ typedef struct xb_match_stats {
   double kernel_ms[XB_K_CLASSES];
   double kernel_bytes[XB_K_CLASSES];
   int64_t kernel_launches[XB_K_CLASSES];
+  /* edges_scanned split as the reference engine splits its counters
+   * (num_aug_path_edges_processed_, num_expand_edges_processed_,
+   * num_blossom_edges_processed_; xblossom_gpu_engine_10.cu:220-221,
+   * 303-304, 400-401). */
+  int64_t aug_path_edges;     /* level-0 cursor scans, augmenting-edge scans, bridge passes */
+  int64_t expand_edges;       /* claim passes                                 */
+  int64_t blossom_scan_edges; /* blossom-candidate passes                     */
+  /* Per phase set-up, summed over phases (the reference's
+   * num_reused_tree_nodes_ / num_reset_tree_nodes_): matched vertices kept
+   * in a tree whose root is still exposed, and every other vertex. */
+  int64_t reused_tree_vertices;
+  int64_t reset_tree_vertices;
   int32_t reserved[8];
 } xb_match_stats;
 
 /* init_mate: NULL (empty matching) or n entries (a valid matching, -1 for
  * exposed); mate_out: n entries.  flags: XB_OUT_DEVICE (both are device
- * pointers) | XB_PROFILE (per-class kernel times).  An invalid init_mate (out of range, asymmetric, or a pair that
+ * pointers) | XB_PROFILE (per-class kernel times) | XB_MATCH_REGROW.  An invalid init_mate (out of range, asymmetric, or a pair that
  * is not an edge) is XB_ERR_INVALID_ARG. */
 int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, uint32_t flags,
                      xb_match_stats* stats);

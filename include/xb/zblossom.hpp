@@ -1,14 +1,26 @@
 // zblossom.hpp -- header-only C++ shim over the C ABI (xb_capi.h) that keeps
-// the reference's entry points, so existing C++ call sites compile unchanged
-// and gain MultiSourceBFS():
+// the reference's engine interface and gains MultiSourceBFS():
 //
 //   zblossom::HGraph(row_offsets_path, col_indices_path)
 //       apps/xb/src/include/graph/hgraph_gpu.h:37-48 (host CSR, move-only;
 //       nnodes = |rowOffsets| - 1, nedges = |colIndices| / 2 as at :44-45)
-//   zblossom::XBlossomGpuEngine10(const HGraph&, size_t path_table_buffer_ratio)
-//       apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:25 (the engine
+//   zblossom::XBlossomGpuEngine10(const HGraph&[, size_t path_table_buffer_ratio])
+//       apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:23-25 (the engine
 //       owns the device copy of the graph, as DGraph does in
-//       apps/xb/src/include/graph/dgraph.h:50-63)
+//       apps/xb/src/include/graph/dgraph.h:50-63), with FindMaximumMatch
+//       (max_match_engine.h:84-113) and the statistics and timing getters of
+//       xblossom_gpu_engine_10.h:39-95 (PrintTimingSummary, GetTimingSummary,
+//       NumEdgesProcessed, NumAugmentingPathEdgesProcessed,
+//       NumExpandEdgesProcessed, NumBlossomEdgesProcessed, NumBlossoms,
+//       NumReusedTreeNodes, NumResetTreeNodes)
+//   Matching, host_graph_t  the global aliases of
+//       apps/xb/src/include/utils/mm_types_gpu.h:9,20
+//
+// A call site that uses these (the engine calls of
+// apps/xb/runner/xblsm_gpu/run_xb_pp_by_dataset.cu:21-69) compiles against this
+// header; tests/cpp/shim_test.cpp is such a call site.  The runner's glog,
+// gflags and timing helpers (common/*_guard.h) are command-line scaffolding
+// outside the engine and are not provided.
 //
 // Status codes become exceptions (zblossom::XbException), replacing the
 // reference's CUDA_CHECK terminate (apps/xb/src/include/utils/utils_gpu.h:54-63).
@@ -113,6 +125,11 @@ class HGraph {
 // Engine bound to one device-resident copy of a graph.
 class XBlossomGpuEngine10 {
  public:
+  static std::string EngineName() { return "XBlossomGpuEngine10(MI355X: Reuse+ParBlossom+MinKey)"; }
+
+  // The reference's one-argument form (xblossom_gpu_engine_10.h:23) sizes its
+  // path table by a default ratio; no path table exists here.
+  explicit XBlossomGpuEngine10(const HGraph& g) : XBlossomGpuEngine10(g, 0) {}
   XBlossomGpuEngine10(const HGraph& g, size_t path_table_buffer_ratio)
       : n_(g.nnodes), path_table_buffer_ratio_(path_table_buffer_ratio) {
     XbCheck(xb_graph_create_csr(g.row_offsets, (int64_t)g.GetRowOffsetsVector().size(), g.col_indices,
@@ -167,18 +184,66 @@ class XBlossomGpuEngine10 {
   // The result is a maximum matching (blossoms included).
   Matching FindMaximumMatch() { return FindMaximumMatch(Matching(n_, (int32_t)n_)); }
 
+  // The forest keeps the trees of still-exposed roots between phases, as the
+  // reference engine does; RegrowForest(true) resets it every phase instead.
   Matching FindMaximumMatch(const Matching& initial_matching) {
     if (initial_matching.size() != n_)
       throw XbException(XB_ERR_INVALID_ARG, "initial matching must have nnodes entries");
     Matching mate(initial_matching);
     for (auto& m : mate)
       if (m == (int32_t)n_) m = -1;
-    XbCheck(xb_match_run(graph_, mate.data(), mate.data(), 0, &match_stats_));
+    const uint32_t flags = XB_PROFILE | (regrow_ ? XB_MATCH_REGROW : 0u);
+    XbCheck(xb_match_run(graph_, mate.data(), mate.data(), flags, &match_stats_));
     for (auto& m : mate)
       if (m < 0) m = (int32_t)n_;
+    // Counters and times add up over the engine's calls, as the reference's
+    // members do (xblossom_gpu_engine_10.h:99-116).
+    const xb_match_stats& s = match_stats_;
+    aug_edges_ += (size_t)s.aug_path_edges;
+    expand_edges_ += (size_t)s.expand_edges;
+    blossom_edges_ += (size_t)s.blossom_scan_edges;
+    blossoms_ += (size_t)s.blossoms;
+    reused_ += (size_t)s.reused_tree_vertices;
+    reset_ += (size_t)s.reset_tree_vertices;
+    t_aug_ms_ += s.kernel_ms[XB_K_MT_SCAN] + s.kernel_ms[XB_K_MT_TREE];
+    t_expand_ms_ += s.kernel_ms[XB_K_MT_EXPAND];
+    t_blossom_ms_ += s.kernel_ms[XB_K_MT_BLOSSOM];
+    t_pre_ms_ += s.kernel_ms[XB_K_MT_INIT];
     return mate;
   }
   const xb_match_stats& LastMatchStats() const { return match_stats_; }
+  void RegrowForest(bool on) { regrow_ = on; }
+
+  // ---- statistics and timing of xblossom_gpu_engine_10.h:39-95 ----------
+  // Times are device time per kernel class (HIP events), in SECONDS like the
+  // reference's (which divides nanoseconds by 1e9 despite the _ms names):
+  // augmenting path = scan, bridge, pairing and flip passes; expand = claim
+  // and commit; blossom = candidate collection, sort and labelling;
+  // preprocessing = the phase set-up (forest reset or reuse).
+  void GetTimingSummary(double& augmenting_path_ms, double& expand_ms, double& blossom_ms,
+                        double& total_ms) const {
+    augmenting_path_ms = t_aug_ms_ / 1e3;
+    expand_ms = t_expand_ms_ / 1e3;
+    blossom_ms = t_blossom_ms_ / 1e3;
+    total_ms = augmenting_path_ms + expand_ms + blossom_ms;
+  }
+  void PrintTimingSummary() const {
+    double a = 0, e = 0, b = 0, t = 0;
+    GetTimingSummary(a, e, b, t);
+    std::printf("%s Timing Summary:\n", EngineName().c_str());
+    std::printf("Augmenting Path: %g\n", a);
+    std::printf("Expand: %g\n", e);
+    std::printf("Blossom: %g\n", b);
+    std::printf("Preprocessing Time: %g\n", t_pre_ms_ / 1e3);
+    std::printf("Total Time: %g\n", t);
+  }
+  size_t NumEdgesProcessed() const { return aug_edges_ + expand_edges_ + blossom_edges_; }
+  size_t NumAugmentingPathEdgesProcessed() const { return aug_edges_; }
+  size_t NumExpandEdgesProcessed() const { return expand_edges_; }
+  size_t NumBlossomEdgesProcessed() const { return blossom_edges_; }
+  size_t NumBlossoms() const { return blossoms_; }
+  size_t NumReusedTreeNodes() const { return reused_; }
+  size_t NumResetTreeNodes() const { return reset_; }
 
   xb_graph* handle() const { return graph_; }
 
@@ -189,6 +254,15 @@ class XBlossomGpuEngine10 {
   xb_msbfs_opts opts_{};
   xb_run_stats stats_{};
   xb_match_stats match_stats_{};
+  bool regrow_ = false;
+  size_t aug_edges_ = 0, expand_edges_ = 0, blossom_edges_ = 0, blossoms_ = 0, reused_ = 0, reset_ = 0;
+  double t_aug_ms_ = 0, t_expand_ms_ = 0, t_blossom_ms_ = 0, t_pre_ms_ = 0;
 };
 
 }  // namespace zblossom
+
+// The reference's global aliases (apps/xb/src/include/utils/mm_types_gpu.h:9,20).
+#ifndef XB_NO_GLOBAL_ALIASES
+using Matching = zblossom::Matching;
+using host_graph_t = zblossom::HGraph;
+#endif

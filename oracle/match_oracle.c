@@ -31,6 +31,13 @@
  *                           lowest (vertex in the lower tree << 32 | vertex in
  *                           the higher tree); taken trees leave the later
  *                           rounds of the phase.
+ *                       Between phases the trees whose root is still exposed
+ *                       are kept, as the reference's lazy reset keeps them
+ *                       (xblossom_gpu_engine_10.cu:119-175; CPU:
+ *                       apps/xb/src/include/xblsm_cpu/xb_cpu_reuse.h:8-71),
+ *                       and the next phase grows from all their outer
+ *                       vertices; the regrow mode (reuse = 0) resets the
+ *                       whole forest every phase.
  *                       The GPU result must equal this one entry for entry.
  * oracle_match_edmonds  Maximum cardinality matching of a general graph by
  *                       Edmonds' blossom algorithm (J. Edmonds, "Paths, trees,
@@ -67,6 +74,17 @@ typedef struct oracle_match_stats {
   int64_t blossom_vertices; /* odd vertices made even by them */
   int32_t phases;
   int32_t levels;
+  /* edges_scanned split as the reference's counters split it
+   * (xblossom_gpu_engine_10.cu:220-221, 303-304, 400-401): */
+  int64_t aug_path_edges;     /* level-0 cursor scans, augmenting-edge scans, bridge passes */
+  int64_t expand_edges;       /* claim passes */
+  int64_t blossom_scan_edges; /* blossom-candidate passes */
+  /* Per phase, summed (the reference's lazy reset, xblossom_gpu_engine_10.cu:
+   * 119-155): matched vertices kept in a tree whose root is still exposed,
+   * and the others (exposed roots, vertices of augmented trees, vertices in
+   * no tree); the two add up to n per phase. */
+  int64_t reused_tree_vertices;
+  int64_t reset_tree_vertices;
 } oracle_match_stats;
 
 #define NO_KEY (~0ull)
@@ -135,11 +153,10 @@ static int is_flagged(const Forest* f, int32_t u, uint64_t stamp, int dflag) {
 /* Gabow's procedure L for the even-even edge (x, y) of one tree: walks up
  * from FIRST(x) and FIRST(y) alternately to their join, then makes every
  * non-outer vertex below the join outer with the edge label of its side
- * (x side: (x, y); y side: (y, x)), FIRST = join, level next_level, appended
- * to next[].  Returns the number of vertices made outer (0: x and y already
+ * (x side: (x, y); y side: (y, x)), FIRST = join, its (odd) level plus one,
+ * appended to next[].  Returns the number of vertices made outer (0: x and y already
  * share a blossom). */
-static int64_t blossom_step(Forest* f, int32_t x, int32_t y, uint64_t stamp, int32_t next_level, int32_t* next,
-                            int32_t* nn) {
+static int64_t blossom_step(Forest* f, int32_t x, int32_t y, uint64_t stamp, int32_t* next, int32_t* nn) {
   int32_t r = first_of(f, x), s = first_of(f, y);
   if (r == s) return 0;
   int dflag = 0;
@@ -168,7 +185,7 @@ static int64_t blossom_step(Forest* f, int32_t x, int32_t y, uint64_t stamp, int
       const int32_t nv = first_of(f, f->par[v]);
       f->elab[v] = label;
       f->fp[v] = join;
-      f->lvl[v] = next_level;
+      f->lvl[v] += 1;  /* inner (odd) -> outer (even), still above its FIRST chain */
       next[(*nn)++] = v;
       ++made;
       v = nv;
@@ -214,9 +231,48 @@ static int cmp_u64(const void* a, const void* b) {
   return x < y ? -1 : x > y;
 }
 
-/* max_phases > 0 stops after that many phases (a bounded sample for timing). */
+/* Phase set-up with tree reuse (the reference's lazy reset,
+ * apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:119-175, and the CPU
+ * engine's parReuseRemainingTrees, apps/xb/src/include/xblsm_cpu/
+ * xb_cpu_reuse.h:8-71): the vertices of the trees that augmented in the last
+ * phase (their roots are matched now) are reset; the trees whose root is still
+ * exposed keep every label.  The phase's first frontier is every outer vertex
+ * of those trees (the reference's ENode2 list), in ascending id order, and
+ * its roots are the exposed vertices.  Returns the frontier size. */
+static int32_t reuse_forest(int32_t n, const int32_t* mate, int32_t* root, int32_t* lvl, int32_t* par, int32_t* fp,
+                            uint64_t* elab, uint64_t* claim, uint8_t* used, int32_t* front, int32_t* roots,
+                            int32_t* nroots, oracle_match_stats* s) {
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t r = root[v];
+    if (r >= 0 && used[r]) {
+      root[v] = -1;
+      lvl[v] = -1;
+      par[v] = -1;
+      fp[v] = DUMMY;
+      elab[v] = NO_KEY;
+      claim[v] = NO_KEY;
+      ++s->reset_tree_vertices;
+    } else if (r >= 0 && mate[v] >= 0) {
+      ++s->reused_tree_vertices;
+    } else {
+      ++s->reset_tree_vertices;  /* an exposed root, or a vertex in no tree */
+    }
+  }
+  int32_t fn = 0, nr = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    used[v] = 0;
+    if (lvl[v] >= 0 && !(lvl[v] & 1)) front[fn++] = v;
+    if (mate[v] < 0) roots[nr++] = v;
+  }
+  *nroots = nr;
+  return fn;
+}
+
+/* max_phases > 0 stops after that many phases (a bounded sample for timing).
+ * reuse != 0 keeps the trees of still-exposed roots from one phase to the
+ * next (reuse_forest); reuse == 0 regrows the whole forest every phase. */
 int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, int32_t* mate,
-                            oracle_match_stats* st, int32_t max_phases) {
+                            oracle_match_stats* st, int32_t max_phases, int32_t reuse) {
   oracle_match_stats s;
   memset(&s, 0, sizeof(s));
   int32_t* root = xcalloc((size_t)n, sizeof(int32_t));
@@ -251,32 +307,46 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
     if (max_phases > 0 && s.phases == max_phases) break;
     ++s.phases;
     int64_t same_tree = 0;
-    int32_t nroots = 0;
-    for (int32_t v = 0; v < n; ++v) {
-      const int ex = mate[v] < 0;
-      root[v] = ex ? v : -1;
-      lvl[v] = ex ? 0 : -1;
-      par[v] = -1;
-      fp[v] = DUMMY;
-      elab[v] = NO_KEY;
-      claim[v] = NO_KEY;
-      best[v] = INT32_MAX;
-      acc[v] = -1;
-      bridge[v] = NO_KEY;
-      used[v] = 0;
-      cursor[v] = off[v];
-      if (ex) roots[nroots++] = v;
+    int32_t nroots = 0, fn = 0;
+    /* roots_only: the frontier is the phase's roots, every tree one root. */
+    int roots_only;
+    if (reuse && s.phases > 1) {
+      fn = reuse_forest(n, mate, root, lvl, par, fp, elab, claim, used, front, roots, &nroots, &s);
+      for (int32_t v = 0; v < n; ++v) {
+        best[v] = INT32_MAX;
+        acc[v] = -1;
+        bridge[v] = NO_KEY;
+      }
+      roots_only = 0;
+    } else {
+      for (int32_t v = 0; v < n; ++v) {
+        const int ex = mate[v] < 0;
+        root[v] = ex ? v : -1;
+        lvl[v] = ex ? 0 : -1;
+        par[v] = -1;
+        fp[v] = DUMMY;
+        elab[v] = NO_KEY;
+        claim[v] = NO_KEY;
+        best[v] = INT32_MAX;
+        acc[v] = -1;
+        bridge[v] = NO_KEY;
+        used[v] = 0;
+        cursor[v] = off[v];
+        if (ex) roots[nroots++] = v;
+      }
+      s.reset_tree_vertices += n;
+      memcpy(front, roots, sizeof(int32_t) * (size_t)nroots);
+      fn = nroots;
+      roots_only = 1;
     }
     if (nroots < 2) break;
-    memcpy(front, roots, sizeof(int32_t) * (size_t)nroots);
-    int32_t fn = nroots, level = 0;
     int64_t found = 0;
     while (fn > 0) {
       int64_t fe = frontier_edges(off, front, fn);
       int64_t level_same_tree = 0;
       for (;;) {
-        if (level == 0 && roots_independent) break;
-        if (level == 0) {
+        if (roots_only && roots_independent) break;
+        if (roots_only) {
           /* Every tree is one root: its lowest neighbouring tree is its first
            * adjacency entry that is an untaken root, found by a cursor that
            * only moves forward (taken roots never return). */
@@ -289,12 +359,12 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
               if (w != t && lvl[w] == 0 && !used[w]) break;
               ++j;
             }
-            s.edges_scanned += (j - j0) + (j < e);
+            s.aug_path_edges += (j - j0) + (j < e);
             cursor[t] = j;
             best[t] = j < e ? col[j] : INT32_MAX;
           }
         } else {
-          s.edges_scanned += fe;
+          s.aug_path_edges += fe;
           for (int32_t i = 0; i < nroots; ++i) best[roots[i]] = INT32_MAX;
           level_same_tree = 0;
           for (int32_t i = 0; i < fn; ++i) {
@@ -319,13 +389,13 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           const int32_t t = roots[i], b = best[t];
           if (b != INT32_MAX && t < b && best[b] == t) {
             acc[t] = b;
-            if (level == 0) bridge[t] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)b;
+            if (roots_only) bridge[t] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)b;
             pairs[np++] = t;
           }
         }
         if (np == 0) break;
-        if (level != 0) {
-          s.edges_scanned += fe;
+        if (!roots_only) {
+          s.aug_path_edges += fe;
           for (int32_t i = 0; i < fn; ++i) {
             const int32_t v = front[i];
             for (int32_t j = off[v]; j < off[v + 1]; ++j) {
@@ -361,7 +431,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         nroots = k;
         fe = frontier_edges(off, front, fn);
       }
-      if (level == 0) roots_independent = 1;
+      if (roots_only) roots_independent = 1;
       if (found) break;
       /* Blossom candidates: the same-tree even-even frontier edges, keyed
        * (root << 32 | edge position), taken before the claims. */
@@ -372,7 +442,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           cand_cap = level_same_tree;
           cand = xcalloc((size_t)cand_cap, sizeof(uint64_t));
         }
-        s.edges_scanned += fe;
+        s.blossom_scan_edges += fe;
         for (int32_t i = 0; i < fn; ++i) {
           const int32_t v = front[i];
           for (int32_t j = off[v]; j < off[v + 1]; ++j) {
@@ -382,7 +452,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
           }
         }
       }
-      s.edges_scanned += fe;
+      s.expand_edges += fe;
       int32_t tn = 0;
       for (int32_t i = 0; i < fn; ++i) {
         const int32_t v = front[i];
@@ -399,14 +469,17 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
         const int32_t w = claimed[i], x = mate[w];
         const uint64_t kw = claim[w], kx = claim[x];
         if (kw < kx || (kw == kx && w < x)) {
-          const int32_t rv = (int32_t)(kw >> 32);
+          /* Levels follow the claimer (the frontier of a reused forest mixes
+           * levels): odd for the claimed vertex, even for its mate, so levels
+           * rise strictly down every tree path. */
+          const int32_t rv = (int32_t)(kw >> 32), cl = (int32_t)(kw & 0xFFFFFFFFu);
           root[w] = rv;
-          par[w] = (int32_t)(kw & 0xFFFFFFFFu);
-          lvl[w] = level + 1;
+          par[w] = cl;
+          lvl[w] = lvl[cl] + 1;
           root[x] = rv;
           par[x] = w;
           fp[x] = w;
-          lvl[x] = level + 2;
+          lvl[x] = lvl[cl] + 2;
           next[nn++] = x;
         }
       }
@@ -420,7 +493,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
             const int32_t mid = (lo + hi + 1) >> 1;
             if (off[mid] <= j) lo = mid; else hi = mid - 1;
           }
-          const int64_t made = blossom_step(&F, lo, col[j], stamp_base + (uint64_t)i + 1, level + 2, next, &nn);
+          const int64_t made = blossom_step(&F, lo, col[j], stamp_base + (uint64_t)i + 1, next, &nn);
           if (made > 0) {
             ++s.blossoms;
             s.blossom_vertices += made;
@@ -432,7 +505,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
       front = next;
       next = t;
       fn = nn;
-      level += 2;
+      roots_only = 0;
       ++s.levels;
     }
     s.augmentations += found;
@@ -441,6 +514,7 @@ int64_t oracle_match_forest(int32_t n, const int32_t* off, const int32_t* col, i
       break;
     }
   }
+  s.edges_scanned = s.aug_path_edges + s.expand_edges + s.blossom_scan_edges;
   s.matched_vertices = count_matched(n, mate);
   free(root);
   free(lvl);

@@ -22,7 +22,10 @@ class MatchStats(ctypes.Structure):
     _fields_ = [("matched_vertices", ctypes.c_int64), ("augmentations", ctypes.c_int64),
                 ("blossom_edges", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
                 ("blossoms", ctypes.c_int64), ("blossom_vertices", ctypes.c_int64),
-                ("phases", ctypes.c_int32), ("levels", ctypes.c_int32)]
+                ("phases", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("aug_path_edges", ctypes.c_int64), ("expand_edges", ctypes.c_int64),
+                ("blossom_scan_edges", ctypes.c_int64), ("reused_tree_vertices", ctypes.c_int64),
+                ("reset_tree_vertices", ctypes.c_int64)]
 
 
 def load():
@@ -42,7 +45,7 @@ def load():
         lib.oracle_edge_map.restype = I64
         lib.oracle_exclusive_scan.argtypes = [_P32, I64, _P64]
         lib.oracle_exclusive_scan.restype = I64
-        lib.oracle_match_forest.argtypes = [I32, _P32, _P32, _P32, ctypes.POINTER(MatchStats), I32]
+        lib.oracle_match_forest.argtypes = [I32, _P32, _P32, _P32, ctypes.POINTER(MatchStats), I32, I32]
         lib.oracle_match_forest.restype = I64
         lib.oracle_match_edmonds.argtypes = [I32, _P32, _P32, _P32]
         lib.oracle_match_edmonds.restype = I64
@@ -105,9 +108,11 @@ def _csr(row_offsets, col_indices):
     return ro, ci, ro.size - 1
 
 
-def match_forest(row_offsets, col_indices, init_mate=None, max_phases: int = 0):
+def match_forest(row_offsets, col_indices, init_mate=None, max_phases: int = 0, reuse: bool = True):
     """Deterministic alternating-forest matching: (mate, MatchStats); -1 = exposed.
 
+    reuse keeps the trees of still-exposed roots across phases (the default,
+    as in the engine); reuse=False regrows the forest every phase.
     max_phases > 0 stops after that many phases (a bounded timing sample)."""
     lib = load()
     ro, ci, n = _csr(row_offsets, col_indices)
@@ -115,7 +120,7 @@ def match_forest(row_offsets, col_indices, init_mate=None, max_phases: int = 0):
     if init_mate is not None:
         mate[:n] = np.asarray(init_mate, dtype=np.int32)
     st = MatchStats()
-    lib.oracle_match_forest(n, _p(ro), _p(ci), _p(mate), ctypes.byref(st), max_phases)
+    lib.oracle_match_forest(n, _p(ro), _p(ci), _p(mate), ctypes.byref(st), max_phases, 1 if reuse else 0)
     return mate[:n], st
 
 

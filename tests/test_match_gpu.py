@@ -21,16 +21,20 @@ import xbmsbfs as xb
 pytestmark = pytest.mark.gpu
 
 STATS = ("matched_vertices", "augmentations", "blossom_edges", "edges_scanned", "blossoms", "blossom_vertices",
-         "phases", "levels")
+         "phases", "levels", "aug_path_edges", "expand_edges", "blossom_scan_edges", "reused_tree_vertices",
+         "reset_tree_vertices")
 
 
-def _same(ro, ci, init=None, label=""):
+def _same(ro, ci, init=None, label="", regrow=False):
+    """GPU vs oracle in one forest mode (tree reuse, the default, or regrow)."""
     hg = xb.HGraph(ro, ci)
-    got, gs = xb.match(hg, init)
-    want, ws = oracle.match_forest(ro, ci, init)
+    got, gs = xb.match(hg, init, regrow=regrow)
+    want, ws = oracle.match_forest(ro, ci, init, reuse=not regrow)
+    label = f"{label} ({'regrow' if regrow else 'reuse'})"
     assert np.array_equal(got, want), f"{label}: mate differs at {np.flatnonzero(got != want)[:8]}"
     for k in STATS:
         assert getattr(gs, k) == getattr(ws, k), f"{label}: {k} {getattr(gs, k)} != {getattr(ws, k)}"
+    assert gs.reused_tree_vertices + gs.reset_tree_vertices == (ro.size - 1) * gs.phases
     tg.check_matching(ro, ci, got)
     hg.close()
     return got, gs
@@ -53,63 +57,73 @@ def _graphs():
             yield name, (ro, ci)
 
 
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
 @pytest.mark.parametrize("name,csr", list(_graphs()), ids=[n for n, _ in _graphs()])
-def test_small_graphs_equal_oracle(name, csr):
-    _same(*csr, label=name)
+def test_small_graphs_equal_oracle(name, csr, regrow):
+    _same(*csr, label=name, regrow=regrow)
 
 
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
 @pytest.mark.parametrize("seed", range(4))
-def test_random_bipartite_equal_oracle_and_maximum(seed):
+def test_random_bipartite_equal_oracle_and_maximum(seed, regrow):
     nl, nr = 20_000 + 5_000 * seed, 18_000 + 7_000 * seed
     ro, ci = tg.random_bipartite(nl, nr, 1.0 + 0.7 * seed, 40 + seed)
-    got, st = _same(ro, ci, label=f"bipartite seed {seed}")
+    got, st = _same(ro, ci, label=f"bipartite seed {seed}", regrow=regrow)
     assert st.blossom_edges == 0
     assert tg.matching_size(got) == tg.matching_size(oracle.match_hk(ro, ci))
 
 
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
 @pytest.mark.parametrize("seed", range(6))
-def test_random_general_equal_oracle_and_maximum(seed):
+def test_random_general_equal_oracle_and_maximum(seed, regrow):
     n = 400 + 700 * seed
     ro, ci = tg.gnp(n, (1.2 + 0.8 * seed) / n, 70 + seed)
-    got, st = _same(ro, ci, label=f"gnp seed {seed}")
+    got, st = _same(ro, ci, label=f"gnp seed {seed}", regrow=regrow)
     assert tg.matching_size(got) == tg.matching_size(oracle.match_edmonds(ro, ci))
 
 
-def test_many_small_general_graphs_maximum():
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
+def test_many_small_general_graphs_maximum(regrow):
     """300 small graphs (sparse to dense, and nested odd cycles): GPU equals
-    the oracle and Edmonds' maximum; the blossom step runs in many of them."""
+    the oracle and Edmonds' maximum; the blossom step runs in many of them,
+    and with tree reuse later phases keep trees in many of them."""
     rng = np.random.default_rng(5)
-    with_blossoms = 0
+    with_blossoms = with_reuse = 0
     for seed in range(300):
         if seed % 3 == 2:
             ro, ci = tg.nested_odd_cycles(int(rng.choice([3, 5, 7])), int(rng.integers(1, 4)), seed)
         else:
             n = int(rng.integers(3, 200))
             ro, ci = tg.gnp(n, min(1.0, float(rng.uniform(0.5, 10.0)) / n), 500 + seed)
-        got, st = _same(ro, ci, label=f"small graph {seed}")
+        got, st = _same(ro, ci, label=f"small graph {seed}", regrow=regrow)
         assert tg.matching_size(got) == tg.matching_size(oracle.match_edmonds(ro, ci)), f"small graph {seed}"
         with_blossoms += st.blossoms > 0
+        with_reuse += st.reused_tree_vertices > 0
     assert with_blossoms > 30
+    assert (with_reuse == 0) if regrow else (with_reuse > 30)
 
 
-def test_blossom_step_is_needed():
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
+def test_blossom_step_is_needed(regrow):
     ro, ci = tg.csr_from_adj([[1, 2], [0, 2, 3], [0, 1], [1]])
-    got, st = _same(ro, ci, np.array([-1, 2, 1, -1], dtype=np.int32), label="triangle with pendant")
+    got, st = _same(ro, ci, np.array([-1, 2, 1, -1], dtype=np.int32), label="triangle with pendant", regrow=regrow)
     assert list(got) == [2, 3, 0, 1] and st.blossoms == 1
 
 
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
 @pytest.mark.parametrize("seed", range(3))
-def test_large_general_equal_oracle(seed):
+def test_large_general_equal_oracle(seed, regrow):
     """200k-vertex random graphs, average degree 3-5: many blossoms per phase."""
     n = 200_000
     ro, ci = tg.gnp(n, (3.0 + seed) / n, 900 + seed)
-    got, st = _same(ro, ci, label=f"large gnp {seed}")
+    got, st = _same(ro, ci, label=f"large gnp {seed}", regrow=regrow)
     assert st.blossoms > 0
 
 
-def test_rmat_equal_oracle():
+@pytest.mark.parametrize("regrow", [False, True], ids=["reuse", "regrow"])
+def test_rmat_equal_oracle(regrow):
     g = xb.gen_rmat(16, 8, 0.57, 0.19, 0.19, seed=21)
-    _same(g.row_offsets, g.col_indices, label="rmat16")
+    _same(g.row_offsets, g.col_indices, label="rmat16", regrow=regrow)
 
 
 def test_initial_matching_host_and_device():
@@ -149,6 +163,27 @@ def test_find_maximum_match_shim_sentinel():
     tg.check_matching(ro, ci, mm)
     assert eng.last_match_stats.blossom_edges == 0
     assert np.array_equal(eng.FindMaximumMatch(), m)
+
+
+def test_engine_statistics_api():
+    """The counters and timing getters of xblossom_gpu_engine_10.h:39-95 on
+    the Python engine: equal to the oracle's counts, adding up over calls."""
+    ro, ci = tg.gnp(5_000, 3.0 / 5_000, 31)
+    n = ro.size - 1
+    _, ws = oracle.match_forest(ro, ci)
+    eng = xb.XBlossomGpuEngine10(xb.HGraph(ro, ci), 80)
+    for call in (1, 2):
+        eng.FindMaximumMatch(np.full(n, n, dtype=np.int32))
+        assert eng.NumBlossoms() == call * ws.blossoms
+        assert eng.NumAugmentingPathEdgesProcessed() == call * ws.aug_path_edges
+        assert eng.NumExpandEdgesProcessed() == call * ws.expand_edges
+        assert eng.NumBlossomEdgesProcessed() == call * ws.blossom_scan_edges
+        assert eng.NumEdgesProcessed() == call * ws.edges_scanned
+        assert eng.NumReusedTreeNodes() == call * ws.reused_tree_vertices > 0
+        assert eng.NumResetTreeNodes() == call * ws.reset_tree_vertices
+        a, e, b, t = eng.GetTimingSummary()
+        assert a > 0 and e > 0 and t == a + e + b
+    eng.PrintTimingSummary()
 
 
 def test_rejections():

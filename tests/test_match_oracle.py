@@ -100,11 +100,12 @@ def test_hk_against_scipy(seed):
     assert tg.matching_size(mate) == int(np.count_nonzero(ref >= 0))
 
 
+@pytest.mark.parametrize("reuse", [True, False], ids=["reuse", "regrow"])
 @pytest.mark.parametrize("seed", range(8))
-def test_forest_maximum_on_bipartite(seed):
+def test_forest_maximum_on_bipartite(seed, reuse):
     nl, nr = 200 + 40 * seed, 220 + 30 * seed
     ro, ci = tg.random_bipartite(nl, nr, 0.8 + 0.4 * seed, 100 + seed)
-    mate, st = oracle.match_forest(ro, ci)
+    mate, st = oracle.match_forest(ro, ci, reuse=reuse)
     tg.check_matching(ro, ci, mate)
     assert st.blossom_edges == 0
     assert tg.matching_size(mate) == tg.matching_size(oracle.match_hk(ro, ci))
@@ -124,11 +125,13 @@ def _greedy(ro, ci, order):
     return init
 
 
-def test_forest_maximum_on_general_graphs():
+@pytest.mark.parametrize("reuse", [True, False], ids=["reuse", "regrow"])
+def test_forest_maximum_on_general_graphs(reuse):
     """Equal to Edmonds' maximum on sparse, dense and nested-blossom graphs,
-    from the empty matching and from a greedy start in random order."""
+    from the empty matching and from a greedy start in random order, whether
+    the trees of still-exposed roots are kept between phases or not."""
     rng = np.random.default_rng(0)
-    with_blossoms = 0
+    with_blossoms = with_reuse = 0
     for seed in range(400):
         if seed % 4 == 3:
             ro, ci = tg.nested_odd_cycles(int(rng.choice([3, 5, 7])), int(rng.integers(1, 4)), seed)
@@ -136,17 +139,40 @@ def test_forest_maximum_on_general_graphs():
             n = int(rng.integers(3, 160))
             ro, ci = tg.gnp(n, min(1.0, float(rng.uniform(0.5, 8.0)) / n), 1000 + seed)
         best = tg.matching_size(oracle.match_edmonds(ro, ci))
-        mate, st = oracle.match_forest(ro, ci)
+        mate, st = oracle.match_forest(ro, ci, reuse=reuse)
         tg.check_matching(ro, ci, mate)
         assert tg.matching_size(mate) == best, f"graph {seed}"
         with_blossoms += st.blossoms > 0
+        with_reuse += st.reused_tree_vertices > 0
         if seed % 2 == 0:
             init = _greedy(ro, ci, rng.permutation(ro.size - 1))
-            mate2, st2 = oracle.match_forest(ro, ci, init)
+            mate2, st2 = oracle.match_forest(ro, ci, init, reuse=reuse)
             tg.check_matching(ro, ci, mate2)
             assert tg.matching_size(mate2) == best, f"graph {seed}, greedy start"
             assert st2.augmentations == best - tg.matching_size(init)
     assert with_blossoms > 50  # the blossom step is exercised, not just bipartite-like luck
+    assert (with_reuse > 50) if reuse else (with_reuse == 0)
+
+
+def test_tree_reuse_counters_and_levels():
+    """Every phase set-up counts each vertex once (reused or reset), the first
+    phase resets all of them, regrowing never reuses, and keeping trees grows
+    fewer levels in total (the point of the reference's lazy reset)."""
+    rng = np.random.default_rng(3)
+    levels = {True: 0, False: 0}
+    for seed in range(120):
+        n = int(rng.integers(50, 400))
+        ro, ci = tg.gnp(n, float(rng.uniform(1.0, 6.0)) / n, 2000 + seed)
+        for reuse in (True, False):
+            _, st = oracle.match_forest(ro, ci, reuse=reuse)
+            assert st.reused_tree_vertices + st.reset_tree_vertices == n * st.phases
+            assert st.edges_scanned == st.aug_path_edges + st.expand_edges + st.blossom_scan_edges
+            if not reuse:
+                assert st.reused_tree_vertices == 0
+            _, st1 = oracle.match_forest(ro, ci, reuse=reuse, max_phases=1)
+            assert st1.reset_tree_vertices == n and st1.reused_tree_vertices == 0
+            levels[reuse] += st.levels
+    assert levels[True] < levels[False]
 
 
 def test_blossom_step_is_needed():

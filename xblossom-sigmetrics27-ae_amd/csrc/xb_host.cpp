@@ -2184,11 +2184,14 @@ unsigned long long read_u64(Workspace& w, const unsigned long long* d) {
 
 // Phase loop of FindMaximumMatch (max_match_engine.h:93-113) over the
 // alternating-forest search (xblossom_gpu_engine_10.cu:72-81): grow the
-// forest level by level from all exposed vertices; at every level, take
-// rounds of tree-disjoint augmenting paths until none is left, and end the
-// phase after the first level that yields any; stop when a whole phase
-// (the forest grown until no vertex is added) yields none.
-void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out) {
+// forest step by step; at every step, take rounds of tree-disjoint augmenting
+// paths until none is left, and end the phase after the first step that
+// yields any; stop when a whole phase (the forest grown until no vertex is
+// added) yields none.  The first phase grows from the exposed vertices; with
+// `reuse` every later phase keeps the trees whose root is still exposed
+// (xblossom_gpu_engine_10.cu:119-175) and grows from all their outer
+// vertices, else it regrows the forest from the exposed vertices.
+void run_match(xb_graph* g, MatchState& ms, Profiler& prof, bool reuse, xb_match_stats* out) {
   Workspace& w = g->ws;
   ensure_queues(g, w);
   const int32_t n = g->n;
@@ -2197,10 +2200,12 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   // Algorithmic bytes: per frontier edge col + level (+ root of an even
   // end) for the scans, col + level + the 8-byte claim for the claim pass;
   // per frontier vertex its queue entry, offsets and root; per committed
-  // pair its list entry, mates, claims and the six label writes; per vertex
-  // the 41 bytes of forest state the reset writes.
-  constexpr double kScanEdge = 12, kClaimEdge = 16, kFrontVertex = 16, kCommitPair = 64, kResetVertex = 41,
-                   kRootItem = 12;
+  // pair its list entry, mates, claims, the claimer's level and the six label
+  // writes; per vertex the 41 bytes of forest state the reset writes (the
+  // reuse set-up reads root, mate, the root's flag, level and offsets, and
+  // clears 17 bytes of pairing state).
+  constexpr double kScanEdge = 12, kClaimEdge = 16, kFrontVertex = 16, kCommitPair = 68, kResetVertex = 41,
+                   kRootItem = 12, kReuseVertex = 38;
   HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 16, st));
   HIPCHK(hipMemsetAsync(ms.lab, 0xFF, sizeof(unsigned long long) * (size_t)n, st));
   const int32_t max_steps = 4 * n + 16;
@@ -2209,30 +2214,46 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   // its exposed ends and exposes nothing.  Later phases skip the level-0
   // rounds, which could only scan the roots' lists and find nothing.
   bool roots_independent = false;
+  // The phase's roots: the previous phase's untaken roots (kept by the
+  // rounds' compaction) are exactly the exposed vertices of the next.
+  int32_t nroots = 0;
   for (;;) {
     ++s.phases;
     HIPCHK(hipMemsetAsync(ms.ctl, 0, sizeof(unsigned long long) * 6, st));
     HIPCHK(hipMemsetAsync(w.q[0].ctr, 0, sizeof(unsigned long long), st));
-    prof.begin(XB_K_MT_TREE);
-    xb::launch_mt_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.claim, ms.best, ms.acc, ms.bridge,
-                       ms.used, ms.cursor, ms.fp, ms.elab, w.q[0]);
-    prof.end();
-    prof.add(XB_K_MT_TREE, (kResetVertex + 20) * n);
     int32_t qn = 0, E = 0;
-    read_ctr(w, w.q[0].ctr, &qn, &E);
-    int32_t nroots = qn;
-    if (nroots < 2) break;  // at most one exposed vertex: no augmenting path exists
-    HIPCHK(hipMemcpyAsync(ms.roots, w.q[0].v, sizeof(int32_t) * (size_t)nroots, hipMemcpyDeviceToDevice, st));
+    bool roots_only;  // the frontier is the phase's roots, every tree one root
+    if (reuse && s.phases > 1) {
+      prof.begin(XB_K_MT_INIT);
+      xb::launch_mt_reuse_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.fp, ms.elab, ms.claim, ms.used,
+                               w.q[0], ms.ctl + 12);
+      xb::launch_mt_reuse_clear(st, n, ms.best, ms.acc, ms.bridge, ms.used);
+      prof.end();
+      prof.add(XB_K_MT_INIT, kReuseVertex * n);
+      read_ctr(w, w.q[0].ctr, &qn, &E);
+      roots_only = false;
+      if (nroots < 2) break;
+    } else {
+      prof.begin(XB_K_MT_INIT);
+      xb::launch_mt_init(st, n, ms.mate, g->d_off, ms.root, ms.lvl, ms.par, ms.claim, ms.best, ms.acc, ms.bridge,
+                         ms.used, ms.cursor, ms.fp, ms.elab, w.q[0]);
+      prof.end();
+      prof.add(XB_K_MT_INIT, (kResetVertex + 20) * n);
+      s.reset_tree_vertices += n;
+      read_ctr(w, w.q[0].ctr, &qn, &E);
+      nroots = qn;
+      roots_only = true;
+      if (nroots < 2) break;  // at most one exposed vertex: no augmenting path exists
+      HIPCHK(hipMemcpyAsync(ms.roots, w.q[0].v, sizeof(int32_t) * (size_t)nroots, hipMemcpyDeviceToDevice, st));
+    }
     int cur = 0;
-    int32_t level = 0;
     int64_t found = 0;
     unsigned long long same_tree = 0;  // the phase's count of same-tree even-even edges so far
     while (qn > 0) {
       int64_t level_same_tree = 0;
       for (;;) {
-        if (level == 0 && roots_independent) break;
-        // Level 0: the frontier is the root list, every tree one root.
-        if (level == 0) {
+        if (roots_only && roots_independent) break;
+        if (roots_only) {
           prof.begin(XB_K_MT_SCAN);
           xb::launch_mt_scan0(st, ms.roots, nroots, g->d_off, g->d_col, ms.lvl, ms.used, ms.cursor, ms.best,
                               ms.ctl + 6);
@@ -2248,14 +2269,14 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
           xb::launch_mt_scan(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.used, ms.best, ms.ctl);
           prof.end();
           prof.add(XB_K_MT_SCAN, kScanEdge * E + kFrontVertex * qn);
-          s.edges_scanned += E;
+          s.aug_path_edges += E;
         }
         // The round's counters: pairs, flip-stack pool, kept roots and the
         // next frontier queue (free since the last round's compaction).
         xb::launch_mt_zero_ctrs(st, ms.npairs(), ms.pool_ctr(), ms.kept(), w.q[cur ^ 1].ctr);
         prof.begin(XB_K_MT_TREE);
         xb::launch_mt_accept(st, ms.roots, nroots, ms.best, ms.acc, ms.pairs, ms.npairs(),
-                             level == 0 ? ms.bridge : nullptr);
+                             roots_only ? ms.bridge : nullptr);
         prof.end();
         prof.add(XB_K_MT_TREE, 2 * kRootItem * nroots);
         const unsigned long long* c = read_ctl(w, ms.ctl, 4);
@@ -2263,12 +2284,12 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         level_same_tree = (int64_t)(c[0] - same_tree);
         same_tree = c[0];
         if (np == 0) break;
-        if (level != 0) {
+        if (!roots_only) {
           prof.begin(XB_K_MT_SCAN);
           xb::launch_mt_bridge(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.acc, ms.bridge);
           prof.end();
           prof.add(XB_K_MT_SCAN, kScanEdge * E + kFrontVertex * qn);
-          s.edges_scanned += E;
+          s.aug_path_edges += E;
         }
         prof.begin(XB_K_MT_TREE);
         xb::launch_mt_flip(st, ms.pairs, ms.npairs(), np, ms.acc, ms.bridge, ms.par, ms.elab, ms.mate, ms.used,
@@ -2290,7 +2311,7 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         E = (int32_t)(uint32_t)(w.hctr[0] >> 32);
         nroots = (int32_t)(w.hctr[1] & 0xFFFFFFFFull);
       }
-      if (level == 0) roots_independent = true;
+      if (roots_only) roots_independent = true;
       if (found) break;
       // Blossom candidates are the same-tree even-even edges the scan met;
       // take them before the claims grow the forest.
@@ -2303,7 +2324,7 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
                               (uint32_t)C, ms.err());
         prof.end();
         prof.add(XB_K_MT_BLOSSOM, kScanEdge * E + kFrontVertex * qn + 8.0 * C);
-        s.edges_scanned += E;
+        s.blossom_scan_edges += E;
       }
       HIPCHK(hipMemsetAsync(ms.tcount(), 0, sizeof(uint32_t), st));
       HIPCHK(hipMemsetAsync(w.q[cur ^ 1].ctr, 0, sizeof(unsigned long long), st));
@@ -2311,11 +2332,11 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
       xb::launch_mt_claim(st, g->d_off, g->d_col, w.q[cur], qn, E, ms.root, ms.lvl, ms.claim, ms.tlist,
                           ms.tcount());
       // First claims come one per frontier edge at most, and one per vertex.
-      xb::launch_mt_commit(st, ms.tlist, ms.tcount(), std::min(E, n), level + 2, ms.mate, g->d_off, ms.claim,
-                           ms.root, ms.lvl, ms.par, ms.fp, w.q[cur ^ 1]);
+      xb::launch_mt_commit(st, ms.tlist, ms.tcount(), std::min(E, n), ms.mate, g->d_off, ms.claim, ms.root, ms.lvl,
+                           ms.par, ms.fp, w.q[cur ^ 1]);
       prof.end();
       prof.add(XB_K_MT_EXPAND, kClaimEdge * E + kFrontVertex * qn);
-      s.edges_scanned += E;
+      s.expand_edges += E;
       if (C > 0) {
         // Per tree, in (root, edge position) order: odd vertices on each new
         // blossom become outer and join the next frontier.
@@ -2324,15 +2345,14 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
         HIPCHK(hipMemsetAsync(ms.nmarked(), 0, sizeof(uint32_t), st));
         HIPCHK(hipMemsetAsync(ms.formed, 0, sizeof(uint32_t) * (size_t)C, st));
         xb::launch_mt_blossom(st, g->d_off, g->d_col, n, ms.par, ms.lvl, ms.fp, ms.elab, ms.lab, ms.cand_sorted, C,
-                              ms.join_of, ms.marked, ms.nmarked(), ms.formed, level + 2, w.q[cur ^ 1],
-                              ms.ctl + 10, ms.err());
+                              ms.join_of, ms.marked, ms.nmarked(), ms.formed, w.q[cur ^ 1], ms.ctl + 10, ms.err());
         prof.end();
         prof.add(XB_K_MT_BLOSSOM, 48.0 * C);
       }
       read_ctr(w, w.q[cur ^ 1].ctr, &qn, &E);
       prof.add(XB_K_MT_EXPAND, kCommitPair * qn);
       cur ^= 1;
-      level += 2;
+      roots_only = false;
       ++s.levels;
     }
     s.augmentations += found;
@@ -2344,14 +2364,17 @@ void run_match(xb_graph* g, MatchState& ms, Profiler& prof, xb_match_stats* out)
   HIPCHK(hipMemsetAsync(ms.ctl + 1, 0, sizeof(unsigned long long), st));
   xb::launch_mt_count(st, n, ms.mate, ms.ctl + 1);
   s.matched_vertices = (int64_t)read_u64(w, ms.ctl + 1);
-  const unsigned long long* c = read_ctl(w, ms.ctl + 6, 6);  // [6] .. [11]
+  const unsigned long long* c = read_ctl(w, ms.ctl + 6, 8);  // [6] .. [13]
   const int64_t examined = (int64_t)c[0];
   const unsigned err = (unsigned)(c[1] & 0xFFFFFFFFull);
   s.blossoms = (int64_t)c[4];
   s.blossom_vertices = (int64_t)c[5];
+  s.reused_tree_vertices += (int64_t)c[6];
+  s.reset_tree_vertices += (int64_t)c[7];
   if (err)
     fail(XB_ERR_DEVICE, "matching: a device-side bound was exceeded (error bits " + std::to_string(err) + ")");
-  s.edges_scanned += examined;
+  s.aug_path_edges += examined;
+  s.edges_scanned = s.aug_path_edges + s.expand_edges + s.blossom_scan_edges;
   prof.add(XB_K_MT_SCAN, 9.0 * examined);
   *out = s;
 }
@@ -2363,7 +2386,8 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
   return guarded([&] {
     if (!g || !mate_out) fail(XB_ERR_INVALID_ARG, "null graph or mate_out");
     if (g->flags & XB_GRAPH_DIRECTED) fail(XB_ERR_UNSUPPORTED, "matching needs a symmetric graph");
-    if (flags & ~(uint32_t)(XB_OUT_DEVICE | XB_PROFILE)) fail(XB_ERR_INVALID_ARG, "unknown matching flags");
+    if (flags & ~(uint32_t)(XB_OUT_DEVICE | XB_PROFILE | XB_MATCH_REGROW))
+      fail(XB_ERR_INVALID_ARG, "unknown matching flags");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const int32_t n = g->n;
     if (n == 0) return;
@@ -2398,7 +2422,7 @@ extern "C" int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* 
     Profiler prof;
     prof.on = (flags & XB_PROFILE) != 0;
     prof.st = st;
-    run_match(g, ms, prof, &s);
+    run_match(g, ms, prof, (flags & XB_MATCH_REGROW) == 0, &s);
     HIPCHK(hipMemcpyAsync(mate_out, ms.mate, sizeof(int32_t) * (size_t)n,
                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(e1, st));

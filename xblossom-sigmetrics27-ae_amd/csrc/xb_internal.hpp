@@ -195,6 +195,14 @@ struct FlipStack {
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
                     uint8_t* used, int32_t* cursor, int32_t* fp, unsigned long long* elab, Queue q);
+// Phase set-up with tree reuse: resets the trees of augmented roots, queues
+// the outer vertices of the kept trees (q), counts reused / reset vertices
+// (counts[0], counts[1]); then the pairing state of every vertex is cleared.
+void launch_mt_reuse_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root,
+                          int32_t* lvl, int32_t* par, int32_t* fp, unsigned long long* elab, unsigned long long* claim,
+                          const uint8_t* used, Queue q, unsigned long long* counts);
+void launch_mt_reuse_clear(hipStream_t st, int32_t n, int32_t* best, int32_t* acc, unsigned long long* bridge,
+                           uint8_t* used);
 // Level 0: each root's lowest neighbouring untaken root, by a forward-only
 // cursor over its sorted adjacency; adds the entries read to `examined`.
 void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
@@ -221,8 +229,7 @@ void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Q
 void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
                        int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* lab,
                        const unsigned long long* cand, int32_t C, int32_t* join_of, int32_t* marked,
-                       uint32_t* nmarked, uint32_t* formed, int32_t next_level, Queue nxt,
-                       unsigned long long* counts, unsigned* err);
+                       uint32_t* nmarked, uint32_t* formed, Queue nxt, unsigned long long* counts, unsigned* err);
 // Ascending sort of 64-bit keys (device radix sort); temp_bytes(n) sizes the scratch.
 size_t sort_u64_temp_bytes(int32_t n);
 void sort_u64(void* temp, size_t bytes, const unsigned long long* in, unsigned long long* out, int32_t n,
@@ -238,8 +245,8 @@ void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Que
                      const int32_t* root, const int32_t* lvl, unsigned long long* claim, int32_t* tlist,
                      uint32_t* tcount);
 void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
-                      int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
-                      int32_t* root, int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt);
+                      const int32_t* mate, const int32_t* off, const unsigned long long* claim, int32_t* root,
+                      int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt);
 void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
                         unsigned long long* bad);
 void launch_mt_count(hipStream_t st, int32_t n, const int32_t* mate, unsigned long long* out);

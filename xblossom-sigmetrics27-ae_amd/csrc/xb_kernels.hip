@@ -124,6 +124,43 @@ __device__ __forceinline__ void wave_push_list(bool pred, int32_t v, int32_t* li
   if (pred) list[old + rank] = v;
 }
 
+// Block-aggregated append of v to a plain list with a 32-bit counter: ONE
+// global atomic per block and call (a per-wave atomic on one counter
+// serializes when most waves append).  Block-uniform call.
+__device__ __forceinline__ void block_push_list(bool pred, int32_t v, int32_t* list, uint32_t* ctr) {
+  __shared__ uint32_t s_cnt[kBlock / kWave];
+  __shared__ uint32_t s_base;
+  const uint64_t mask = __ballot(pred);
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == 0) s_cnt[wv] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    if (w < wv) off += s_cnt[w];
+    tot += s_cnt[w];
+  }
+  if (threadIdx.x == 0 && tot) s_base = atomicAdd(ctr, tot);
+  __syncthreads();
+  if (pred) list[s_base + off + (uint32_t)__popcll(mask & lanemask_lt())] = v;
+  __syncthreads();  // s_cnt and s_base are reused by the next call
+}
+
+// Adds a block's sum of x to *dst with ONE atomic.  Block-uniform call.
+__device__ __forceinline__ void block_add(unsigned long long x, unsigned long long* dst) {
+  __shared__ unsigned long long s_part[kBlock / kWave];
+  x = wave_sum64(x);
+  if (lane_id() == 0) s_part[threadIdx.x / kWave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_part[w];
+    if (t) atomicAdd(dst, t);
+  }
+  __syncthreads();
+}
+
 // Rightmost i in [lo, hi) with eo[i] <= e, given eo[lo] <= e.  Zero-degree
 // entries share their successor's offset and are skipped by "rightmost".
 __device__ __forceinline__ int owner_in(const int32_t* __restrict__ eo, int lo, int hi, int e) {
@@ -1446,8 +1483,9 @@ __global__ void k_reset_ctl(TeamCtl* ctl, int32_t n_teams, TileJob* jobs, int32_
 //    lowest (vertex in the lower tree, vertex in the higher tree) edge, and
 //    taken trees drop out of later rounds -- so each phase flips a maximal
 //    set of tree-disjoint paths, identically on every run.
-// Edges closing an odd cycle inside one tree (blossoms) are counted; the
-// blossom transform is not part of this engine.
+// Edges closing an odd cycle inside one tree form blossoms, whose odd vertices
+// become outer (the blossom step below); between phases the trees whose root
+// is still exposed are kept (k_mt_reuse_init).
 
 constexpr unsigned long long kNoKey = ~0ull;
 constexpr int32_t kDummy = -1;  // the non-outer vertex above a root on every path (fp of a root)
@@ -1489,6 +1527,72 @@ __global__ __launch_bounds__(kBlock) void k_mt_init(int32_t n, const int32_t* __
   stage_sync_flush(sg, true, q);
 }
 
+// Phase set-up with tree reuse, first pass (the reference's lazy reset,
+// apps/xb/src/xblsm_gpu/xblossom_gpu_engine_10.cu:119-175): a vertex of a
+// tree whose root augmented last phase (used[root]) is reset; every other
+// label is kept.  Outer vertices of the kept trees are queued as the phase's
+// first frontier (the reference's ENode2 list); the phase's roots are the
+// previous phase's untaken roots, whose list the rounds already keep.
+// counts[0] += matched vertices kept in a tree (reused), counts[1] += the
+// others (exposed roots, reset vertices, vertices in no tree).  `used` is
+// only read here; the second pass clears it.
+__global__ __launch_bounds__(kBlock) void k_mt_reuse_init(int32_t n, const int32_t* __restrict__ mate,
+                                                          const int32_t* __restrict__ off,
+                                                          int32_t* __restrict__ root, int32_t* __restrict__ lvl,
+                                                          int32_t* __restrict__ par, int32_t* __restrict__ fp,
+                                                          unsigned long long* __restrict__ elab,
+                                                          unsigned long long* __restrict__ claim,
+                                                          const uint8_t* __restrict__ used, Queue q,
+                                                          unsigned long long* __restrict__ counts) {
+  AppendStage& sg = g_lds.q.sg;
+  stage_init(sg);
+  unsigned long long reused = 0, reset = 0;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < n; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = b0 + threadIdx.x;
+    bool outer = false;
+    int32_t dv = 0;
+    if (v < n) {
+      const int32_t r = root[v];
+      const bool matched = mate[v] >= 0;
+      if (r >= 0 && used[r]) {
+        root[v] = -1;
+        lvl[v] = -1;
+        par[v] = -1;
+        fp[v] = kDummy;
+        elab[v] = kNoKey;
+        claim[v] = kNoKey;
+        ++reset;
+      } else {
+        if (r >= 0 && matched)
+          ++reused;
+        else
+          ++reset;
+        const int32_t l = lvl[v];
+        outer = l >= 0 && !(l & 1);
+        if (outer) dv = off[v + 1] - off[v];
+      }
+    }
+    stage_put(sg, outer, (int32_t)v, dv);
+    stage_sync_flush(sg, false, q);
+  }
+  stage_sync_flush(sg, true, q);
+  block_add(reused, &counts[0]);
+  block_add(reset, &counts[1]);
+}
+
+// Second pass: the per-phase pairing state of every vertex.
+__global__ __launch_bounds__(kBlock) void k_mt_reuse_clear(int32_t n, int32_t* __restrict__ best,
+                                                           int32_t* __restrict__ acc,
+                                                           unsigned long long* __restrict__ bridge,
+                                                           uint8_t* __restrict__ used) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    best[v] = INT32_MAX;
+    acc[v] = -1;
+    bridge[v] = kNoKey;
+    used[v] = 0;
+  }
+}
+
 // Calls f(v, w, j) for every edge (v, w) = col[j] out of the frontier queue
 // (edge-balanced chunks of kChunk edges per block, as in the push).
 template <class F>
@@ -1521,18 +1625,20 @@ __global__ __launch_bounds__(kBlock) void k_mt_scan(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ lvl, const uint8_t* __restrict__ used,
                                                     int32_t* __restrict__ best,
                                                     unsigned long long* __restrict__ blossoms) {
+  unsigned long long same = 0;
   for_frontier_edges(off, col, q, qn, E, [&](int32_t v, int32_t w, int32_t) {
     const int32_t lw = lvl[w];
     if (lw < 0 || (lw & 1) || w == v) return;
     const int32_t rv = root[v], rw = root[w];
     if (rv == rw) {
-      atomicAdd(blossoms, 1ull);
+      ++same;
       return;
     }
     if (used[rv] || used[rw]) return;
     atomicMin(&best[rv], rw);
     atomicMin(&best[rw], rv);
   });
+  block_add(same, blossoms);
 }
 
 // Level-0 scan.  At level 0 every even vertex is a root, so a root's lowest
@@ -1609,8 +1715,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_scan0(const int32_t* __restrict__
       }
     }
   }
-  cnt = wave_sum64(cnt);
-  if (lane_id() == 0 && cnt) atomicAdd(examined, cnt);
+  block_add(cnt, examined);
 }
 
 // Mutually lowest tree pairs are taken (recorded at the lower root).  At
@@ -1620,14 +1725,20 @@ __global__ __launch_bounds__(kBlock) void k_mt_accept(const int32_t* __restrict_
                                                       const int32_t* __restrict__ best, int32_t* __restrict__ acc,
                                                       int32_t* __restrict__ pairs, uint32_t* __restrict__ npairs,
                                                       unsigned long long* __restrict__ bridge0) {
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nroots; i += (int64_t)gridDim.x * kBlock) {
-    const int32_t t = roots[i];
-    const int32_t b = best[t];
-    if (b != INT32_MAX && t < b && best[b] == t) {
-      acc[t] = b;
-      if (bridge0) bridge0[t] = ((unsigned long long)(uint32_t)t << 32) | (uint32_t)b;
-      pairs[atomicAdd(npairs, 1u)] = t;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlock; b0 < nroots; b0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = b0 + threadIdx.x;
+    bool take = false;
+    int32_t t = 0;
+    if (i < nroots) {
+      t = roots[i];
+      const int32_t b = best[t];
+      if (b != INT32_MAX && t < b && best[b] == t) {
+        take = true;
+        acc[t] = b;
+        if (bridge0) bridge0[t] = ((unsigned long long)(uint32_t)t << 32) | (uint32_t)b;
+      }
     }
+    block_push_list(take, t, pairs, npairs);
   }
 }
 
@@ -1901,15 +2012,15 @@ __global__ __launch_bounds__(kBlock) void k_mt_blossom_mark(BlossomState b, cons
 
 // Blossom step, applying pass: every marked vertex becomes outer with its
 // candidate's edge label (oriented to its side), FIRST pointer = that
-// candidate's join, an even level, and a place in the next frontier.  These
+// candidate's join, its odd level plus one, and a place in the next frontier.  These
 // are the reference's odd nodes in a blossom made even
 // (TransformOddNodesInBlossom, xblossom_gpu_engine_10.cu:362-566).
 __global__ __launch_bounds__(kBlock) void k_mt_blossom_apply(BlossomState b, const unsigned long long* __restrict__ cand,
                                                              const int32_t* __restrict__ join_of,
                                                              const int32_t* __restrict__ marked,
                                                              const uint32_t* __restrict__ nmarked,
-                                                             uint32_t* __restrict__ formed, int32_t next_level,
-                                                             Queue nxt, unsigned long long* __restrict__ counts) {
+                                                             uint32_t* __restrict__ formed, Queue nxt,
+                                                             unsigned long long* __restrict__ counts) {
   const uint32_t cnt = *nmarked;
   unsigned long long made = 0, blossoms = 0;
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * kBlock) {
@@ -1931,7 +2042,7 @@ __global__ __launch_bounds__(kBlock) void k_mt_blossom_apply(BlossomState b, con
     const int32_t p = side ? y : x, q = side ? x : y;
     b.elab[u] = ((unsigned long long)(uint32_t)p << 32) | (uint32_t)q;
     b.fp[u] = join_of[i];
-    b.lvl[u] = next_level;
+    b.lvl[u] += 1;  // inner (odd) -> outer (even), still above its FIRST chain
     const int32_t du = b.off[u + 1] - b.off[u];
     const unsigned long long old = atomicAdd(nxt.ctr, ((unsigned long long)(uint32_t)du << 32) | 1ull);
     nxt.v[(uint32_t)(old & 0xFFFFFFFFull)] = u;
@@ -1983,10 +2094,12 @@ __global__ __launch_bounds__(kBlock) void k_mt_claim(const int32_t* __restrict__
 }
 
 // Expansion, commit step: of a claimed matched pair (w, x) the member with
-// the lower (claim, id) becomes odd (level+1) and its mate even (level+2,
-// queued for the next level).
+// the lower (claim, id) becomes odd (its claimer's level + 1) and its mate
+// even (claimer's level + 2, queued for the next step).  Levels follow the
+// claimer because the frontier of a reused forest mixes levels; they rise
+// strictly down every tree path, which is all the blossom step relies on.
 __global__ __launch_bounds__(kBlock) void k_mt_commit(const int32_t* __restrict__ tlist,
-                                                      const uint32_t* __restrict__ tcount, int32_t next_level,
+                                                      const uint32_t* __restrict__ tcount,
                                                       const int32_t* __restrict__ mate, const int32_t* __restrict__ off,
                                                       const unsigned long long* __restrict__ claim,
                                                       int32_t* __restrict__ root, int32_t* __restrict__ lvl,
@@ -2005,14 +2118,15 @@ __global__ __launch_bounds__(kBlock) void k_mt_commit(const int32_t* __restrict_
       const unsigned long long kw = claim[w], kx = claim[x];
       odd = kw < kx || (kw == kx && w < x);
       if (odd) {
-        const int32_t rv = (int32_t)(kw >> 32);
+        const int32_t rv = (int32_t)(kw >> 32), cl = (int32_t)(kw & 0xFFFFFFFFull);
+        const int32_t lc = lvl[cl];  // an outer vertex: not written by this pass
         root[w] = rv;
-        par[w] = (int32_t)(kw & 0xFFFFFFFFull);
-        lvl[w] = next_level - 1;
+        par[w] = cl;
+        lvl[w] = lc + 1;
         root[x] = rv;
         par[x] = w;
         fp[x] = w;
-        lvl[x] = next_level;
+        lvl[x] = lc + 2;
         dx = off[x + 1] - off[x];
       }
     }
@@ -2314,18 +2428,36 @@ static int item_grid(int64_t n) {
   const int64_t g = (n + kBlock - 1) / kBlock;
   return (int)(g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g));
 }
+// Kernels that append through a block stage or end with a per-block atomic:
+// 2,048 blocks (8 waves per SIMD on 256 CUs) and a grid-stride loop, so
+// that stages flush full (one global atomic per ~1,280 entries) and the
+// counters see one atomic per block, not per 256 items.
+static int stage_grid(int64_t n) {
+  const int g = item_grid(n);
+  return g < 2048 ? g : 2048;
+}
 
 void launch_mt_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root, int32_t* lvl,
                     int32_t* par, unsigned long long* claim, int32_t* best, int32_t* acc, unsigned long long* bridge,
                     uint8_t* used, int32_t* cursor, int32_t* fp, unsigned long long* elab, Queue q) {
-  hipLaunchKernelGGL(k_mt_init, dim3(item_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, claim, best,
+  hipLaunchKernelGGL(k_mt_init, dim3(stage_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, claim, best,
                      acc, bridge, used, cursor, fp, elab, q);
+}
+void launch_mt_reuse_init(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, int32_t* root,
+                          int32_t* lvl, int32_t* par, int32_t* fp, unsigned long long* elab, unsigned long long* claim,
+                          const uint8_t* used, Queue q, unsigned long long* counts) {
+  hipLaunchKernelGGL(k_mt_reuse_init, dim3(stage_grid(n)), dim3(kBlock), 0, st, n, mate, off, root, lvl, par, fp, elab,
+                     claim, used, q, counts);
+}
+void launch_mt_reuse_clear(hipStream_t st, int32_t n, int32_t* best, int32_t* acc, unsigned long long* bridge,
+                           uint8_t* used) {
+  hipLaunchKernelGGL(k_mt_reuse_clear, dim3(item_grid(n)), dim3(kBlock), 0, st, n, best, acc, bridge, used);
 }
 void launch_mt_scan0(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* off, const int32_t* col,
                      const int32_t* lvl, const uint8_t* used, int32_t* cursor, int32_t* best,
                      unsigned long long* examined) {
   if (nroots <= 0) return;
-  hipLaunchKernelGGL(k_mt_scan0, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, off, col, lvl, used,
+  hipLaunchKernelGGL(k_mt_scan0, dim3(stage_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, off, col, lvl, used,
                      cursor, best, examined);
 }
 void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
@@ -2338,7 +2470,7 @@ void launch_mt_scan(hipStream_t st, const int32_t* off, const int32_t* col, Queu
 void launch_mt_accept(hipStream_t st, const int32_t* roots, int32_t nroots, const int32_t* best, int32_t* acc,
                       int32_t* pairs, uint32_t* npairs, unsigned long long* bridge0) {
   if (nroots <= 0) return;
-  hipLaunchKernelGGL(k_mt_accept, dim3(item_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, best, acc, pairs,
+  hipLaunchKernelGGL(k_mt_accept, dim3(stage_grid(nroots)), dim3(kBlock), 0, st, roots, nroots, best, acc, pairs,
                      npairs, bridge0);
 }
 void launch_mt_bridge(hipStream_t st, const int32_t* off, const int32_t* col, Queue q, int32_t qn, int32_t E,
@@ -2365,20 +2497,19 @@ void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Q
 void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
                        int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* lab,
                        const unsigned long long* cand, int32_t C, int32_t* join_of, int32_t* marked,
-                       uint32_t* nmarked, uint32_t* formed, int32_t next_level, Queue nxt,
-                       unsigned long long* counts, unsigned* err) {
+                       uint32_t* nmarked, uint32_t* formed, Queue nxt, unsigned long long* counts, unsigned* err) {
   if (C <= 0) return;
   const BlossomState b{off, col, n, par, lvl, fp, elab, lab};
   hipLaunchKernelGGL(k_mt_blossom_mark, dim3(item_grid(C)), dim3(kBlock), 0, st, b, cand, C, join_of, marked, nmarked,
                      err);
   // Every vertex is marked at most once per step: n bounds the list.
   hipLaunchKernelGGL(k_mt_blossom_apply, dim3(item_grid(n)), dim3(kBlock), 0, st, b, cand, join_of, marked, nmarked,
-                     formed, next_level, nxt, counts);
+                     formed, nxt, counts);
 }
 void launch_mt_compact(hipStream_t st, const int32_t* in, int32_t cnt, const int32_t* root, const uint8_t* used,
                        const int32_t* off, Queue out, int32_t* list, uint32_t* ctr32) {
   if (cnt <= 0) return;
-  hipLaunchKernelGGL(k_mt_compact, dim3(item_grid(cnt)), dim3(kBlock), 0, st, in, cnt, root, used, off, out, list,
+  hipLaunchKernelGGL(k_mt_compact, dim3(stage_grid(cnt)), dim3(kBlock), 0, st, in, cnt, root, used, off, out, list,
                      ctr32);
 }
 void launch_mt_zero_ctrs(hipStream_t st, uint32_t* a, uint32_t* b, uint32_t* c, unsigned long long* d) {
@@ -2396,11 +2527,11 @@ void launch_mt_claim(hipStream_t st, const int32_t* off, const int32_t* col, Que
                      tlist, tcount);
 }
 void launch_mt_commit(hipStream_t st, const int32_t* tlist, const uint32_t* tcount, int32_t max_t,
-                      int32_t next_level, const int32_t* mate, const int32_t* off, const unsigned long long* claim,
-                      int32_t* root, int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt) {
+                      const int32_t* mate, const int32_t* off, const unsigned long long* claim, int32_t* root,
+                      int32_t* lvl, int32_t* par, int32_t* fp, Queue nxt) {
   if (max_t <= 0) return;
-  hipLaunchKernelGGL(k_mt_commit, dim3(item_grid(max_t)), dim3(kBlock), 0, st, tlist, tcount, next_level, mate, off,
-                     claim, root, lvl, par, fp, nxt);
+  hipLaunchKernelGGL(k_mt_commit, dim3(stage_grid(max_t)), dim3(kBlock), 0, st, tlist, tcount, mate, off, claim, root,
+                     lvl, par, fp, nxt);
 }
 void launch_mt_validate(hipStream_t st, int32_t n, const int32_t* mate, const int32_t* off, const int32_t* col,
                         unsigned long long* bad) {

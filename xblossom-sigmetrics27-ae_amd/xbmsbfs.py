@@ -50,6 +50,7 @@ XB_OUT_DEVICE = 1
 XB_PROFILE = 2
 XB_NO_LEVELS = 4
 XB_FIXED_TEAMS = 8
+XB_MATCH_REGROW = 16
 
 # Every symbol include/xb/xb_capi.h declares.
 EXPORTED_SYMBOLS = (
@@ -117,9 +118,9 @@ class Opts(ctypes.Structure):
 # Kernel classes of xb_run_stats (XB_K_* in include/xb/xb_capi.h).
 KERNEL_CLASSES = ("memset", "seed", "push", "commit", "pull", "pull_hub", "clear", "emit_heavy", "emit",
                   "plain_init", "plain_push", "persistent")
-K_CLASSES = 16
+K_CLASSES = 17
 # Kernel classes of xb_match_stats (XB_K_MT_*).
-MATCH_KERNEL_CLASSES = ("",) * 12 + ("mt_scan", "mt_expand", "mt_tree", "mt_blossom")
+MATCH_KERNEL_CLASSES = ("",) * 12 + ("mt_scan", "mt_expand", "mt_tree", "mt_blossom", "mt_init")
 
 
 class RunStats(ctypes.Structure):
@@ -201,7 +202,10 @@ class MatchStats(ctypes.Structure):
                 ("blossoms", ctypes.c_int64), ("blossom_vertices", ctypes.c_int64),
                 ("phases", ctypes.c_int32), ("levels", ctypes.c_int32), ("ms_total", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double * K_CLASSES), ("kernel_bytes", ctypes.c_double * K_CLASSES),
-                ("kernel_launches", ctypes.c_int64 * K_CLASSES), ("reserved", ctypes.c_int32 * 8)]
+                ("kernel_launches", ctypes.c_int64 * K_CLASSES),
+                ("aug_path_edges", ctypes.c_int64), ("expand_edges", ctypes.c_int64),
+                ("blossom_scan_edges", ctypes.c_int64), ("reused_tree_vertices", ctypes.c_int64),
+                ("reset_tree_vertices", ctypes.c_int64), ("reserved", ctypes.c_int32 * 8)]
 
     def kernels(self) -> dict:
         return {name: {"ms": self.kernel_ms[i], "bytes": self.kernel_bytes[i],
@@ -407,10 +411,61 @@ class XBlossomGpuEngine10:
             if init.size != n:
                 raise XbError(XB_ERR_INVALID_ARG, "initial matching must have nnodes entries")
             init[init == n] = -1
-        mate, st = match(self.graph, init)
+        mate, st = match(self.graph, init, profile=True)
         self.last_match_stats = st
+        # Counters and times add up over the engine's calls, as the
+        # reference's members do (xblossom_gpu_engine_10.h:99-116).
+        acc = self.__dict__.setdefault("_match_totals", dict.fromkeys(
+            ("aug", "expand", "blossom_edges", "blossoms", "reused", "reset", "t_aug", "t_expand", "t_blossom",
+             "t_pre"), 0))
+        acc["aug"] += st.aug_path_edges
+        acc["expand"] += st.expand_edges
+        acc["blossom_edges"] += st.blossom_scan_edges
+        acc["blossoms"] += st.blossoms
+        acc["reused"] += st.reused_tree_vertices
+        acc["reset"] += st.reset_tree_vertices
+        acc["t_aug"] += st.kernel_ms[12] + st.kernel_ms[14]
+        acc["t_expand"] += st.kernel_ms[13]
+        acc["t_blossom"] += st.kernel_ms[15]
+        acc["t_pre"] += st.kernel_ms[16]
         mate[mate < 0] = n
         return mate
+
+    # Statistics and timing of apps/xb/src/include/xbsm_gpu/xblossom_gpu_engine_10.h:39-95.
+    def _tot(self, k):
+        return self.__dict__.get("_match_totals", {}).get(k, 0)
+
+    def GetTimingSummary(self):
+        """(augmenting_path, expand, blossom, total) device time in SECONDS,
+        as the reference returns them (nanoseconds / 1e9 under _ms names)."""
+        a, e, b = self._tot("t_aug") / 1e3, self._tot("t_expand") / 1e3, self._tot("t_blossom") / 1e3
+        return a, e, b, a + e + b
+
+    def PrintTimingSummary(self):
+        a, e, b, t = self.GetTimingSummary()
+        print(f"XBlossomGpuEngine10(MI355X: Reuse+ParBlossom+MinKey) Timing Summary:\nAugmenting Path: {a}\n"
+              f"Expand: {e}\nBlossom: {b}\nPreprocessing Time: {self._tot('t_pre') / 1e3}\nTotal Time: {t}")
+
+    def NumEdgesProcessed(self) -> int:
+        return self._tot("aug") + self._tot("expand") + self._tot("blossom_edges")
+
+    def NumAugmentingPathEdgesProcessed(self) -> int:
+        return self._tot("aug")
+
+    def NumExpandEdgesProcessed(self) -> int:
+        return self._tot("expand")
+
+    def NumBlossomEdgesProcessed(self) -> int:
+        return self._tot("blossom_edges")
+
+    def NumBlossoms(self) -> int:
+        return self._tot("blossoms")
+
+    def NumReusedTreeNodes(self) -> int:
+        return self._tot("reused")
+
+    def NumResetTreeNodes(self) -> int:
+        return self._tot("reset")
 
 
 class DeviceBuffer:
@@ -584,16 +639,19 @@ def autotune(graph: HGraph, sample, **base):
     return {k: int(getattr(best, k)) for k in TUNABLES}, rep
 
 
-def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False, profile: bool = False):
+def match(graph: HGraph, init_mate=None, mate_out=None, *, device: bool = False, profile: bool = False,
+          regrow: bool = False):
     """xb_match_run: (mate, MatchStats), -1 for exposed vertices.
 
     Host mode: ``init_mate`` is None or n int32 entries and a new numpy mate
     array is returned.  Device mode: ``init_mate`` (or None) and ``mate_out``
-    are device addresses of n int32 entries."""
+    are device addresses of n int32 entries.  ``regrow`` resets the whole
+    forest every phase (XB_MATCH_REGROW) instead of keeping the trees of
+    still-exposed roots."""
     lib = load_engine()
     n = graph.nnodes
     st = MatchStats()
-    prof = XB_PROFILE if profile else 0
+    prof = (XB_PROFILE if profile else 0) | (XB_MATCH_REGROW if regrow else 0)
     if device:
         if mate_out is None:
             raise XbError(XB_ERR_INVALID_ARG, "device mode needs mate_out")
