@@ -154,6 +154,7 @@ struct Workspace {
     dfree(bp.front[1]);
     dfree(bp.nextw);
     dfree(bp.planes);
+    dfree(bp.planes_hi);
     dfree(bp.tflag);
     dfree(bp.tlist);
     bp = xb::BPState{};
@@ -708,7 +709,8 @@ void ensure_bp(const xb_graph* g, Workspace& w, int T) {
   w.bp.front[0] = dalloc<uint64_t>(nw);
   w.bp.front[1] = dalloc<uint64_t>(nw);
   w.bp.nextw = dalloc<uint64_t>(nw);
-  w.bp.planes = dalloc<uint64_t>(nw * 8);
+  w.bp.planes = dalloc<uint64_t>(nw * 4);
+  w.bp.planes_hi = nullptr;  // allocated when a tile first reaches level 15
   w.bp.tflag = dalloc<uint32_t>((size_t)g->n + 1);
   w.bp.tlist = dalloc<int32_t>((size_t)g->n + 1);
   w.bp.tcount = w.tcount;
@@ -950,8 +952,9 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
       break;
     }
     if (code >= 16 && !hi_planes_clear) {
+      if (!s.planes_hi) s.planes_hi = dalloc<uint64_t>((size_t)g->n * (size_t)w.T * 4 + 4);
       prof.begin(XB_K_MEMSET);
-      HIPCHK(hipMemsetAsync(s.planes + half, 0, sizeof(uint64_t) * half, w.st));
+      HIPCHK(hipMemsetAsync(s.planes_hi, 0, sizeof(uint64_t) * half, w.st));
       prof.end();
       prof.add(XB_K_MEMSET, 8.0 * half);
       hi_planes_clear = true;
@@ -1453,7 +1456,8 @@ size_t carve_workspace(const xb_graph* g, int T, char* base, Workspace* w) {
     ws.bp.front[0] = (uint64_t*)take(sizeof(uint64_t) * nw);
     ws.bp.front[1] = (uint64_t*)take(sizeof(uint64_t) * nw);
     ws.bp.nextw = (uint64_t*)take(sizeof(uint64_t) * nw);
-    ws.bp.planes = (uint64_t*)take(sizeof(uint64_t) * nw * 8);
+    ws.bp.planes = (uint64_t*)take(sizeof(uint64_t) * nw * 4);
+    ws.bp.planes_hi = (uint64_t*)take(sizeof(uint64_t) * nw * 4);
     ws.bp.tflag = (uint32_t*)take(sizeof(uint32_t) * n1);
     ws.bp.tlist = (int32_t*)take(sizeof(int32_t) * n1);
     ws.bp.tcount = ws.tcount;

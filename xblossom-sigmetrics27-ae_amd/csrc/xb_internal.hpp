@@ -20,22 +20,25 @@ namespace xb {
 // hubs; a parent found there is translated back with orig_of.
 //
 // Bit-parallel tile state.  seen, front and nextw are vertex-major with T
-// 64-bit source words per vertex ([v*T + t], engine ids).  planes hold 8
-// bit-sliced planes of the level code (code = level + 1, 0 = unreached), bit
-// b of plane k = bit k of code(source 64t+b, v), in word groups of
+// 64-bit source words per vertex ([v*T + t], engine ids).  Level codes
+// (code = level + 1, 0 = unreached) are 8 bit-sliced planes, bit b of plane
+// k = bit k of code(source 64t+b, v), in word groups of
 // W = min(T, XB_PLANE_GROUP_MAX) words (1 by default: word-major) and two
-// halves:
+// arrays:
 //   plane k < 4:   planes[((t/W * n + v) * W + t%W) * 4 + k]
-//   plane k >= 4:  planes[T*n*4 + ((t/W * n + v) * W + t%W) * 4 + k - 4]
+//   plane k >= 4:  planes_hi[((t/W * n + v) * W + t%W) * 4 + k - 4]
 // so the first 4 planes of a (vertex, word) are 32 contiguous bytes (codes up
 // to 15, i.e. levels <= 14), one gather per in-neighbour in the emit, and one
-// emit pass (one word) reads a single n*32-byte region.
+// emit pass (one word) reads a single n*32-byte region.  planes_hi serves
+// only tiles deeper than 14 levels; the host engine allocates it when a tile
+// first needs it.
 struct BPState {
   int64_t n;            // vertices (plane addressing)
   uint64_t* seen;
   uint64_t* front[2];
   uint64_t* nextw;
   uint64_t* planes;
+  uint64_t* planes_hi;  // planes 4-7 (null until a tile reaches level 15 on the host engine)
   uint32_t* tflag;
   int32_t* tlist;       // vertices whose next words were OR-ed atomically
   uint32_t* tcount;     // device count of tlist

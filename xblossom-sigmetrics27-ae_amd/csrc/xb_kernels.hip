@@ -442,7 +442,7 @@ __device__ __forceinline__ uint64_t* plane_lo(const BPState& s, int64_t v, int t
 }
 template <int T>
 __device__ __forceinline__ uint64_t* plane_hi(const BPState& s, int64_t v, int t) {
-  return s.planes + (int64_t)T * s.n * 4 + plane_index<T>(s, v, t);
+  return s.planes_hi + plane_index<T>(s, v, t);
 }
 
 // OR `bits` into the planes of (v, t) whose bit is set in `code`.  One lane
@@ -1247,6 +1247,7 @@ __device__ __forceinline__ BPState uni(const BPState& s) {
   u.front[1] = uni(s.front[1]);
   u.nextw = uni(s.nextw);
   u.planes = uni(s.planes);
+  u.planes_hi = uni(s.planes_hi);
   u.tflag = uni(s.tflag);
   u.tlist = uni(s.tlist);
   u.tcount = uni(s.tcount);
@@ -1334,6 +1335,8 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   // Tile set-up: seen and planes cleared, heavy parents unresolved; fronts,
   // next words and touched flags are clean between tiles by construction.
   nl_fill_u64(g, s.seen, nw, 0ull);
+  // The team's plane block holds both arrays, planes 4-7 after this tile's 0-3.
+  s.planes_hi = s.planes + nw * 4;
   nl_fill_u64(g, s.planes, nw * 8, 0ull);
   if (gv.n_heavy > 0) nl_fill_u32(g, s.hpar, (int64_t)gv.n_heavy * T * 64, 0xFFFFFFFFu);
   if (g.bid == 0 && threadIdx.x == 0) {
