@@ -6,9 +6,12 @@ batch of the workload (default: the LiveJournal-shaped Chung-Lu graph,
 n = 4.85M, m ~ 69M, 1,024 degree-weighted sources), with the graph and the
 S x n level/parent output tables resident in HBM.  The plain per-source engine
 (one top-down push launch sequence per source) runs the same batch once, and
-its tables must match the batched ones bit for bit.  The CPU baseline is the
-oracle (a per-source BFS port with the same output rule) on a bounded sample of
-the same sources, on the host's cores.
+its tables must match the batched ones bit for bit.  The CPU baseline times
+the multi-threaded CPU restatements of the same path (bit-parallel and
+per-source, both with the |E|/20 push/pull switch and the same output rule) on
+bounded samples of the same sources, on every usable host core and on one.
+The line also carries a serving sub-object: the web-crawl-shaped graph
+resident on the device, queries at 2,000/s offered, and the knee.
 
 Prints one JSON line.  GTEPS = sum over sources of the out-degrees of the
 vertices each source reaches, divided by wall time.
@@ -87,26 +90,89 @@ def traffic_from_profiles(workload: str, kernel: str):
     return float(v) if v else None
 
 
-def cpu_baseline(g, src, seconds: float, threads: int):
-    """Times the oracle (per-source BFS port) on chunks of sources until `seconds` elapse."""
+def host_cores():
+    """(physical cores, this process's CPU share, threads used).  Physical
+    cores follow the reference's policy, cores per socket x sockets
+    (scripts/4-expr/common.sh:164-184), counted here as distinct (physical
+    id, core id) pairs of /proc/cpuinfo.  The threads used are capped by the
+    CPUs this process may run on and by OMP_NUM_THREADS when it is set."""
+    pairs, pid = set(), None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "physical id":
+                    pid = v.strip()
+                elif k == "core id":
+                    pairs.add((pid, v.strip()))
+    except OSError:
+        pass
+    physical = len(pairs) or (os.cpu_count() or 1)
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    used = min(physical, share)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        used = min(used, int(omp))
+    return physical, share, max(1, used)
+
+
+def trimmed_mean(xs):
+    """Mean with the minimum and maximum dropped (the reference's timing
+    rule, apps/xb/src/include/common/timing_guard.h:43-71)."""
+    xs = sorted(xs)
+    if len(xs) > 2:
+        xs = xs[1:-1]
+    return sum(xs) / len(xs)
+
+
+def cpu_baseline(g, src, rounds: int):
+    """Times the multi-threaded CPU restatements (oracle/msbfs_cpu_baseline.c)
+    on bounded samples of the workload's sources: the bit-parallel mode (64
+    sources per word, levels parallel over vertices in chunks of 64) and the
+    per-source direction-optimizing mode (threads over sources), each with
+    every usable core and with one thread, `rounds` rounds each on different
+    sources, GTEPS as the mean with the minimum and maximum dropped."""
     import oracle
 
-    done, edges, t = 0, 0, 0.0
-    chunk = max(threads, 1)
-    while done < src.size and t < seconds:
-        part = src[done:done + chunk]
-        t0 = time.perf_counter()
-        _, _, e = oracle.msbfs(g.row_offsets, g.col_indices, part, threads=threads)
-        t += time.perf_counter() - t0
-        edges += e
-        done += part.size
+    physical, share, threads = host_cores()
+    log(f"cpu baseline: {physical} physical cores (cores/socket x sockets), {share} CPUs in this process's "
+        f"share, {threads} threads used")
+    plan = {"bitpar": ((threads, 64), (1, 64)), "dir": ((threads, threads), (1, 1))}
+    modes = {}
+    total_s = 0.0
+    for mode, cfgs in plan.items():
+        res = {}
+        for t, S in cfgs:
+            vals = []
+            for r in range(rounds):
+                b0 = (r * S) % max(1, src.size - S + 1)
+                part = src[b0:b0 + S]
+                t0 = time.perf_counter()
+                _, _, e = oracle.msbfs_cpu(g.row_offsets, g.col_indices, part, threads=t, mode=mode)
+                dt = time.perf_counter() - t0
+                total_s += dt
+                vals.append(e / dt / 1e9)
+            res["all_cores" if t == threads else "one_thread"] = {
+                "GTEPS": round(trimmed_mean(vals), 4), "threads": t, "sources_per_round": int(S),
+                "rounds_GTEPS": [round(v, 4) for v in vals]}
+        modes[mode] = res
+    best = max(("bitpar", "dir"), key=lambda m: modes[m]["all_cores"]["GTEPS"])
     return {
-        "value": round(edges / t / 1e9, 4) if t > 0 else None,
+        "value": modes[best]["all_cores"]["GTEPS"],
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"{done} of {src.size} sources of the same workload, per-source BFS with lowest-id parents, "
-                  f"{threads} threads, {t:.1f} s",
+        "mode": best,
+        "host_physical_cores": physical,
+        "cpu_share": share,
+        "rounds": rounds,
+        "statistic": "mean of the rounds with the minimum and maximum dropped",
+        "modes": modes,
+        "sample": (f"{rounds} rounds per mode and thread count on sources of the same workload: bit-parallel "
+                   f"64 sources per round, per-source {threads} (all cores) or 1 (one thread) per round; "
+                   f"levels, lowest-id parents and the |E|/20 push/pull switch as the engine; "
+                   f"{total_s:.1f} s in all"),
     }
 
 
@@ -206,41 +272,53 @@ def bench_uneven(args) -> int:  # noqa: C901
     return 0
 
 
-def bench_serving(args) -> int:
-    """Serving: directed web-crawl-shaped R-MAT (n = 20M, ~300M edges, a=.6
-    b=.15 c=.15, seed 7) resident on the device; queries arrive by a Poisson
-    process (rate --rate, seed 7), each with 1-256 sources drawn log-uniformly
-    (seed 8) among vertices with out-edges.  Latency runs from a query's
-    scheduled arrival to the end of its last tile; aggregate GTEPS is the
-    traversed edges of all queries over the span from the first arrival to
-    the last completion."""
-    import queue
-    import threading
+SERVING_DESC = ("serving: directed web-crawl-shaped R-MAT n=20M (a=.6 b=.15 c=.15, seed 7), queries of 1-256 "
+                "sources (log-uniform, seed 8), Poisson arrivals (seed 7)")
 
+
+def serving_setup(slots: int, words_per_tile: int):
+    """The serving workload's graph (directed R-MAT, n = 20M, ~300M edges,
+    a=.6 b=.15 c=.15, seed 7) resident on the device, a server over it, and
+    the candidate source vertices (those with out-edges)."""
     import numpy as np
 
     import xbmsbfs as xb
 
     t0 = time.perf_counter()
     g = xb.gen_rmat_capped(25, 20_000_000, 386_000_000, 0.6, 0.15, 0.15, seed=7)
-    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
     hg = xb.HGraph(g.row_offsets, g.col_indices, directed=True)
-    rng_a = np.random.default_rng(7)
-    rng_s = np.random.default_rng(8)
+    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
     cand = np.flatnonzero(g.degrees() > 0).astype(np.int32)
-    N = args.requests
-    arrivals = np.cumsum(rng_a.exponential(1.0 / args.rate, size=N))
-    reqs = []
-    for _ in range(N + args.warmup):
-        S = max(1, min(256, int(np.floor(np.exp(rng_s.uniform(0.0, np.log(257.0)))))))
-        reqs.append(rng_s.choice(cand, size=S, replace=True).astype(np.int32))
-    warm, reqs = reqs[: args.warmup], reqs[args.warmup:]
-    tun = {"words_per_tile": args.words_per_tile} if args.words_per_tile else {}
-    srv = xb.Server(hg, slots=args.slots, **tun)
-    for s_ in warm:
-        rid = srv.submit(s_)
-        srv.wait(rid)
-        srv.release(rid)
+    tun = {"words_per_tile": words_per_tile} if words_per_tile else {}
+    srv = xb.Server(hg, slots=slots, **tun)
+    return g, hg, srv, cand
+
+
+def serving_requests(cand, count: int, seed: int):
+    """Source sets of 1-256 vertices, log-uniform in size, uniform over cand."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        S = max(1, min(256, int(np.floor(np.exp(rng.uniform(0.0, np.log(257.0)))))))
+        out.append(rng.choice(cand, size=S, replace=True).astype(np.int32))
+    return out
+
+
+def serve_once(srv, n: int, reqs, rate: float, seed: int) -> dict:
+    """Submits reqs at Poisson arrivals of `rate` per second (seed), waits for
+    all, and returns latency and throughput figures.  Latency runs from a
+    query's scheduled arrival to the end of its last tile; service time from
+    its first tile's start; aggregate GTEPS is the traversed edges of all
+    queries over the span from the first arrival to the last completion."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    N = len(reqs)
+    arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, size=N))
     base = srv.counters()
     infos = [None] * N
     todo = queue.Queue()
@@ -258,21 +336,17 @@ def bench_serving(args) -> int:
     workers = [threading.Thread(target=collect) for _ in range(16)]
     for w in workers:
         w.start()
-    sub_us = np.zeros(N)
     start = time.perf_counter()
-    # Server clock of the first submit anchors the arrival schedule.
     for i in range(N):
         delay = start + arrivals[i] - arrivals[0] - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        rid = srv.submit(reqs[i])
-        todo.put((i, rid))
+        todo.put((i, srv.submit(reqs[i])))
     for _ in workers:
         todo.put(None)
     for w in workers:
         w.join()
     c = srv.counters()
-    srv.close()
     t_sub = np.array([x[0] for x in infos])
     sched = t_sub[0] + (arrivals - arrivals[0]) * 1e6  # scheduled arrivals on the server clock
     done = np.array([x[3] for x in infos])
@@ -283,41 +357,113 @@ def bench_serving(args) -> int:
     wait_ms = (admit - t_sub) / 1e3
     edges = int(sum(x[4] for x in infos))
     span_s = (done.max() - sched.min()) / 1e6
-    gteps = edges / span_s / 1e9
-    out_bytes = sum(8 * x[5] * g.n for x in infos)
-    line = {
-        "metric": METRIC,
-        "value": round(gteps, 3),
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": N,
-        "warmup": args.warmup,
-        "ms_per_step": round(span_s * 1e3 / N, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "serving: directed R-MAT n=20M (a=.6 b=.15 c=.15, seed 7), Poisson arrivals "
-                               f"{args.rate:g}/s (seed 7), 1-256 sources per query (seed 8)",
-                   "n": g.n, "m": g.m, "queries": N, "slots": args.slots, "words_per_tile": args.words_per_tile or 8,
-                   "sources": int(sum(x[5] for x in infos))},
+    out_bytes = sum(8 * x[5] * n for x in infos)
+    tiles = int(c.tiles - base.tiles)
+    return {
+        "offered_per_s": round(rate, 1),
+        "queries": N,
+        "sources": int(sum(x[5] for x in infos)),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
         "service_p50_ms": round(float(np.percentile(svc_ms, 50)), 2),
         "service_p99_ms": round(float(np.percentile(svc_ms, 99)), 2),
         "admission_wait_p50_ms": round(float(np.percentile(wait_ms, 50)), 2),
         "queries_per_s": round(N / span_s, 1),
-        "offered_per_s": args.rate,
+        "gteps": round(edges / span_s / 1e9, 1),
         "output_TBps": round(out_bytes / span_s / 1e12, 3),
-        "tiles": int(c.tiles - base.tiles),
-        "rows_per_tile": round((c.tile_rows - base.tile_rows) / max(1, c.tiles - base.tiles), 1),
+        "tiles": tiles,
+        "rows_per_tile": round((c.tile_rows - base.tile_rows) / max(1, tiles), 1),
         "peak_table_GB": round(c.peak_used_bytes / 1e9, 1),
-        "budget_GB": round(c.budget_bytes / 1e9, 1),
+        "span_ms": round(span_s * 1e3, 1),
+    }
+
+
+def serving_summary(slots: int, words_per_tile: int, requests: int, knee_requests: int) -> dict:
+    """The serving sub-object of the default bench line: the configured 2,000
+    queries/s offered rate (far past one GPU's capacity: latency is admission
+    wait), the capacity that run measures, and a sweep of lower offered rates
+    for the knee -- the highest rate whose p99 stays within 2x the service
+    p99 (the latency of a query that never waits for admission)."""
+    g, hg, srv, cand = serving_setup(slots, words_per_tile)
+    warm = serving_requests(cand, 3, seed=9)
+    serve_once(srv, g.n, warm, 2000.0, seed=9)
+    at_2k = serve_once(srv, g.n, serving_requests(cand, requests, seed=8), 2000.0, seed=7)
+    cap = at_2k["queries_per_s"]
+    log(f"serving at 2000/s: {at_2k}")
+    sweep = []
+    for k, frac in enumerate((0.5, 0.65, 0.8, 0.95)):
+        r = serve_once(srv, g.n, serving_requests(cand, knee_requests, seed=20 + k), frac * cap, seed=30 + k)
+        r["within_2x_service_p99"] = r["p99_ms"] <= 2.0 * r["service_p99_ms"]
+        sweep.append(r)
+        log(f"serving at {frac:.2f} x capacity: {r}")
+    ok = [r for r in sweep if r["within_2x_service_p99"]]
+    knee = max(ok, key=lambda r: r["offered_per_s"]) if ok else None
+    srv.close()
+    hg.close()
+    return {"workload": SERVING_DESC, "n": g.n, "m": g.m, "slots": slots,
+            "words_per_tile": words_per_tile or 8, "at_2000_per_s": at_2k, "capacity_queries_per_s": cap,
+            "knee": ({"offered_per_s": knee["offered_per_s"], "p50_ms": knee["p50_ms"], "p99_ms": knee["p99_ms"],
+                      "service_p99_ms": knee["service_p99_ms"], "gteps": knee["gteps"]} if knee else None),
+            "sweep": sweep}
+
+
+def bench_serving(args) -> int:
+    """Serving on its own line: the web-crawl-shaped graph resident on the
+    device, --requests queries at Poisson arrivals of --rate per second."""
+    g, hg, srv, cand = serving_setup(args.slots, args.words_per_tile)
+    serve_once(srv, g.n, serving_requests(cand, args.warmup, seed=9), args.rate, seed=9)
+    r = serve_once(srv, g.n, serving_requests(cand, args.requests, seed=8), args.rate, seed=7)
+    srv.close()
+    line = {
+        "metric": METRIC,
+        "value": r["gteps"],
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": r["queries"],
+        "warmup": args.warmup,
+        "ms_per_step": round(r["span_ms"] / r["queries"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": SERVING_DESC + f", {args.rate:g}/s offered", "n": g.n, "m": g.m,
+                   "queries": r["queries"], "slots": args.slots, "words_per_tile": args.words_per_tile or 8,
+                   "sources": r["sources"]},
+        **{k: r[k] for k in ("p50_ms", "p99_ms", "service_p50_ms", "service_p99_ms", "admission_wait_p50_ms",
+                             "queries_per_s", "offered_per_s", "output_TBps", "tiles", "rows_per_tile",
+                             "peak_table_GB")},
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_calibrate(args) -> int:
+    """Known-traffic kernels for the PMC byte counters: a linear device copy of
+    --copy-calibration-mib MiB (that many bytes read and written) and a gather
+    of random 128-byte rows (int32 x 32, line-aligned) from a 4 GiB table into
+    a dense output (rows x 128 bytes read and written, plus 8 bytes of index
+    per row).  Prints the bytes each kernel must move."""
+    import torch
+
+    mib = args.copy_calibration_mib or 1024
+    a = torch.ones(mib << 18, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    table = torch.zeros((4 << 30) // 128, 32, dtype=torch.int32, device="cuda")
+    rows = (mib << 20) // 128
+    idx = torch.randint(0, table.shape[0], (rows,), device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    torch.cuda.synchronize()
+    out = torch.index_select(table, 0, idx)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy": {"read_bytes": mib << 20, "write_bytes": mib << 20},
+                      "gather_128B_rows": {"rows": rows, "read_bytes": rows * 128 + rows * 8,
+                                           "write_bytes": rows * 128, "table_bytes": table.numel() * 4},
+                      "checks": [int(out.shape[0])]}), flush=True)
     return 0
 
 
@@ -441,7 +587,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="livejournal", choices=["livejournal", "amazon", "rmat16", "uneven", "serving",
-                                                                 "match"])
+                                                                 "match", "calibrate"])
     ap.add_argument("--match-graph", default="livejournal", choices=["livejournal", "gnp"],
                     help="match: the LiveJournal-shaped graph, or a blossom-heavy G(n, p)")
     ap.add_argument("--match-regrow", action="store_true",
@@ -458,9 +604,15 @@ def main() -> int:
                     help="skip the autotuner (words per tile, alpha, pull cut-offs) before the timed region")
     ap.add_argument("--words-per-tile", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-rounds", type=int, default=5, help="CPU baseline: rounds per mode and thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving sub-object of the default line (web-crawl graph, 2k/s and the knee)")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="default line: queries of the 2,000/s serving run")
+    ap.add_argument("--knee-requests", type=int, default=100,
+                    help="default line: queries per offered rate of the knee sweep")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -482,6 +634,8 @@ def main() -> int:
         return bench_match(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "calibrate":
+        return bench_calibrate(args)
 
     t0 = time.perf_counter()
     g, src, desc = make_workload(args.workload)
@@ -632,9 +786,18 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(g, src, args.cpu_seconds, threads)
+        cpu = cpu_baseline(g, src, args.cpu_rounds)
         log(f"cpu baseline: {cpu}")
+
+    # Serving on the web-crawl-shaped graph: the C3 tables are freed first.
+    serving = None
+    if not args.no_serving and args.workload == "livejournal":
+        if batch is not None:
+            batch.close()
+        del lv, pa
+        torch.cuda.empty_cache()
+        hg.close()
+        serving = serving_summary(args.slots, 0, args.serving_requests, args.knee_requests)
 
     line = {
         "metric": METRIC,
@@ -669,6 +832,7 @@ def main() -> int:
         "step_traffic": step_traffic,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

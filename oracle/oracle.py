@@ -41,6 +41,9 @@ def load():
         lib.oracle_msbfs.restype = I64
         lib.oracle_msbfs_bitparallel.argtypes = [I32, _P32, _P32, _P32, I32, _P32, _P32]
         lib.oracle_msbfs_bitparallel.restype = I64
+        for fn in ("oracle_cpu_msbfs_dir", "oracle_cpu_msbfs_bitpar"):
+            getattr(lib, fn).argtypes = [I32, _P32, _P32, _P32, _P32, _P32, I32, _P32, _P32, I32]
+            getattr(lib, fn).restype = I64
         lib.oracle_edge_map.argtypes = [_P32, I64, _P32, _P32]
         lib.oracle_edge_map.restype = I64
         lib.oracle_exclusive_scan.argtypes = [_P32, I64, _P64]
@@ -78,6 +81,45 @@ def msbfs(row_offsets, col_indices, sources, threads: int = 1, bitparallel: bool
         e = lib.oracle_msbfs_bitparallel(n, _p(ro), _p(ci), _p(src), S, _p(lv), _p(pa))
     else:
         e = lib.oracle_msbfs(n, _p(ro), _p(ci), _p(src), S, _p(lv), _p(pa), threads)
+    return lv, pa, int(e)
+
+
+def transpose(row_offsets, col_indices):
+    """In-edge CSR of a directed CSR: (in_offsets, in_cols), each in-list in
+    ascending source order."""
+    ro = np.asarray(row_offsets, dtype=np.int64)
+    ci = np.asarray(col_indices, dtype=np.int64)
+    n = ro.size - 1
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(ro))
+    order = np.lexsort((src, ci))
+    in_off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(in_off, ci + 1, 1)
+    return np.cumsum(in_off).astype(np.int32), src[order].astype(np.int32)
+
+
+def msbfs_cpu(row_offsets, col_indices, sources, threads: int, mode: str = "bitpar", transposed=None):
+    """Multi-threaded CPU restatements (the timed CPU baseline): "dir" = one
+    direction-optimizing BFS per source, threads over sources; "bitpar" = 64
+    sources per word, each level parallel over vertices in chunks of 64.
+    transposed: (in_offsets, in_cols) of a directed graph (None: symmetric).
+    Returns (levels, parents, traversed_edges), as msbfs()."""
+    lib = load()
+    ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
+    ci = np.ascontiguousarray(col_indices, dtype=np.int32)
+    if ci.size == 0:
+        ci = np.zeros(1, dtype=np.int32)
+    io, ic = (ro, ci) if transposed is None else (np.ascontiguousarray(transposed[0], dtype=np.int32),
+                                                   np.ascontiguousarray(transposed[1], dtype=np.int32))
+    if ic.size == 0:
+        ic = np.zeros(1, dtype=np.int32)
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    n, S = ro.size - 1, src.size
+    lv = np.empty((S, n), dtype=np.int32)
+    pa = np.empty((S, n), dtype=np.int32)
+    if S == 0:
+        return lv, pa, 0
+    fn = {"dir": lib.oracle_cpu_msbfs_dir, "bitpar": lib.oracle_cpu_msbfs_bitpar}[mode]
+    e = fn(n, _p(ro), _p(ci), _p(io), _p(ic), _p(src), S, _p(lv), _p(pa), max(1, threads))
     return lv, pa, int(e)
 
 

@@ -194,3 +194,66 @@ def test_server_refuses_tiles_after_layout_rebuild(web):
     assert ei.value.code == xb.XB_ERR_INVALID_ARG
     srv.release(rid)
     srv.close()
+
+
+@pytest.mark.timeout(900)
+def test_web_crawl_full_size():
+    """The serving workload at its configured size: a directed R-MAT web-crawl
+    graph (n = 20M, ~300M edges, a=.6 b=.15 c=.15, seed 7) resident on the
+    device, queries of 1-256 sources (log-uniform, seed 8) arriving by a
+    Poisson process at 2,000/s (seed 7).  Every request's tables, computed in
+    tiles shared with other requests, equal the tables xb_msbfs_run writes for
+    the request alone (device checksums over all S x n entries) and its
+    traversed-edge count; sampled rows of three requests equal the oracle, and
+    one row satisfies the definition invariants."""
+    import time
+
+    g = xb.gen_rmat_capped(25, 20_000_000, 386_000_000, 0.6, 0.15, 0.15, seed=7)
+    assert g.n == 20_000_000 and 250_000_000 < g.m < 350_000_000
+    hg = xb.HGraph(g.row_offsets, g.col_indices, directed=True)
+    rng_a = np.random.default_rng(7)
+    rng_s = np.random.default_rng(8)
+    cand = np.flatnonzero(g.degrees() > 0).astype(np.int32)
+    N = 24
+    arrivals = np.cumsum(rng_a.exponential(1.0 / 2000.0, size=N))
+    reqs = []
+    for _ in range(N):
+        S = max(1, min(256, int(np.floor(np.exp(rng_s.uniform(0.0, np.log(257.0)))))))
+        reqs.append(rng_s.choice(cand, size=S, replace=True).astype(np.int32))
+    srv = xb.Server(hg, slots=2, words_per_tile=8)
+    t0 = time.perf_counter()
+    ids = []
+    for i in range(N):
+        delay = t0 + arrivals[i] - arrivals[0] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(reqs[i]))
+    mixed = []
+    for i, rid in enumerate(ids):
+        info = srv.wait(rid, timeout_ms=300_000)
+        assert info.status == xb.XB_OK and info.S == reqs[i].size and info.n == g.n
+        cells = info.S * info.n
+        mixed.append((xb.device_checksum(info.levels, cells), xb.device_checksum(info.parents, cells), info.edges))
+        srv.release(rid)
+    c = srv.counters()
+    assert c.completed == N and c.failed == 0 and c.used_bytes == 0
+    assert c.tiles < N  # rows of several requests shared tiles
+    srv.close()
+    eng = xb.XBlossomGpuEngine10(hg)
+    checked = {0: [0], N // 2: [0], N - 1: [0]}
+    for i, s in enumerate(reqs):
+        lv = xb.DeviceBuffer((s.size, g.n))
+        pa = xb.DeviceBuffer((s.size, g.n))
+        eng.MultiSourceBFS(s, lv.ptr, pa.ptr, mode=xb.XB_MODE_BATCHED, device_out=True, words_per_tile=8)
+        assert (lv.checksum(), pa.checksum()) == mixed[i][:2], f"request {i}: mixed != alone"
+        assert eng.CountTraversed(lv.ptr, s.size) == mixed[i][2], f"request {i}: traversed edges"
+        if i in checked:
+            rows = sorted({0, s.size - 1})
+            lr, pr, _ = oracle.msbfs(g.row_offsets, g.col_indices, s[rows], threads=16)
+            got_l = np.concatenate([lv.rows(r, r + 1) for r in rows])
+            got_p = np.concatenate([pa.rows(r, r + 1) for r in rows])
+            assert np.array_equal(got_l, lr) and np.array_equal(got_p, pr), f"request {i}: oracle rows"
+            if i == 0:
+                tg.check_invariants(g.row_offsets, g.col_indices, s[rows[:1]], got_l[:1], got_p[:1])
+        lv.free()
+        pa.free()
