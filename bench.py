@@ -64,29 +64,55 @@ def make_workload(name: str):
     return g, src, desc
 
 
-def profile_values(workload: str, kernel: str) -> dict:
+# Engine sources whose hash keys a committed PMC summary: a summary measured
+# on other kernels, or on another graph, is not used.
+ENGINE_SOURCES = ("xblossom-sigmetrics27-ae_amd/csrc/xb_kernels.hip", "xblossom-sigmetrics27-ae_amd/csrc/xb_host.cpp",
+                  "xblossom-sigmetrics27-ae_amd/csrc/xb_internal.hpp")
+TRAFFIC_SOURCE = ("rocprofv3 PMC, L2-to-fabric requests by size (TCC_EA0_RDREQ_32B/64B/128B x 32/64/128 B read, "
+                  "TCC_EA0_WRREQ_64B x 64 B + other writes x 32 B), one pass per counter set; a 1 GiB copy reads "
+                  "exactly 1.00x its size with these counters (FETCH_SIZE books 128-B requests as 64 B: 0.50x). "
+                  "Infinity Cache hits are included, so this is an upper bound on HBM bytes")
+
+
+def engine_source_hash() -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in ENGINE_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_values(workload: str, kernel: str, graph_checksum: str = None) -> dict:
     """The key=value lines of a committed rocprofv3 PMC summary,
-    profiles/pmc_<kernel>_<workload>.txt, if present: traffic_bytes_per_launch
-    (FETCH_SIZE + WRITE_SIZE per dispatch of `kernel`, scaled by the
-    copy-kernel calibration factors), and for the whole step
-    step_traffic_bytes (the same over every engine kernel of one step) with
-    the configuration it was profiled in (profiled_engine,
+    profiles/pmc_<kernel>_<workload>.txt, when it was measured on the engine
+    sources of this run (source_hash) and, if given, on the same graph
+    (graph_checksum); otherwise {} with the reason under "_stale".  Keys:
+    traffic_bytes_per_launch (read + write bytes per dispatch of `kernel`),
+    step_traffic_bytes (every engine kernel and fill of one step), and the
+    configuration it was measured in (profiled_engine,
     profiled_words_per_tile)."""
     sym = KERNEL_SYMBOL.get(kernel, kernel)
     path = os.path.join(PROFILES, f"pmc_{sym}_{workload}.txt")
     vals = {}
-    if os.path.exists(path):
-        with open(path) as f:
-            for line in f:
-                key, eq, val = line.strip().partition("=")
-                if eq and key.isidentifier():
-                    vals[key] = val
+    if not os.path.exists(path):
+        return {"_stale": f"no {os.path.basename(path)}"}
+    with open(path) as f:
+        for line in f:
+            key, eq, val = line.strip().partition("=")
+            if eq and key.isidentifier():
+                vals[key] = val
+    if vals.get("source_hash") != engine_source_hash():
+        return {"_stale": "profiled on other engine sources"}
+    if graph_checksum and vals.get("graph_checksum") not in (None, graph_checksum):
+        return {"_stale": "profiled on another graph"}
     return vals
 
 
-def traffic_from_profiles(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` from the committed PMC summary, or None."""
-    v = profile_values(workload, kernel).get("traffic_bytes_per_launch")
+def traffic_from_profiles(workload: str, kernel: str, graph_checksum: str = None):
+    """Measured bytes per launch of `kernel` from the committed PMC summary, or None."""
+    v = profile_values(workload, kernel, graph_checksum).get("traffic_bytes_per_launch")
     return float(v) if v else None
 
 
@@ -748,7 +774,9 @@ def main() -> int:
     avg_ms = agg_ms[dom] / agg_launch[dom]
     bytes_per_launch = agg_bytes[dom] / agg_launch[dom]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(args.workload, dom)
+    gsum = f"{g.checksum():016x}"
+    prof_vals = profile_values(args.workload, dom, gsum)
+    traffic = float(prof_vals["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in prof_vals else None
     roofline = {
         "bound": "hbm",
         "kernel": KERNEL_SYMBOL.get(dom, dom),
@@ -757,10 +785,12 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        # The same launch's measured HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-        # calibrated) over its measured time, beside the algorithmic figure.
+        # The same launch's measured bytes (committed PMC summary of these
+        # engine sources and this graph) over its measured time: an upper
+        # bound on its HBM traffic, beside the algorithmic figure above.
         "traffic_GBps": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
         "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "traffic_source": TRAFFIC_SOURCE if traffic else prof_vals.get("_stale"),
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "avg_launch_ms": round(avg_ms, 4),
         "launches": agg_launch[dom],
@@ -769,16 +799,22 @@ def main() -> int:
     # level, plus the output tables) over the step time.
     step_bytes = sum(agg_bytes.values()) / args.steps
     # Whole-step measured traffic, when this run is in the profiled
-    # configuration: the engine kernels' calibrated PMC bytes of one step plus
-    # the fills' bytes (a fill writes exactly its size).
-    prof_vals = profile_values(args.workload, dom)
+    # configuration (engine sources, graph, engine and words per tile): every
+    # engine kernel and fill of one step, from the committed PMC summary.
     step_traffic = None
     wpt = tun.get("words_per_tile", args.words_per_tile or 8)
     if ("step_traffic_bytes" in prof_vals and prof_vals.get("profiled_engine") == engine
             and prof_vals.get("profiled_words_per_tile") == str(wpt)):
-        tb = float(prof_vals["step_traffic_bytes"]) + agg_bytes.get("memset", 0.0) / args.steps
+        tb = float(prof_vals["step_traffic_bytes"])
         step_traffic = {"bytes": tb, "GBps": round(tb / (ms_per_step * 1e-3) / 1e9, 1),
-                        "frac": round(tb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                        "frac": round(tb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "source": TRAFFIC_SOURCE}
+    # Gather-aware bound of the pull: every pull level may read each in-edge's
+    # source frontier line (8 B per word) besides the CSR and the per-vertex
+    # state the algorithmic count holds; the early exit reads fewer.
+    T_used = int(wpt)
+    pull_levels = last["pull_levels"] if last else 0
+    pull_gather_bound = pull_levels * ((4.0 + 8.0 * T_used) * g.m + 4.0 * (n + 1) + 24.0 * T_used * n)
     kernels = {k: {"ms_per_step": round(agg_ms[k] / args.steps, 3),
                    "launches_per_step": agg_launch[k] / args.steps,
                    "GBps": round(agg_bytes[k] / (agg_ms[k] * 1e-3) / 1e9, 1) if agg_ms[k] > 0 else None}
@@ -829,6 +865,8 @@ def main() -> int:
         "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,
         "roofline": roofline,
         "step_algorithmic_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "step_algorithmic_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pull_gather_bound_bytes_per_step": pull_gather_bound,
         "step_traffic": step_traffic,
         "kernels": kernels,
         "cpu_baseline": cpu,
