@@ -78,8 +78,7 @@ typedef struct xb_graph xb_graph;
 typedef struct xb_msbfs_opts {
   int32_t mode;            /* XB_MODE_*                                        */
   uint32_t flags;          /* XB_OUT_DEVICE | XB_PROFILE | ...                 */
-  int32_t words_per_tile;  /* 64-bit source words traversed together: 1,2,4,8,16
-                              (a persistent batch caps its tiles at 8)       */
+  int32_t words_per_tile;  /* 64-bit source words traversed together: 1,2,4,8,16 */
   int32_t alpha;           /* pull when |F| + outdeg(F) > m / alpha            */
   int32_t pull_cut_small;
   int32_t pull_cut_medium;

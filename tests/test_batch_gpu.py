@@ -41,7 +41,7 @@ def mixed():
     return gs, hgs, srcs, refs
 
 
-@pytest.mark.parametrize("T", [1, 8, 16])  # 16: the batch runs 8-word tiles
+@pytest.mark.parametrize("T", [1, 8, 16])
 def test_batch_mixed_vs_oracle(mixed, T):
     gs, hgs, srcs, refs = mixed
     res, st = xb.run_batched(hgs, srcs, words_per_tile=T, pull_cut_small=4, pull_cut_medium=16, pull_cut_hub=128,

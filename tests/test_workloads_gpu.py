@@ -54,3 +54,38 @@ def test_livejournal_shaped_sampled():
     assert 0 < edges <= S * g.m
     deg = g.degrees().astype(np.int64)
     assert e_rows == int(sum(deg[lv_h[i] >= 0].sum() for i in range(len(rows))))
+
+
+@pytest.mark.timeout(900)
+def test_livejournal_shaped_16_words_both_engines():
+    """The benched configuration: all 1,024 sources in one 16-word tile, on
+    the host-driven engine and as a single persistent launch (the batch API on
+    one graph, replayed twice).  Both engines' tables are identical over all
+    S x n entries (device checksums); sampled rows equal the oracle, one row
+    satisfies the definition invariants, and the traversed-edge count equals
+    the oracle's for those rows."""
+    g, src = xb.workload_livejournal_like()
+    S, n = src.size, g.n
+    hg = xb.HGraph(g.row_offsets, g.col_indices)
+    eng = xb.XBlossomGpuEngine10(hg)
+    lv = xb.DeviceBuffer((S, n))
+    pa = xb.DeviceBuffer((S, n))
+    eng.MultiSourceBFS(src, lv.ptr, pa.ptr, mode=xb.XB_MODE_BATCHED, device_out=True, words_per_tile=16)
+    assert eng.last_stats.tiles == 1
+    host = (lv.checksum(), pa.checksum())
+    lv2 = xb.DeviceBuffer((S, n), fill=-7)
+    pa2 = xb.DeviceBuffer((S, n), fill=-7)
+    b = xb.Batch([hg], [src], [lv2.ptr], [pa2.ptr], words_per_tile=16)
+    for _ in range(2):
+        st = b.run()
+        assert st.kernel_launches[11] == 1 and st.tiles == 1
+        assert (lv2.checksum(), pa2.checksum()) == host
+    b.close()
+    rows = [0, 100, 511, 512, 1023]
+    lv_h = np.concatenate([lv2.rows(r, r + 1) for r in rows])
+    pa_h = np.concatenate([pa2.rows(r, r + 1) for r in rows])
+    lv_ref, pa_ref, e_rows = oracle.msbfs(g.row_offsets, g.col_indices, src[rows], threads=16)
+    assert np.array_equal(lv_h, lv_ref) and np.array_equal(pa_h, pa_ref)
+    graphs.check_invariants(g.row_offsets, g.col_indices, src[rows], lv_h, pa_h, rows=[2])
+    deg = g.degrees().astype(np.int64)
+    assert e_rows == int(sum(deg[lv_h[i] >= 0].sum() for i in range(len(rows))))

@@ -1547,9 +1547,9 @@ extern "C" int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int
 #define XB_REBALANCE_FLOOR 4
 #endif
 
-// The persistent kernel instantiates its tile loop for 1, 2, 4 and 8 words;
-// a batch asked for 16-word tiles runs 512-source tiles.
-constexpr int kBatchMaxWords = 8;
+// The persistent kernel instantiates its tile loop for 1, 2, 4, 8 and 16
+// words, as the host-driven engine does.
+constexpr int kBatchMaxWords = 16;
 
 struct xb_batch {
   hipStream_t stream = nullptr;

@@ -112,7 +112,7 @@ struct TileJob {
   int32_t* par_out;
   int32_t status;          // out: 0 done, 1 deeper than the level code (re-run on the plain path)
   int32_t levels;          // out: deepest level reached
-  int32_t words;           // 64-bit words per vertex of this tile (1, 2, 4 or 8; 64 * words >= cnt)
+  int32_t words;           // 64-bit words per vertex of this tile (1, 2, 4, 8 or 16; 64 * words >= cnt)
   int32_t pad_;
 };
 

@@ -803,16 +803,175 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
   return m;
 }
 
-// Light vertices, one word group (W = min(T, 4) words, 64*W rows) per pass.
-// A block takes VG = 64/W consecutive vertices (output order); lane (vertex,
-// partition p, word w) takes every 4th in-neighbour of the vertex (kEmitU per
-// step) and tests the 64 sources of its word at once.  The W word-lanes of a
-// (vertex, partition) read the same neighbour's 128-byte plane line, so the
-// random gathers use every byte they fetch.  A segmented prefix-OR over the
-// 4 partitions, in ascending neighbour position, gives the first (lowest-id)
-// provider of every bit, staged in an LDS tile [row][vertex]; heavy vertices
-// copy their recorded parents into it.  Write phase: VG consecutive lanes
-// write one row's VG vertices (4*VG contiguous bytes), 256/VG rows per store.
+// Light-vertex emit, compute phase of one chunk of VG = 64/W consecutive
+// vertices (output order) and one word group (W words, 64*W rows).  Lane
+// (vertex vi, partition p, word w) takes every 4th in-neighbour of the vertex
+// (kEmitU per step) and tests the 64 sources of its word at once.  The W
+// word-lanes of a (vertex, partition) read the same neighbour's plane line.
+// A segmented prefix-OR over the 4 partitions, in ascending neighbour
+// position, gives the first (lowest-id) provider of every bit, staged in the
+// LDS tile [row][vertex] (pitch VG + 1); heavy vertices copy their recorded
+// parents into it.  Planes 0-3 of each (vertex, word) go to pl[k][vi*W + w]
+// for the write phase.  No block barrier inside: lanes write disjoint tile
+// entries.
+template <int T, int K, int W>
+__device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t vb, int w, int p, int vi,
+                                                const int32_t* __restrict__ in_off,
+                                                const int32_t* __restrict__ in_col,
+                                                const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
+                                                int32_t* tile, uint64_t (*pl)[64]) {
+  constexpr int PART = 4;
+  constexpr int LD = 64 / W + 1;
+  constexpr int KL = K < 4 ? K : 4;
+  const int l = lane_id();
+  const int rb = 64 * w;  // this lane's rows in the tile
+  const int32_t v = (int32_t)(vb + vi);
+  const bool act = v < n;
+  uint64_t pk[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) pk[k] = 0;
+  int32_t b = 0, e = 0, vn = 0;
+  if (act) {
+    vn = s.new_of[v];
+    load_planes<T, K>(s, vn, t, pk);
+    b = in_off[v];
+    e = in_off[v + 1];
+  }
+  if (p == 0) {
+#pragma unroll
+    for (int k = 0; k < KL; ++k) pl[k][vi * W + w] = pk[k];
+  }
+  const CodeMasks<K> cm = code_masks<K>(pk);
+  const bool heavy = act && (e - b > heavy_cut);
+  uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
+  if (heavy) {
+    const uint32_t* hp = s.hpar + ((int64_t)s.hidx[vn] * T + t) * 64;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tile[(rb + 16 * p + i) * LD + vi] = (int32_t)hp[16 * p + i];
+  }
+  for (int32_t k0 = 0;; k0 += PART * kEmitU) {
+    const bool more = need != 0 && (b + k0 + p < e);
+    if (!__any(more)) break;
+    int32_t us[kEmitU], uo[kEmitU];
+    uint64_t eqs[kEmitU];
+#pragma unroll
+    for (int i = 0; i < kEmitU; ++i) {
+      const int32_t j = b + k0 + i * PART + p;
+      const bool in = more && j < e;
+      us[i] = in ? in_col[j] : -1;
+      uo[i] = in ? in_col_orig[j] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < kEmitU; ++i) {
+      uint64_t eq = 0;
+      if (us[i] >= 0) {
+        uint64_t pu[K];
+        load_planes<T, K>(s, us[i], t, pu);
+        eq = ~0ull;
+#pragma unroll
+        for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
+      }
+      eqs[i] = eq;
+    }
+#pragma unroll
+    for (int i = 0; i < kEmitU; ++i) {
+      const uint64_t c = eqs[i] & need;
+      uint64_t incl = c, excl, all;
+      if constexpr (W == 1) {
+        // The 4 partitions of a vertex are one lane quad.
+        incl |= quad_perm64<quad_sel(0, 0, 1, 2)>(incl);  // lane p <- p-1 (lane 0 <- itself)
+        incl |= quad_perm64<quad_sel(0, 1, 0, 1)>(incl);  // lane p <- p-2 (lanes 0, 1 <- themselves)
+        excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
+        if (p == 0) excl = 0;
+        all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
+      } else {
+        uint64_t y = __shfl_up(incl, W, kWave);
+        if (p >= 1) incl |= y;
+        y = __shfl_up(incl, 2 * W, kWave);
+        if (p >= 2) incl |= y;
+        excl = __shfl_up(incl, W, kWave);
+        if (p == 0) excl = 0;
+        all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
+      }
+      uint64_t first = c & ~excl;
+      while (first) {
+        const int bit = __builtin_ctzll(first);
+        first &= first - 1;
+        tile[(rb + bit) * LD + vi] = uo[i];
+      }
+      need &= ~all;
+    }
+  }
+}
+
+// Light-vertex emit, write phase of one 64-vertex chunk of one word (W = 1,
+// K <= 4), by 256 lanes: lane (row group, quad c4) writes 4 consecutive
+// vertices of a row as one 16-byte non-temporal store when the row segment
+// is aligned, else one by one; a wave writes 4 rows x 256 contiguous bytes
+// per store instruction.  The level codes of the lane's 4 vertices in its 4
+// rows are packed from the LDS planes once.
+template <int K>
+__device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t grp, int32_t rows,
+                                                 const int32_t* __restrict__ src, int32_t n, const int32_t* tile,
+                                                 const uint64_t (*pl)[64], int32_t* __restrict__ lvl_out,
+                                                 int32_t* __restrict__ par_out, RowMap map) {
+  static_assert(K <= 4, "the packed write covers codes below 16");
+  constexpr int LD = 65;
+  constexpr int RSTEP = 16;  // rows per pass: 256 lanes / 16 quads per row
+  const int c4 = 4 * (tid % 16);
+  const int row0 = tid / 16;
+  uint32_t cw[64 / RSTEP];
+#pragma unroll
+  for (int it = 0; it < 64 / RSTEP; ++it) cw[it] = 0;
+#pragma unroll 1
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t q = pl[k][c4 + a] >> row0;
+#pragma unroll
+      for (int it = 0; it < 64 / RSTEP; ++it) cw[it] |= (uint32_t)((q >> (RSTEP * it)) & 1ull) << (8 * a + k);
+    }
+#pragma unroll 1
+  for (int it = 0; it < 64 / RSTEP; ++it) {
+    const int rr = row0 + RSTEP * it;
+    const int r = 64 * grp + rr;
+    if (r >= rows) break;
+    int32_t lv[4], pv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const uint32_t c = (cw[it] >> (8 * a)) & 0xFFu;
+      const int32_t va = (int32_t)(vb + c4 + a);
+      lv[a] = (int32_t)c - 1;
+      pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? src[r] : tile[rr * LD + c4 + a]));
+    }
+    const int64_t v0 = vb + c4;
+    const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v0;
+    const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v0;
+    if (v0 + 3 < n && ((lo | po) & 3) == 0) {
+      typedef int32_t v4i __attribute__((ext_vector_type(4)));
+      if (lvl_out) {
+        const v4i lq = {lv[0], lv[1], lv[2], lv[3]};
+        __builtin_nontemporal_store(lq, reinterpret_cast<v4i*>(lvl_out + lo));
+      }
+      const v4i pq = {pv[0], pv[1], pv[2], pv[3]};
+      __builtin_nontemporal_store(pq, reinterpret_cast<v4i*>(par_out + po));
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (v0 + a < n) {
+          if (lvl_out) __builtin_nontemporal_store(lv[a], &lvl_out[lo + a]);
+          __builtin_nontemporal_store(pv[a], &par_out[po + a]);
+        }
+      }
+    }
+  }
+}
+
+// Light vertices, one word group (W = min(T, XB_EMIT_GROUP_MAX) words, 64*W
+// rows) per pass; a block takes chunks of VG = 64/W consecutive vertices:
+// emit_scan_chunk, then the write phase -- emit_write_chunk for one word and
+// codes below 16, else VG consecutive lanes write one row's VG vertices
+// (4*VG contiguous bytes), 256/VG rows per store.
 template <int T, int K>
 __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
@@ -828,12 +987,10 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
   static_assert(64 * W * LD <= kEmitTileInts, "emit tile");
   EmitLds& L = g_lds.e;
   const int tid = threadIdx.x;
-  const int l = lane_id();
   const int w = tid % W;
   const int p = (tid / W) % PART;
   const int vi = tid / (W * PART);
   int t = grp * W + w;                // this lane's word of the tile
-  const int rb = 64 * w;              // its rows in the block tile
   const int wcol = tid % VG;          // write phase: vertex column
   const int wrow = tid / VG;          // write phase: first row
   __syncthreads();
@@ -855,140 +1012,11 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
       vb = it * VG;
       if (vb >= n) break;
     }
-    // ---- compute phase ----
-    const int32_t v = (int32_t)(vb + vi);
-    const bool act = v < n;
-    uint64_t pk[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) pk[k] = 0;
-    int32_t b = 0, e = 0, vn = 0;
-    if (act) {
-      vn = s.new_of[v];
-      load_planes<T, K>(s, vn, t, pk);
-      b = in_off[v];
-      e = in_off[v + 1];
-    }
-    if (p == 0) {
-#pragma unroll
-      for (int k = 0; k < KL; ++k) L.pl[k][vi * W + w] = pk[k];
-    }
-    const CodeMasks<K> cm = code_masks<K>(pk);
-    const bool heavy = act && (e - b > heavy_cut);
-    uint64_t need = (act && !heavy) ? (cm.reached & ~cm.lvl0 & ~cm.lvl1) : 0ull;
-    if (heavy) {
-      const uint32_t* hp = s.hpar + ((int64_t)s.hidx[vn] * T + t) * 64;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) L.tile[(rb + 16 * p + i) * LD + vi] = (int32_t)hp[16 * p + i];
-    }
-    for (int32_t k0 = 0;; k0 += PART * kEmitU) {
-      const bool more = need != 0 && (b + k0 + p < e);
-      if (!__any(more)) break;
-      int32_t us[kEmitU], uo[kEmitU];
-      uint64_t eqs[kEmitU];
-#pragma unroll
-      for (int i = 0; i < kEmitU; ++i) {
-        const int32_t j = b + k0 + i * PART + p;
-        const bool in = more && j < e;
-        us[i] = in ? in_col[j] : -1;
-        uo[i] = in ? in_col_orig[j] : -1;
-      }
-#pragma unroll
-      for (int i = 0; i < kEmitU; ++i) {
-        uint64_t eq = 0;
-        if (us[i] >= 0) {
-          uint64_t pu[K];
-          load_planes<T, K>(s, us[i], t, pu);
-          eq = ~0ull;
-#pragma unroll
-          for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
-        }
-        eqs[i] = eq;
-      }
-#pragma unroll
-      for (int i = 0; i < kEmitU; ++i) {
-        const uint64_t c = eqs[i] & need;
-        uint64_t incl = c, excl, all;
-        if constexpr (W == 1) {
-          // The 4 partitions of a vertex are one lane quad.
-          incl |= quad_perm64<quad_sel(0, 0, 1, 2)>(incl);  // lane p <- p-1 (lane 0 <- itself)
-          incl |= quad_perm64<quad_sel(0, 1, 0, 1)>(incl);  // lane p <- p-2 (lanes 0, 1 <- themselves)
-          excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
-          if (p == 0) excl = 0;
-          all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
-        } else {
-          uint64_t y = __shfl_up(incl, W, kWave);
-          if (p >= 1) incl |= y;
-          y = __shfl_up(incl, 2 * W, kWave);
-          if (p >= 2) incl |= y;
-          excl = __shfl_up(incl, W, kWave);
-          if (p == 0) excl = 0;
-          all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
-        }
-        uint64_t first = c & ~excl;
-        while (first) {
-          const int bit = __builtin_ctzll(first);
-          first &= first - 1;
-          L.tile[(rb + bit) * LD + vi] = uo[i];
-        }
-        need &= ~all;
-      }
-    }
+    emit_scan_chunk<T, K, W>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
     __syncthreads();
-    // ---- write phase ----
 #if XB_EMIT_VEC_STORE
     if constexpr (W == 1 && K <= 4) {
-      // Lane (row group, quad c4) writes 4 consecutive vertices of a row as
-      // one 16-byte store when the row segment is aligned, else one by one.
-      constexpr int RSTEP = 256 / (VG / 4);  // rows per pass (16)
-      const int c4 = 4 * (tid % (VG / 4));
-      const int row0 = tid / (VG / 4);
-      // The level codes of this lane's 4 vertices in its 64 / RSTEP rows,
-      // one byte per (row, vertex), taken from the planes once.
-      uint32_t cw[64 / RSTEP];
-#pragma unroll
-      for (int it = 0; it < 64 / RSTEP; ++it) cw[it] = 0;
-#pragma unroll 1
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int k = 0; k < KL; ++k) {
-          const uint64_t q = L.pl[k][c4 + a] >> row0;
-#pragma unroll
-          for (int it = 0; it < 64 / RSTEP; ++it) cw[it] |= (uint32_t)((q >> (RSTEP * it)) & 1ull) << (8 * a + k);
-        }
-#pragma unroll 1
-      for (int it = 0; it < 64 / RSTEP; ++it) {
-        const int rr = row0 + RSTEP * it;
-        const int r = 64 * grp + rr;
-        if (r >= rows) break;
-        int32_t lv[4], pv[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const uint32_t c = (cw[it] >> (8 * a)) & 0xFFu;
-          const int32_t va = (int32_t)(vb + c4 + a);
-          lv[a] = (int32_t)c - 1;
-          pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? src[r] : L.tile[rr * LD + c4 + a]));
-        }
-        const int64_t v0 = vb + c4;
-        const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v0;
-        const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v0;
-        if (v0 + 3 < n && ((lo | po) & 3) == 0) {
-          typedef int32_t v4i __attribute__((ext_vector_type(4)));
-          if (lvl_out) {
-            const v4i lq = {lv[0], lv[1], lv[2], lv[3]};
-            __builtin_nontemporal_store(lq, reinterpret_cast<v4i*>(lvl_out + lo));
-          }
-          const v4i pq = {pv[0], pv[1], pv[2], pv[3]};
-          __builtin_nontemporal_store(pq, reinterpret_cast<v4i*>(par_out + po));
-        } else {
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            if (v0 + a < n) {
-              if (lvl_out) __builtin_nontemporal_store(lv[a], &lvl_out[lo + a]);
-              __builtin_nontemporal_store(pv[a], &par_out[po + a]);
-            }
-          }
-        }
-      }
+      emit_write_chunk<K>(tid, vb, grp, rows, src, n, L.tile, L.pl, lvl_out, par_out, map);
       __syncthreads();
       continue;
     }
@@ -1335,9 +1363,11 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   // Tile set-up: seen and planes cleared, heavy parents unresolved; fronts,
   // next words and touched flags are clean between tiles by construction.
   nl_fill_u64(g, s.seen, nw, 0ull);
-  // The team's plane block holds both arrays, planes 4-7 after this tile's 0-3.
+  // The team's plane block holds both arrays, planes 4-7 after this tile's
+  // 0-3.  Planes 0-3 are cleared here, 4-7 only before the first level whose
+  // code needs them (level 15), which most tiles never reach.
   s.planes_hi = s.planes + nw * 4;
-  nl_fill_u64(g, s.planes, nw * 8, 0ull);
+  nl_fill_u64(g, s.planes, nw * 4, 0ull);
   if (gv.n_heavy > 0) nl_fill_u32(g, s.hpar, (int64_t)gv.n_heavy * T * 64, 0xFFFFFFFFu);
   if (g.bid == 0 && threadIdx.x == 0) {
     ctl->ctr[1] = 0ull;
@@ -1360,6 +1390,10 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
       nl_clear<T>(g, s, cur, qc.v, qn);
       if (g.bid == 0 && threadIdx.x == 0) job.status = 1;
       return team_barrier(ts, g, gerr, XB_K_CLEAR);
+    }
+    if (code == 16u) {
+      nl_fill_u64(g, s.planes_hi, nw * 4, 0ull);
+      if (!team_barrier(ts, g, gerr, XB_K_MEMSET)) return false;
     }
     const Queue qx{tm.qv[(d + 1) & 1], tm.qeo[(d + 1) & 1], &ctl->ctr[(d + 1) % 3]};
     s.tcount = &ctl->tcount[d & 1];
@@ -1446,7 +1480,8 @@ __global__ __launch_bounds__(kBlock, XB_PERSISTENT_MIN_BLOCKS) void k_bp_persist
       case 1: ok = run_tile<1>(g, tm, ts, gv, job, alpha, gerr); break;
       case 2: ok = run_tile<2>(g, tm, ts, gv, job, alpha, gerr); break;
       case 4: ok = run_tile<4>(g, tm, ts, gv, job, alpha, gerr); break;
-      default: ok = run_tile<8>(g, tm, ts, gv, job, alpha, gerr); break;
+      case 8: ok = run_tile<8>(g, tm, ts, gv, job, alpha, gerr); break;
+      default: ok = run_tile<16>(g, tm, ts, gv, job, alpha, gerr); break;
     }
     if (!ok) return;
   }
