@@ -302,10 +302,14 @@ SERVING_DESC = ("serving: directed web-crawl-shaped R-MAT n=20M (a=.6 b=.15 c=.1
                 "sources (log-uniform, seed 8), Poisson arrivals (seed 7)")
 
 
-def serving_setup(slots: int, words_per_tile: int):
+def serving_setup(slots: int, words_per_tile: int, tune: bool = True):
     """The serving workload's graph (directed R-MAT, n = 20M, ~300M edges,
     a=.6 b=.15 c=.15, seed 7) resident on the device, a server over it, and
-    the candidate source vertices (those with out-edges)."""
+    the candidate source vertices (those with out-edges).  With `tune`, the
+    autotuner picks the push/pull switch and pull cut-offs on 512 candidate
+    sources (seed 11) before the server starts; the server keeps its own
+    tile width (words_per_tile, or its default), which sets how many queued
+    rows share a tile."""
     import numpy as np
 
     import xbmsbfs as xb
@@ -316,7 +320,19 @@ def serving_setup(slots: int, words_per_tile: int):
     log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
     cand = np.flatnonzero(g.degrees() > 0).astype(np.int32)
     tun = {"words_per_tile": words_per_tile} if words_per_tile else {}
+    tuned = None
+    if tune:
+        t1 = time.perf_counter()
+        sample = np.random.default_rng(11).choice(cand, size=512, replace=False).astype(np.int32)
+        best, rep = xb.autotune(hg, sample, **tun)
+        best.pop("words_per_tile", None)
+        tuned = {"seconds": round(time.perf_counter() - t1, 2), "trials": rep.trials,
+                 "default_ms_per_source": round(rep.default_ms_per_source, 5),
+                 "best_ms_per_source": round(rep.best_ms_per_source, 5), "chosen": best}
+        log(f"serving autotune: {tuned}")
+        tun.update(best)
     srv = xb.Server(hg, slots=slots, **tun)
+    srv.tuned = tuned
     return g, hg, srv, cand
 
 
@@ -404,13 +420,13 @@ def serve_once(srv, n: int, reqs, rate: float, seed: int) -> dict:
     }
 
 
-def serving_summary(slots: int, words_per_tile: int, requests: int, knee_requests: int) -> dict:
+def serving_summary(slots: int, words_per_tile: int, requests: int, knee_requests: int, tune: bool = True) -> dict:
     """The serving sub-object of the default bench line: the configured 2,000
     queries/s offered rate (far past one GPU's capacity: latency is admission
     wait), the capacity that run measures, and a sweep of lower offered rates
     for the knee -- the highest rate whose p99 stays within 2x the service
     p99 (the latency of a query that never waits for admission)."""
-    g, hg, srv, cand = serving_setup(slots, words_per_tile)
+    g, hg, srv, cand = serving_setup(slots, words_per_tile, tune)
     warm = serving_requests(cand, 3, seed=9)
     serve_once(srv, g.n, warm, 2000.0, seed=9)
     at_2k = serve_once(srv, g.n, serving_requests(cand, requests, seed=8), 2000.0, seed=7)
@@ -427,7 +443,7 @@ def serving_summary(slots: int, words_per_tile: int, requests: int, knee_request
     srv.close()
     hg.close()
     return {"workload": SERVING_DESC, "n": g.n, "m": g.m, "slots": slots,
-            "words_per_tile": words_per_tile or 8, "at_2000_per_s": at_2k, "capacity_queries_per_s": cap,
+            "words_per_tile": words_per_tile or 8, "autotune": srv.tuned, "at_2000_per_s": at_2k, "capacity_queries_per_s": cap,
             "knee": ({"offered_per_s": knee["offered_per_s"], "p50_ms": knee["p50_ms"], "p99_ms": knee["p99_ms"],
                       "service_p99_ms": knee["service_p99_ms"], "gteps": knee["gteps"]} if knee else None),
             "sweep": sweep}
@@ -436,7 +452,7 @@ def serving_summary(slots: int, words_per_tile: int, requests: int, knee_request
 def bench_serving(args) -> int:
     """Serving on its own line: the web-crawl-shaped graph resident on the
     device, --requests queries at Poisson arrivals of --rate per second."""
-    g, hg, srv, cand = serving_setup(args.slots, args.words_per_tile)
+    g, hg, srv, cand = serving_setup(args.slots, args.words_per_tile, not args.no_autotune)
     serve_once(srv, g.n, serving_requests(cand, args.warmup, seed=9), args.rate, seed=9)
     r = serve_once(srv, g.n, serving_requests(cand, args.requests, seed=8), args.rate, seed=7)
     srv.close()
@@ -455,7 +471,7 @@ def bench_serving(args) -> int:
         "data": "synthetic",
         "config": {"workload": SERVING_DESC + f", {args.rate:g}/s offered", "n": g.n, "m": g.m,
                    "queries": r["queries"], "slots": args.slots, "words_per_tile": args.words_per_tile or 8,
-                   "sources": r["sources"]},
+                   "sources": r["sources"], "autotune": srv.tuned},
         **{k: r[k] for k in ("p50_ms", "p99_ms", "service_p50_ms", "service_p99_ms", "admission_wait_p50_ms",
                              "queries_per_s", "offered_per_s", "output_TBps", "tiles", "rows_per_tile",
                              "peak_table_GB")},
@@ -833,7 +849,7 @@ def main() -> int:
         del lv, pa
         torch.cuda.empty_cache()
         hg.close()
-        serving = serving_summary(args.slots, 0, args.serving_requests, args.knee_requests)
+        serving = serving_summary(args.slots, 0, args.serving_requests, args.knee_requests, not args.no_autotune)
 
     line = {
         "metric": METRIC,

@@ -377,7 +377,7 @@ typedef struct xb_server xb_server;
 typedef struct xb_server_opts {
   int32_t slots;          /* worker slots (streams), default 4                 */
   int32_t max_queued;     /* submitted and unreleased requests, default 65536  */
-  int64_t budget_bytes;   /* device bytes for request tables; 0 = 3/4 of free  */
+  int64_t budget_bytes;   /* device bytes for request tables; 0 = 7/8 of free  */
   uint32_t flags;         /* XB_SERVE_COUNT_EDGES (default on)                 */
   int32_t reserved[7];
 } xb_server_opts;

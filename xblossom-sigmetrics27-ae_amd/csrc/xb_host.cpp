@@ -2899,7 +2899,7 @@ int32_t xb_server_create(xb_graph* g, const xb_server_opts* opts, const xb_msbfs
     if (budget == 0) {
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
-      budget = (int64_t)(fr / 4 * 3);
+      budget = (int64_t)(fr / 8 * 7);
     }
     if (budget < 256) fail(XB_ERR_BUDGET, "serving budget below 256 bytes");
     sv->arena.init(budget);
