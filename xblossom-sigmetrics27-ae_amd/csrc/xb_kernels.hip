@@ -484,12 +484,35 @@ __device__ __forceinline__ void load_planes(const BPState& s, int64_t v, int t, 
   }
 }
 
+// 16-byte stores over the 16-byte aligned body, 8-byte stores for the
+// unaligned first word and an odd last one.
 __device__ void fill_u64(Grid g, uint64_t* __restrict__ p, int64_t count, uint64_t val) {
-  for (int64_t i = gthread(g); i < count; i += gnthreads(g)) p[i] = val;
+  const int64_t head = ((uintptr_t)p & 15) && count > 0 ? 1 : 0;
+  const int64_t pairs = (count - head) / 2;
+  typedef uint64_t v2u __attribute__((ext_vector_type(2)));
+  v2u* __restrict__ q = reinterpret_cast<v2u*>(p + head);
+  const v2u vv = {val, val};
+  for (int64_t i = gthread(g); i < pairs; i += gnthreads(g)) q[i] = vv;
+  if (gthread(g) == 0) {
+    if (head) p[0] = val;
+    if (head + 2 * pairs < count) p[count - 1] = val;
+  }
 }
 
+// 16-byte stores over the 16-byte aligned body, 4-byte stores for the up to
+// 3 words before it and after it.
 __device__ void fill_u32(Grid g, uint32_t* __restrict__ p, int64_t count, uint32_t val) {
-  for (int64_t i = gthread(g); i < count; i += gnthreads(g)) p[i] = val;
+  int64_t head = (int64_t)((16u - ((uintptr_t)p & 15u)) & 15u) / 4;
+  if (head > count) head = count;
+  const int64_t quads = (count - head) / 4;
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  v4u* __restrict__ q = reinterpret_cast<v4u*>(p + head);
+  const v4u vv = {val, val, val, val};
+  for (int64_t i = gthread(g); i < quads; i += gnthreads(g)) q[i] = vv;
+  const int64_t t = gthread(g);
+  if (t < head) p[t] = val;
+  const int64_t tail0 = head + 4 * quads;
+  if (t < count - tail0) p[tail0 + t] = val;
 }
 
 template <int T>
@@ -1248,12 +1271,15 @@ __device__ bool team_barrier(const TeamSync& ts, Grid g, unsigned* gerr, int pha
   return s_ok != 0;
 }
 
-// Out-of-line entry points of the phases for the persistent kernel, so each
-// phase keeps its own register allocation instead of one inlined giant.
-// Arguments of a device-function call arrive in vector registers even when
-// every lane holds the same value; the wrappers move them back to scalar
-// registers (readfirstlane) so the phase code addresses memory as it does in
-// a standalone kernel, where they are kernel arguments.
+// Entry points of the phases for the persistent kernel.  They are inlined:
+// as out-of-line calls every call passed the phase state (BPState, Queue) by
+// value through per-lane scratch and spilled the caller's live values per
+// lane, a round trip of scratch traffic for all 524,288 lanes of the team at
+// every phase (LiveJournal-shaped 16-word tile: 39.4 -> 38.0 ms).  Values the
+// tile loop derives from memory may sit in vector registers although every
+// lane holds the same value; the wrappers move them to scalar registers
+// (readfirstlane) so the phase code addresses memory as it does in a
+// standalone kernel, where they are kernel arguments.
 __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ int64_t uni(int64_t v) {
@@ -1288,21 +1314,21 @@ __device__ __forceinline__ BPState uni(const BPState& s) {
 }
 
 template <int T>
-__device__ __noinline__ void nl_seed(Grid g, BPState s, const int32_t* src, int32_t cnt) {
+__device__ __forceinline__ void nl_seed(Grid g, BPState s, const int32_t* src, int32_t cnt) {
   seed_phase<T>(uni(g), uni(s), uni(src), uni(cnt));
 }
 template <int T>
-__device__ __noinline__ void nl_push(Grid g, BPState s, int cur, const int32_t* off, const int32_t* col, Queue q,
+__device__ __forceinline__ void nl_push(Grid g, BPState s, int cur, const int32_t* off, const int32_t* col, Queue q,
                                      int32_t qn, int32_t E) {
   push_phase<T>(uni(g), uni(s), uni(cur), uni(off), uni(col), uni(q), uni(qn), uni(E));
 }
 template <int T>
-__device__ __noinline__ void nl_commit(Grid g, BPState s, int cur, uint32_t code, const int32_t* out_off,
+__device__ __forceinline__ void nl_commit(Grid g, BPState s, int cur, uint32_t code, const int32_t* out_off,
                                        Queue nxt) {
   commit_phase<T>(uni(g), uni(s), uni(cur), uni(code), uni(out_off), uni(nxt));
 }
 template <int T, int P, bool REC>
-__device__ __noinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
+__device__ __forceinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t code, int32_t cnt,
                                             const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
                                             const int32_t* perm, int32_t shift, int32_t begin, int32_t end,
                                             Queue nxt) {
@@ -1310,22 +1336,22 @@ __device__ __noinline__ void nl_pull_bucket(Grid g, BPState s, int cur, uint32_t
                                uni(perm), uni(shift), uni(begin), uni(end), uni(nxt));
 }
 template <int T>
-__device__ __noinline__ void nl_pull_hub(Grid g, BPState s, int cur, int32_t cnt, const int32_t* in_col,
+__device__ __forceinline__ void nl_pull_hub(Grid g, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                                          const HubChunk* chunks, int32_t n_chunks) {
   pull_hub_phase<T>(uni(g), uni(s), uni(cur), uni(cnt), uni(in_col), uni(chunks), uni(n_chunks));
 }
 template <int T>
-__device__ __noinline__ void nl_clear(Grid g, BPState s, int cur, const int32_t* qv, int32_t qn) {
+__device__ __forceinline__ void nl_clear(Grid g, BPState s, int cur, const int32_t* qv, int32_t qn) {
   clear_phase<T>(uni(g), uni(s), uni(cur), uni(qv), uni(qn));
 }
-__device__ __noinline__ void nl_fill_u64(Grid g, uint64_t* p, int64_t count, uint64_t val) {
+__device__ __forceinline__ void nl_fill_u64(Grid g, uint64_t* p, int64_t count, uint64_t val) {
   fill_u64(uni(g), uni(p), uni(count), (uint64_t)uni((int64_t)val));
 }
-__device__ __noinline__ void nl_fill_u32(Grid g, uint32_t* p, int64_t count, uint32_t val) {
+__device__ __forceinline__ void nl_fill_u32(Grid g, uint32_t* p, int64_t count, uint32_t val) {
   fill_u32(uni(g), uni(p), uni(count), uni(val));
 }
 template <int T, int K>
-__device__ __noinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
+__device__ __forceinline__ void nl_emit_light(Grid g, BPState s, int32_t word, int32_t rows, const int32_t* src,
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
                                            int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out,
                                            unsigned* dyn, int32_t ngrp) {
