@@ -39,8 +39,11 @@
 // (apps/xb/src/include/enode_queue/enode_queue.h:46-64 does one global atomic
 // per warp for the slot range only).
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstdint>
+#include <stdexcept>
+#include <string>
 
 #include "xb_internal.hpp"
 #include "../../include/xb/xb_capi.h"
@@ -2557,6 +2560,22 @@ void launch_mt_collect(hipStream_t st, const int32_t* off, const int32_t* col, Q
   if (E <= 0) return;
   hipLaunchKernelGGL(k_mt_collect, dim3(edge_grid(E)), dim3(kBlock), 0, st, off, col, q, qn, E, root, lvl, cand,
                      ncand, cap, err);
+}
+// The collected blossom candidates (packed 64-bit keys) in ascending order,
+// so the mark step can keep the lowest candidate per inner vertex: a rocPRIM
+// device radix sort over all 64 key bits.
+size_t sort_u64_temp_bytes(int32_t n) {
+  size_t bytes = 0;
+  const hipError_t e = rocprim::radix_sort_keys(nullptr, bytes, (const unsigned long long*)nullptr,
+                                                (unsigned long long*)nullptr, (size_t)(n > 0 ? n : 1), 0, 64);
+  if (e != hipSuccess) throw std::runtime_error(std::string("radix sort size query: ") + hipGetErrorString(e));
+  return bytes;
+}
+void sort_u64(void* temp, size_t bytes, const unsigned long long* in, unsigned long long* out, int32_t n,
+              hipStream_t st) {
+  if (n <= 0) return;
+  const hipError_t e = rocprim::radix_sort_keys(temp, bytes, in, out, (size_t)n, 0, 64, st);
+  if (e != hipSuccess) throw std::runtime_error(std::string("radix sort: ") + hipGetErrorString(e));
 }
 void launch_mt_blossom(hipStream_t st, const int32_t* off, const int32_t* col, int32_t n, const int32_t* par,
                        int32_t* lvl, int32_t* fp, unsigned long long* elab, unsigned long long* lab,
