@@ -1016,8 +1016,9 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   const int words = (cnt + 63) / 64;
   // Heavy vertices need no pass of their own: the pull and the push recorded
   // the parent of every bit they discovered in hpar, and the emit copies it.
-  // Light vertices, one launch per word group (up to 4 words, 256 rows).
-  const int W = xb::emit_group_words(T);
+  // Light vertices, one launch per word group (emit_pass_words(T) words of
+  // 64 rows each).
+  const int W = xb::emit_pass_words(T);
   for (int grp = 0; grp * W < words; ++grp) {
     prof.begin(XB_K_EMIT);
     xb::launch_bp_emit(w.st, T, s, grp, K, cnt, w.d_src, g->d_oin_off, g->d_oin_col, g->d_oin_colo, n, o.cut_medium,

@@ -23,15 +23,14 @@ namespace xb {
 // 64-bit source words per vertex ([v*T + t], engine ids).  Level codes
 // (code = level + 1, 0 = unreached) are 8 bit-sliced planes, bit b of plane
 // k = bit k of code(source 64t+b, v), in word groups of
-// W = min(T, XB_PLANE_GROUP_MAX) words (1 by default: word-major) and two
-// arrays:
+// W = min(T, XB_PLANE_GROUP_MAX) words (2 by default) and two arrays:
 //   plane k < 4:   planes[((t/W * n + v) * W + t%W) * 4 + k]
 //   plane k >= 4:  planes_hi[((t/W * n + v) * W + t%W) * 4 + k - 4]
-// so the first 4 planes of a (vertex, word) are 32 contiguous bytes (codes up
-// to 15, i.e. levels <= 14), one gather per in-neighbour in the emit, and one
-// emit pass (one word) reads a single n*32-byte region.  planes_hi serves
-// only tiles deeper than 14 levels; the host engine allocates it when a tile
-// first needs it.
+// so the first 4 planes of a vertex's W words of one group are 32W
+// contiguous bytes (codes up to 15, i.e. levels <= 14): one gathered line per
+// in-neighbour serves an emit pass over the group's W words, and that pass
+// reads a single n*32W-byte region.  planes_hi serves only tiles deeper than
+// 14 levels; the host engine allocates it when a tile first needs it.
 struct BPState {
   int64_t n;            // vertices (plane addressing)
   uint64_t* seen;
@@ -181,8 +180,9 @@ void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, 
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
 // src: the tile's source ids (level-1 parents are the sources themselves).
-// Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W = min(T, 4)).
-int emit_group_words(int T);
+// Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W =
+// emit_pass_words(T)).
+int emit_pass_words(int T);
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                     int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map);

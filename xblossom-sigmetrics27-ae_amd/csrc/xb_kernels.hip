@@ -69,6 +69,14 @@ constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 #ifndef XB_EMIT_U
 #define XB_EMIT_U 4
 #endif
+// Source words per pass of the standalone light emit (k_bp_emit_light): its
+// workgroups have 256 * W threads and take chunks of 64 vertices in W words,
+// so every output row segment stays 64 vertices (256 bytes) wide while one
+// gathered plane line serves W words.  The persistent kernel's emit keeps
+// one word per pass (256-thread workgroups).
+#ifndef XB_EMIT_WORDS
+#define XB_EMIT_WORDS 2
+#endif
 constexpr int kPullU = XB_PULL_U;
 constexpr int kEmitU = XB_EMIT_U;
 constexpr int kStageCap = 1536;
@@ -195,14 +203,16 @@ struct QueueLds {
   int s_lohi[2];
 };
 
-// The emit's block tile: 64*W rows x 64/W vertices (+1 column of padding),
-// i.e. 64x65, 128x33 or 256x17 int32 for W = 1, 2, 4.
-constexpr int kEmitTileInts = 256 * 17;
-struct EmitLds {
-  int32_t tile[kEmitTileInts];
-  uint64_t pl[4][64];  // plane k of (vertex, word) of the block, compute -> write phase (lane-contiguous)
-  int64_t next;        // dynamic schedule: the block's next (word group, chunk)
+// The light emit's workgroup state for W words per pass: the parent tile
+// [64W rows][64 vertices + 1 column of padding] and planes 0-3 of every
+// (word, vertex) of the chunk.
+template <int W>
+struct EmitLdsT {
+  int32_t tile[64 * W * 65];
+  uint64_t pl[W][4][64];  // plane k of (word w, vertex), scan -> write phase
+  int64_t next;           // dynamic schedule: the block's next (word group, chunk)
 };
+using EmitLds = EmitLdsT<1>;
 
 union Lds {
   QueueLds q;
@@ -414,21 +424,16 @@ __device__ __forceinline__ uint64_t word_mask(int32_t cnt, int t) {
   return nb >= 64 ? ~0ull : (nb <= 0 ? 0ull : ((1ull << nb) - 1ull));
 }
 
-// Level-code planes of (v, word t) in the word-major split layout.
-// Words per emit pass (and per plane group), at most XB_EMIT_GROUP_MAX.  One
-// word per pass keeps the output row segments a block writes at 256 bytes;
-// measured on C3, groups of 2 and 4 words (128- and 64-byte segments) made
-// the emit 7% and 20% slower despite using every fetched plane byte.
-#ifndef XB_EMIT_GROUP_MAX
-#define XB_EMIT_GROUP_MAX 1
-#endif
+// Words per pass of the standalone emit for tiles of T words.
 template <int T>
-constexpr int kGroupWords = T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX;
+constexpr int kEmitWords = T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS;
 
-// Words per plane group (the plane layout), at most XB_PLANE_GROUP_MAX; the
-// emit's words per pass must divide it.
+// Level-code planes of (v, word t): groups of W = min(T, XB_PLANE_GROUP_MAX)
+// words, vertex-major within a group, so the planes of a vertex's W words of
+// one group are 32W contiguous bytes and one gathered line serves an emit
+// pass of W words.  The emit's words per pass must divide the group.
 #ifndef XB_PLANE_GROUP_MAX
-#define XB_PLANE_GROUP_MAX XB_EMIT_GROUP_MAX
+#define XB_PLANE_GROUP_MAX XB_EMIT_WORDS
 #endif
 template <int T>
 constexpr int kPlaneGroup = T < XB_PLANE_GROUP_MAX ? T : XB_PLANE_GROUP_MAX;
@@ -436,7 +441,7 @@ constexpr int kPlaneGroup = T < XB_PLANE_GROUP_MAX ? T : XB_PLANE_GROUP_MAX;
 template <int T>
 __device__ __forceinline__ int64_t plane_index(const BPState& s, int64_t v, int t) {
   constexpr int W = kPlaneGroup<T>;
-  static_assert(kPlaneGroup<T> % kGroupWords<T> == 0, "emit words per pass must divide the plane group");
+  static_assert(kPlaneGroup<T> % kEmitWords<T> == 0, "emit words per pass must divide the plane group");
   return (((int64_t)(t / W) * s.n + v) * W + (t % W)) * 4;
 }
 template <int T>
@@ -829,25 +834,24 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
   return m;
 }
 
-// Light-vertex emit, compute phase of one chunk of VG = 64/W consecutive
-// vertices (output order) and one word group (W words, 64*W rows).  Lane
-// (vertex vi, partition p, word w) takes every 4th in-neighbour of the vertex
-// (kEmitU per step) and tests the 64 sources of its word at once.  The W
-// word-lanes of a (vertex, partition) read the same neighbour's plane line.
-// A segmented prefix-OR over the 4 partitions, in ascending neighbour
-// position, gives the first (lowest-id) provider of every bit, staged in the
-// LDS tile [row][vertex] (pitch VG + 1); heavy vertices copy their recorded
-// parents into it.  Planes 0-3 of each (vertex, word) go to pl[k][vi*W + w]
-// for the write phase.  No block barrier inside: lanes write disjoint tile
-// entries.
+// Light-vertex emit, compute phase of one chunk of 64 consecutive vertices
+// (output order) and one word group (W words, 64*W rows).  Lane (vertex vi,
+// partition p, word w) takes every 4th in-neighbour of the vertex (kEmitU per
+// step) and tests the 64 sources of its word at once.  The W word-lanes of a
+// (vertex, partition) read the same neighbour's plane line.  A segmented
+// prefix-OR over the 4 partitions, in ascending neighbour position, gives the
+// first (lowest-id) provider of every bit, staged in the LDS tile [row][vertex]
+// (pitch 65); heavy vertices copy their recorded parents into it.  Planes 0-3
+// of each (word, vertex) go to pl[w][k][vi] for the write phase.  No block
+// barrier inside: lanes write disjoint tile entries.
 template <int T, int K, int W>
 __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t vb, int w, int p, int vi,
                                                 const int32_t* __restrict__ in_off,
                                                 const int32_t* __restrict__ in_col,
                                                 const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
-                                                int32_t* tile, uint64_t (*pl)[64]) {
+                                                int32_t* tile, uint64_t (*pl)[4][64]) {
   constexpr int PART = 4;
-  constexpr int LD = 64 / W + 1;
+  constexpr int LD = 65;
   constexpr int KL = K < 4 ? K : 4;
   const int l = lane_id();
   const int rb = 64 * w;  // this lane's rows in the tile
@@ -865,7 +869,7 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
   }
   if (p == 0) {
 #pragma unroll
-    for (int k = 0; k < KL; ++k) pl[k][vi * W + w] = pk[k];
+    for (int k = 0; k < KL; ++k) pl[w][k][vi] = pk[k];
   }
   const CodeMasks<K> cm = code_masks<K>(pk);
   const bool heavy = act && (e - b > heavy_cut);
@@ -993,56 +997,56 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
   }
 }
 
-// Light vertices, one word group (W = min(T, XB_EMIT_GROUP_MAX) words, 64*W
-// rows) per pass; a block takes chunks of VG = 64/W consecutive vertices:
-// emit_scan_chunk, then the write phase -- emit_write_chunk for one word and
-// codes below 16, else VG consecutive lanes write one row's VG vertices
-// (4*VG contiguous bytes), 256/VG rows per store.
-template <int T, int K>
-__device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
+// Light vertices, one word group (W words, 64*W rows) per pass, by
+// workgroups of 256*W threads; a workgroup takes chunks of 64 consecutive
+// vertices: emit_scan_chunk, then the write phase -- emit_write_chunk per
+// word (256 lanes each) for codes below 16, else 64 consecutive lanes write
+// one row's 64 vertices (256 contiguous bytes), 4W rows per store.
+//
+// Static schedule (DYN false): chunk bid, bid + nb, ... of word group grp.
+// Dynamic (DYN): the team takes (word group, chunk) pairs of ngrp groups from
+// the shared counter *dyn, so no workgroup idles behind a slower one.
+template <int T, int K, int W, bool DYN>
+__device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
                                  const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
                                  int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out,
-                                 int32_t* __restrict__ par_out, RowMap map, unsigned* dyn = nullptr,
-                                 int32_t ngrp = 0) {
-  constexpr int W = kGroupWords<T>;
+                                 int32_t* __restrict__ par_out, RowMap map, unsigned* dyn, int32_t ngrp) {
+  constexpr int NT = kBlock * W;  // threads per workgroup
   constexpr int PART = 4;
-  constexpr int VG = 64 / W;
-  constexpr int LD = VG + 1;  // tile row pitch
+  constexpr int LD = 65;  // tile row pitch
   constexpr int KL = K < 4 ? K : 4;
-  static_assert(64 * W * LD <= kEmitTileInts, "emit tile");
-  EmitLds& L = g_lds.e;
+  static_assert(kPlaneGroup<T> % W == 0, "emit words per pass must divide the plane group");
   const int tid = threadIdx.x;
   const int w = tid % W;
   const int p = (tid / W) % PART;
   const int vi = tid / (W * PART);
-  int t = grp * W + w;                // this lane's word of the tile
-  const int wcol = tid % VG;          // write phase: vertex column
-  const int wrow = tid / VG;          // write phase: first row
+  int t = grp * W + w;           // this lane's word of the tile
+  const int wcol = tid % 64;     // write phase: vertex column
+  const int wrow = tid / 64;     // write phase: first row
+  const int64_t nchunks = ((int64_t)n + 63) / 64;
   __syncthreads();
-  // Static schedule: chunk bid, bid + nb, ... of word group grp.  Dynamic
-  // (dyn non-null): the team takes (word group, chunk) pairs of ngrp groups
-  // from a shared counter, so no workgroup idles behind a slower one.
-  const int64_t nchunks = ((int64_t)n + VG - 1) / VG;
   for (int64_t it = g.bid;; it += g.nb) {
     int64_t vb;
-    if (dyn) {
+    if (DYN) {
       if (tid == 0) L.next = (int64_t)atomicAdd(dyn, 1u);
       __syncthreads();
       const int64_t c = L.next;
       if (c >= (int64_t)ngrp * nchunks) break;
       grp = (int32_t)(c / nchunks);
-      vb = (c % nchunks) * VG;
+      vb = (c % nchunks) * 64;
       t = grp * W + w;
     } else {
-      vb = it * VG;
+      vb = it * 64;
       if (vb >= n) break;
     }
     emit_scan_chunk<T, K, W>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
     __syncthreads();
 #if XB_EMIT_VEC_STORE
-    if constexpr (W == 1 && K <= 4) {
-      emit_write_chunk<K>(tid, vb, grp, rows, src, n, L.tile, L.pl, lvl_out, par_out, map);
+    if constexpr (K <= 4) {
+      const int w2 = tid / kBlock;
+      emit_write_chunk<K>(tid % kBlock, vb, grp * W + w2, rows, src, n, L.tile + w2 * 64 * LD, L.pl[w2], lvl_out,
+                          par_out, map);
       __syncthreads();
       continue;
     }
@@ -1052,7 +1056,7 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
       int wq = -1;
       uint64_t q[K];
 #pragma unroll 4
-      for (int rr = wrow; rr < 64 * W; rr += 256 / VG) {
+      for (int rr = wrow; rr < 64 * W; rr += NT / 64) {
         const int w2 = rr >> 6;
         const int bit = rr & 63;
         const int r = 64 * (grp * W + w2) + bit;
@@ -1061,7 +1065,7 @@ __device__ void emit_light_phase(Grid g, BPState s, int32_t grp, int32_t rows,
           wq = w2;
           if (K <= 4) {
 #pragma unroll
-            for (int k = 0; k < KL; ++k) q[k] = L.pl[k][wcol * W + w2];
+            for (int k = 0; k < KL; ++k) q[k] = L.pl[w2][k][wcol];
           } else {
             load_planes<T, K>(s, s.new_of[v2], grp * W + w2, q);
           }
@@ -1133,23 +1137,24 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
   clear_phase<T>(self_grid(), s, cur, qv, qn);
 }
 
-// Minimum workgroups per CU the standalone emit's register budget is sized
-// for: 8 (64 VGPRs) hides more of its dependent gathers than the 6 waves per
-// SIMD its natural 76 VGPRs allow (C3: 27.5 -> 24.8 ms of emit per step).
-#ifndef XB_EMIT_MIN_BLOCKS
-#define XB_EMIT_MIN_BLOCKS 8
+// Waves per SIMD the standalone emit's register budget is sized for: 8 (64
+// VGPRs) hides more of its dependent gathers than the 6 waves per SIMD its
+// natural 76 VGPRs allow (LiveJournal-shaped graph: 27.5 -> 24.8 ms of emit
+// per 1,024-source step).
+#ifndef XB_EMIT_WAVES_PER_SIMD
+#define XB_EMIT_WAVES_PER_SIMD 8
 #endif
 
+// Standalone light emit of word group `grp` (words grp*W .. grp*W + W - 1,
+// W = kEmitWords<T>); its LDS is its own, not the shared union.
 template <int T, int K>
-__global__ __launch_bounds__(kBlock, XB_EMIT_MIN_BLOCKS) void k_bp_emit_light(BPState s, int32_t word, int32_t rows,
-                                                          const int32_t* __restrict__ src,
-                                                          const int32_t* __restrict__ in_off,
-                                                          const int32_t* __restrict__ in_col,
-                                                          const int32_t* __restrict__ in_col_orig, int32_t n,
-                                                          int32_t heavy_cut, int32_t* __restrict__ lvl_out,
-                                                          int32_t* __restrict__ par_out, RowMap map) {
-  emit_light_phase<T, K>(self_grid(), s, word, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out,
-                         map);
+__global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / kEmitWords<T>) void k_bp_emit_light(
+    BPState s, int32_t grp, int32_t rows, const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
+    const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
+    int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
+  __shared__ EmitLdsT<kEmitWords<T>> lds;
+  emit_light_phase<T, K, kEmitWords<T>, false>(self_grid(), lds, s, grp, rows, src, in_off, in_col, in_col_orig, n,
+                                               heavy_cut, lvl_out, par_out, map, nullptr, 0);
 }
 
 // ============================================================================
@@ -1358,8 +1363,9 @@ __device__ __forceinline__ void nl_emit_light(Grid g, BPState s, int32_t word, i
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
                                            int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out,
                                            unsigned* dyn, int32_t ngrp) {
-  emit_light_phase<T, K>(uni(g), uni(s), uni(word), uni(rows), uni(src), uni(in_off), uni(in_col), uni(in_col_orig),
-                         uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{}, uni(dyn), uni(ngrp));
+  emit_light_phase<T, K, 1, true>(uni(g), g_lds.e, uni(s), uni(word), uni(rows), uni(src), uni(in_off), uni(in_col),
+                                  uni(in_col_orig), uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{},
+                                  uni(dyn), uni(ngrp));
 }
 
 template <int T, int K>
@@ -1370,7 +1376,7 @@ __device__ bool emit_all_words(Grid g, BPState s, const GraphView& gv, const Til
   // barrier that precedes it (the reset is visible to the team after it).
   if (g.bid == 0 && threadIdx.x == 0) atomicExch(&ts.ctl->emit_chunk, 0u);
   if (!team_barrier(ts, g, gerr, XB_K_EMIT)) return false;
-  const int32_t ngrp = (words + kGroupWords<T> - 1) / kGroupWords<T>;
+  const int32_t ngrp = words;  // one word per pass
   nl_emit_light<T, K>(g, s, 0, job.cnt, job.src, gv.oin_off, gv.oin_col, gv.oin_colo, gv.n, gv.cut_medium,
                       job.lvl_out, job.par_out, &ts.ctl->emit_chunk, ngrp);
   return true;
@@ -2455,14 +2461,14 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 }
 
 template <int T, int K>
-static void emit_light_launch(hipStream_t st, BPState s, int32_t word, int32_t rows, const int32_t* src,
+static void emit_light_launch(hipStream_t st, BPState s, int32_t grp, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                               int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
-  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64 / kGroupWords<T>)), dim3(kBlock), 0, st, s, word,
+  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s, grp,
                      rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
 }
 
-int emit_group_words(int T) { return T < XB_EMIT_GROUP_MAX ? T : XB_EMIT_GROUP_MAX; }
+int emit_pass_words(int T) { return T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS; }
 
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
