@@ -65,6 +65,12 @@ extern "C" {
 #define XB_MATCH_REGROW 16u /* xb_match_run: regrow the whole forest every
                                phase instead of keeping the trees of roots
                                that are still exposed                       */
+#define XB_NO_PIPELINE 32u /* xb_msbfs_run: run the tiles of a multi-tile
+                              batched run one after another on one stream
+                              (default with XB_OUT_DEVICE: tiles alternate
+                              between two workspaces and streams, so a
+                              tile's traversal overlaps the previous tile's
+                              emit; costs a second tile workspace)          */
 
 typedef struct xb_graph xb_graph;
 
@@ -328,10 +334,11 @@ int32_t xb_match_run(xb_graph* g, const int32_t* init_mate, int32_t* mate_out, u
  * (the |E|/20 push/pull switch of apps/ligra/ligra/ligra.h:236-277 and the
  * XB engine's fixed launch shapes).  Coordinate descent over words per
  * tile (1, 2, 4, 8, 16), the push/pull switch alpha and the pull bucket
- * cut-offs: each trial runs one tile of the caller's sample sources (the
- * first 64*words_per_tile of them) on the device and is scored in
- * milliseconds per source (best of two runs); a value replaces the current
- * one only if it is at least 2% faster.  The trials write into two scratch
+ * cut-offs: each trial runs the caller's sample sources (at most 1,024 of
+ * them) on the device as xb_msbfs_run would, narrower tiles as several tiles
+ * on two streams, and is scored in milliseconds per source (best of two
+ * runs); a value replaces the current one only if it is at least 2% faster.
+ * The trials write into two scratch
  * tables of min(S, 1024) x n int32 (fewer rows if those would exceed half
  * the free device memory) that are freed on return; the sample's results
  * are discarded.  *best receives the tuned options (mode

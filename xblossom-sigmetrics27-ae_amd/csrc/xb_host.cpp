@@ -230,6 +230,13 @@ struct xb_graph {
   int32_t* d_hidx = nullptr;   // engine id -> row of a heavy vertex (in-degree > cut_medium) in hpar, else -1
   int32_t n_heavy = 0;
   Workspace ws;
+  // Second workspace of the batched engine, with its own stream: a run of
+  // several tiles alternates the two, so one tile's level loop overlaps the
+  // previous tile's emit.  Created by the first such run (ensure_ws2).
+  hipStream_t stream2 = nullptr;
+  std::unique_ptr<Pinned> pinned2;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  Workspace ws2;
   // Forest-mode state (about 90 B per vertex), kept from the first matching
   // run until the graph is destroyed.
   std::shared_ptr<void> match_state;
@@ -238,6 +245,10 @@ struct xb_graph {
 
   ~xb_graph() {
     ws.free_all();
+    ws2.free_all();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (d_eoff != d_ein_off) dfree(d_eoff);
     if (d_ecol != d_ein_col) dfree(d_ecol);
     dfree(d_ein_off);
@@ -805,6 +816,18 @@ struct Profiler {
     open_cat = -1;
   }
   void add(int cat, double b) { bytes[cat] += b; }
+  // Takes over another profiler's marks and counts (the second workspace's
+  // stream of a pipelined run), so one collect() reports both.
+  void absorb(Profiler& o) {
+    marks.insert(marks.end(), o.marks.begin(), o.marks.end());
+    o.marks.clear();
+    for (int c = 0; c < XB_K_CLASSES; ++c) {
+      bytes[c] += o.bytes[c];
+      launches[c] += o.launches[c];
+      o.bytes[c] = 0;
+      o.launches[c] = 0;
+    }
+  }
   void collect(xb_run_stats* s) {
     for (auto& mk : marks) {
       float t = 0;
@@ -1050,30 +1073,72 @@ int tile_words(int32_t cnt, int Tmax) {
   return t;
 }
 
-void run_batched(xb_graph* g, Workspace& w, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl, int32_t* par,
-                 bool dev_out, Profiler& prof, xb_run_stats* st) {
+// Whether a second workspace for tiles of T words fits: it is already there,
+// or twice its size (tile state 64 B per vertex and word, queues, heavy
+// parents) is free.  A run that does not fit keeps to one stream.
+bool ws2_fits(const xb_graph* g, int T) {
+  if (g->stream2 && g->ws2.T >= T) return true;
+  const double need = 64.0 * T * g->n + 24.0 * g->n + 256.0 * T * std::max(g->n_heavy, 1);
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  return 2.0 * need <= (double)fr;
+}
+
+// The second workspace and its stream (see xb_graph::ws2).
+void ensure_ws2(xb_graph* g) {
+  if (g->stream2) return;
+  HIPCHK(hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking));
+  g->pinned2.reset(new Pinned());
+  HIPCHK(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
+  g->ws2.st = g->stream2;
+  g->ws2.hctr = g->pinned2->p;
+}
+
+// Tiles of up to 64*T sources, one after another.  With a second workspace
+// (w2, prof2 on its stream) and device outputs, tile k runs on workspace
+// k % 2: a tile's level loop, which waits on the host for every level's
+// frontier count, then overlaps the emit its predecessor left queued on the
+// other stream, and the gather-bound traversal shares the device with the
+// write-bound emit.  The second stream starts after the caller's earlier work
+// on the first, and the first waits for the second at the end, so the run is
+// ordered on w.st like a one-stream run.
+void run_batched(xb_graph* g, Workspace& w, const BPOpts& o, const int32_t* sources, int32_t S, int32_t* lvl,
+                 int32_t* par, bool dev_out, Profiler& prof, xb_run_stats* st, Workspace* w2 = nullptr,
+                 Profiler* prof2 = nullptr) {
   const int32_t tile = 64 * o.T;
   const int Tneed = tile_words(std::min(S, tile), o.T);  // the widest tile of this run
-  ensure_queues(g, w);
-  ensure_bp(g, w, Tneed);
+  const bool pipe = w2 != nullptr && prof2 != nullptr && dev_out && S > tile;
+  Workspace* ws[2] = {&w, pipe ? w2 : &w};
+  Profiler* pr[2] = {&prof, pipe ? prof2 : &prof};
   ensure_layout(g, o);
-  ensure_hpar(g, w, Tneed);
+  for (int i = 0; i < (pipe ? 2 : 1); ++i) {
+    ensure_queues(g, *ws[i]);
+    ensure_bp(g, *ws[i], Tneed);
+    ensure_hpar(g, *ws[i], Tneed);
+    ensure_src(*ws[i], tile);
+  }
   const int32_t n = g->n;
-  ensure_src(w, tile);
   int32_t* stage = nullptr;
   if (!dev_out) stage = ensure_stage(w, 2 * (int64_t)tile * n);
-  for (int32_t base = 0; base < S; base += tile) {
+  if (pipe) {
+    HIPCHK(hipEventRecord(g->ev_fork, w.st));
+    HIPCHK(hipStreamWaitEvent(w2->st, g->ev_fork, 0));
+  }
+  for (int32_t base = 0, k = 0; base < S; base += tile, ++k) {
+    Workspace& wk = *ws[k & 1];
+    Profiler& pk = *pr[k & 1];
     const int32_t cnt = std::min(tile, S - base);
     int32_t* lr = dev_out ? (lvl ? lvl + (int64_t)base * n : nullptr) : (lvl ? stage : nullptr);
-    int32_t* pr = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
+    int32_t* pr_rows = dev_out ? par + (int64_t)base * n : stage + (int64_t)tile * n;
     BPOpts ot = o;
     ot.T = tile_words(cnt, o.T);
-    bool ok = run_bp_tile(g, w, ot, sources + base, cnt, lr, pr, xb::RowMap{}, prof, st);
+    bool ok = run_bp_tile(g, wk, ot, sources + base, cnt, lr, pr_rows, xb::RowMap{}, pk, st);
     if (!ok) {
       // Deeper than 254 levels: these sources take the plain path.
       if (st) st->fallback_sources += cnt;
-      run_plain(g, w, sources + base, cnt, lvl ? lvl + (int64_t)base * n : nullptr, par + (int64_t)base * n,
-                dev_out, prof, st);
+      run_plain(g, wk, sources + base, cnt, lvl ? lvl + (int64_t)base * n : nullptr, par + (int64_t)base * n,
+                dev_out, pk, st);
       continue;
     }
     if (!dev_out) {
@@ -1084,6 +1149,10 @@ void run_batched(xb_graph* g, Workspace& w, const BPOpts& o, const int32_t* sour
                             hipMemcpyDeviceToHost, w.st));
       HIPCHK(hipStreamSynchronize(w.st));
     }
+  }
+  if (pipe) {
+    HIPCHK(hipEventRecord(g->ev_join, w2->st));
+    HIPCHK(hipStreamWaitEvent(w.st, g->ev_join, 0));
   }
 }
 
@@ -1272,7 +1341,16 @@ int32_t xb_msbfs_run(xb_graph* g, const int32_t* sources, int32_t S, int32_t* le
       run_plain(g, g->ws, sources, S, levels, parents, dev_out, prof, stats);
     } else {
       const BPOpts bo = bp_opts(o);
-      run_batched(g, g->ws, bo, sources, S, levels, parents, dev_out, prof, stats);
+      Profiler prof2;
+      prof2.on = prof.on;
+      Workspace* w2 = nullptr;
+      if (!(o.flags & XB_NO_PIPELINE) && dev_out && S > 64 * bo.T && ws2_fits(g, bo.T)) {
+        ensure_ws2(g);
+        w2 = &g->ws2;
+        prof2.st = g->stream2;
+      }
+      run_batched(g, g->ws, bo, sources, S, levels, parents, dev_out, prof, stats, w2, &prof2);
+      prof.absorb(prof2);
     }
     HIPCHK(hipEventRecord(e1, g->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -2457,21 +2535,28 @@ struct TuneRun {
   int32_t* dp = nullptr;
   xb_tune_report* rep;
 
-  // Milliseconds per source of one tile of the sample (best of two runs,
-  // after one untimed run that also rebuilds the layout for new cut-offs).
+  // Milliseconds per source of the whole sample (best of two runs, after
+  // one untimed run that also rebuilds the layout for new cut-offs), run as
+  // xb_msbfs_run runs it: narrower tiles are several tiles on two streams.
   double measure(const xb_msbfs_opts& o) {
     const BPOpts bo = bp_opts(o);
-    const int32_t cnt = std::min(S, 64 * bo.T);
-    Profiler prof;
+    const int32_t cnt = S;
+    Profiler prof, prof2;
     prof.st = g->stream;
-    run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr);
+    Workspace* w2 = nullptr;
+    if (cnt > 64 * bo.T && ws2_fits(g, bo.T)) {
+      ensure_ws2(g);
+      w2 = &g->ws2;
+      prof2.st = g->stream2;
+    }
+    run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr, w2, &prof2);
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     double best = 1e300;
     for (int rep_i = 0; rep_i < 2; ++rep_i) {
       HIPCHK(hipEventRecord(a, g->stream));
-      run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr);
+      run_batched(g, g->ws, bo, sample, cnt, dl, dp, true, prof, nullptr, w2, &prof2);
       HIPCHK(hipEventRecord(b, g->stream));
       HIPCHK(hipEventSynchronize(b));
       float ms = 0;

@@ -1017,70 +1017,79 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
   constexpr int LD = 65;  // tile row pitch
   constexpr int KL = K < 4 ? K : 4;
   static_assert(kPlaneGroup<T> % W == 0, "emit words per pass must divide the plane group");
+  // Dynamic schedule: a workgroup takes a chunk of a whole plane group and
+  // passes its SUB word groups back to back, so the later passes find the
+  // in-neighbours' plane lines (shared by the group's words) in the L2.
+  constexpr int SUB = DYN ? kPlaneGroup<T> / W : 1;
   const int tid = threadIdx.x;
   const int w = tid % W;
   const int p = (tid / W) % PART;
   const int vi = tid / (W * PART);
-  int t = grp * W + w;           // this lane's word of the tile
   const int wcol = tid % 64;     // write phase: vertex column
   const int wrow = tid / 64;     // write phase: first row
   const int64_t nchunks = ((int64_t)n + 63) / 64;
+  const int32_t nsup = (ngrp + SUB - 1) / SUB;  // plane groups (dynamic schedule)
   __syncthreads();
   for (int64_t it = g.bid;; it += g.nb) {
     int64_t vb;
+    int32_t grp0 = grp;
     if (DYN) {
       if (tid == 0) L.next = (int64_t)atomicAdd(dyn, 1u);
       __syncthreads();
       const int64_t c = L.next;
-      if (c >= (int64_t)ngrp * nchunks) break;
-      grp = (int32_t)(c / nchunks);
+      if (c >= (int64_t)nsup * nchunks) break;
+      grp0 = (int32_t)(c / nchunks) * SUB;
       vb = (c % nchunks) * 64;
-      t = grp * W + w;
     } else {
       vb = it * 64;
       if (vb >= n) break;
     }
-    emit_scan_chunk<T, K, W>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
-    __syncthreads();
-#if XB_EMIT_VEC_STORE
-    if constexpr (K <= 4) {
-      const int w2 = tid / kBlock;
-      emit_write_chunk<K>(tid % kBlock, vb, grp * W + w2, rows, src, n, L.tile + w2 * 64 * LD, L.pl[w2], lvl_out,
-                          par_out, map);
+    for (int sub = 0; sub < SUB; ++sub) {
+      const int32_t gq = grp0 + sub;  // this pass's word group
+      if (DYN && gq >= ngrp) break;
+      const int t = gq * W + w;       // this lane's word of the tile
+      emit_scan_chunk<T, K, W>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
       __syncthreads();
-      continue;
-    }
-#endif
-    const int32_t v2 = (int32_t)(vb + wcol);
-    if (v2 < n) {
-      int wq = -1;
-      uint64_t q[K];
-#pragma unroll 4
-      for (int rr = wrow; rr < 64 * W; rr += NT / 64) {
-        const int w2 = rr >> 6;
-        const int bit = rr & 63;
-        const int r = 64 * (grp * W + w2) + bit;
-        if (r >= rows) break;
-        if (w2 != wq) {
-          wq = w2;
-          if (K <= 4) {
-#pragma unroll
-            for (int k = 0; k < KL; ++k) q[k] = L.pl[w2][k][wcol];
-          } else {
-            load_planes<T, K>(s, s.new_of[v2], grp * W + w2, q);
-          }
-        }
-        uint32_t c = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
-        const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[rr * LD + wcol]));
-        const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v2;
-        const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v2;
-        if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[lo]);
-        __builtin_nontemporal_store(par, &par_out[po]);
+#if XB_EMIT_VEC_STORE
+      if constexpr (K <= 4) {
+        const int w2 = tid / kBlock;
+        emit_write_chunk<K>(tid % kBlock, vb, gq * W + w2, rows, src, n, L.tile + w2 * 64 * LD, L.pl[w2], lvl_out,
+                            par_out, map);
+        __syncthreads();
+        continue;
       }
+#endif
+      const int32_t v2 = (int32_t)(vb + wcol);
+      if (v2 < n) {
+        int wq = -1;
+        uint64_t q[K];
+#pragma unroll 4
+        for (int rr = wrow; rr < 64 * W; rr += NT / 64) {
+          const int w2 = rr >> 6;
+          const int bit = rr & 63;
+          const int r = 64 * (gq * W + w2) + bit;
+          if (r >= rows) break;
+          if (w2 != wq) {
+            wq = w2;
+            if (K <= 4) {
+#pragma unroll
+              for (int k = 0; k < KL; ++k) q[k] = L.pl[w2][k][wcol];
+            } else {
+              load_planes<T, K>(s, s.new_of[v2], gq * W + w2, q);
+            }
+          }
+          uint32_t c = 0;
+#pragma unroll
+          for (int k = 0; k < K; ++k) c |= (uint32_t)((q[k] >> bit) & 1ull) << k;
+          const int32_t par = (c == 0) ? -1 : (c == 1 ? v2 : (c == 2 ? src[r] : L.tile[rr * LD + wcol]));
+          const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v2;
+          const int64_t po = (map.par_row ? map.par_row[r] : (int64_t)r * n) + v2;
+          if (lvl_out) __builtin_nontemporal_store((int32_t)c - 1, &lvl_out[lo]);
+          __builtin_nontemporal_store(par, &par_out[po]);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

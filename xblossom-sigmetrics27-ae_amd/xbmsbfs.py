@@ -51,6 +51,7 @@ XB_PROFILE = 2
 XB_NO_LEVELS = 4
 XB_FIXED_TEAMS = 8
 XB_MATCH_REGROW = 16
+XB_NO_PIPELINE = 32
 
 # Every symbol include/xb/xb_capi.h declares.
 EXPORTED_SYMBOLS = (
@@ -368,12 +369,14 @@ class XBlossomGpuEngine10:
 
     def MultiSourceBFS(self, sources, levels=None, parents=None, *, mode: int = XB_MODE_AUTO,
                        device_out: bool = False, profile: bool = False, want_levels: bool = True,
-                       **tunables):
+                       pipeline: bool = True, **tunables):
         """Runs len(sources) BFS traversals.
 
         Host mode (default): returns (levels, parents) as S x n int32 numpy
         arrays.  Device mode: ``levels``/``parents`` are device addresses
-        (ints, e.g. ``tensor.data_ptr()``) of S x n int32 buffers.
+        (ints, e.g. ``tensor.data_ptr()``) of S x n int32 buffers; a run of
+        several tiles then alternates two workspaces and streams unless
+        ``pipeline`` is False (XB_NO_PIPELINE).
         """
         lib = load_engine()
         src = np.ascontiguousarray(sources, dtype=np.int32)
@@ -381,6 +384,8 @@ class XBlossomGpuEngine10:
         flags = (XB_OUT_DEVICE if device_out else 0) | (XB_PROFILE if profile else 0)
         if not want_levels:
             flags |= XB_NO_LEVELS
+        if not pipeline:
+            flags |= XB_NO_PIPELINE
         if not device_out:
             if parents is None:
                 parents = np.empty((S, n), dtype=np.int32)
