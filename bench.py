@@ -841,7 +841,7 @@ def main() -> int:
         cpu = cpu_baseline(g, src, args.cpu_rounds)
         log(f"cpu baseline: {cpu}")
 
-    # Serving on the web-crawl-shaped graph: the C3 tables are freed first.
+    # Serving on the web-crawl-shaped graph: the LiveJournal-shaped tables are freed first.
     serving = None
     if not args.no_serving and args.workload == "livejournal":
         if batch is not None:

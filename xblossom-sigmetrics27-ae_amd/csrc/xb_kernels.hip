@@ -1117,7 +1117,8 @@ __global__ __launch_bounds__(kBlock) void k_bp_commit(BPState s, int cur, uint32
 
 // Minimum workgroups per CU the pull's register budget is sized for: 8 (64
 // VGPRs, 8 waves per SIMD) hides more of its dependent gathers than the 7 its
-// natural 65-69 VGPRs allow (C3: 13.2 -> 12.7 ms of pull per step).
+// natural 65-69 VGPRs allow (LiveJournal-shaped graph: 13.2 -> 12.7 ms of pull
+// per 1,024-source step).
 #ifndef XB_PULL_MIN_BLOCKS
 #define XB_PULL_MIN_BLOCKS 8
 #endif
