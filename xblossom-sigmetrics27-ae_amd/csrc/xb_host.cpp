@@ -1045,8 +1045,7 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
   for (int grp = 0; grp * W < words; ++grp) {
     prof.begin(XB_K_EMIT);
     xb::launch_bp_emit(w.st, T, s, grp, K, cnt, w.d_src, g->d_oin_off, g->d_oin_col, g->d_oin_colo, n, o.cut_medium,
-                       lvl_rows,
-                       par_rows, rows);
+                       lvl_rows, par_rows, rows, (g->flags & XB_GRAPH_DIRECTED) == 0);
     prof.end();
     const int rows_g = std::min(64 * W, cnt - 64 * W * grp);
     // level + parent int32 per (source, vertex) of this group's rows

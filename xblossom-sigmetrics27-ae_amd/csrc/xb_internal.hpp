@@ -181,11 +181,13 @@ void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, 
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);
 // src: the tile's source ids (level-1 parents are the sources themselves).
 // Light-vertex emit of word group `grp` (words grp*W .. grp*W+W-1, W =
-// emit_pass_words(T)).
+// emit_pass_words(T)); symmetric: the graph's in- and out-lists are the same
+// (a two-plane level test per in-neighbour).
 int emit_pass_words(int T);
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
-                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map);
+                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map,
+                    bool symmetric);
 
 // ---- forest mode (alternating-forest search of maximum matching) ----
 // Pool of pending second calls of the path flip's recursion (x, y, next

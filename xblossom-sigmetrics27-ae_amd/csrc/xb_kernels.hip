@@ -844,7 +844,12 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
 // (pitch 65); heavy vertices copy their recorded parents into it.  Planes 0-3
 // of each (word, vertex) go to pl[w][k][vi] for the write phase.  No block
 // barrier inside: lanes write disjoint tile entries.
-template <int T, int K, int W>
+//
+// SYM (symmetric graph): an in-neighbour of a vertex reached at level L is
+// reached at level L-1, L or L+1, three codes that differ modulo 4, so
+// planes 0-1 alone decide whether it is one level closer: one 16-byte gather
+// per in-neighbour instead of the 32 bytes of planes 0-3.
+template <int T, int K, int W, bool SYM>
 __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t vb, int w, int p, int vi,
                                                 const int32_t* __restrict__ in_off,
                                                 const int32_t* __restrict__ in_col,
@@ -895,11 +900,12 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
     for (int i = 0; i < kEmitU; ++i) {
       uint64_t eq = 0;
       if (us[i] >= 0) {
-        uint64_t pu[K];
-        load_planes<T, K>(s, us[i], t, pu);
+        constexpr int KC = (SYM && K > 2) ? 2 : K;  // planes compared
+        uint64_t pu[KC];
+        load_planes<T, KC>(s, us[i], t, pu);
         eq = ~0ull;
 #pragma unroll
-        for (int k = 0; k < K; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
+        for (int k = 0; k < KC; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
       }
       eqs[i] = eq;
     }
@@ -1006,7 +1012,7 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
 // Static schedule (DYN false): chunk bid, bid + nb, ... of word group grp.
 // Dynamic (DYN): the team takes (word group, chunk) pairs of ngrp groups from
 // the shared counter *dyn, so no workgroup idles behind a slower one.
-template <int T, int K, int W, bool DYN>
+template <int T, int K, int W, bool DYN, bool SYM>
 __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp, int32_t rows,
                                  const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
                                  const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
@@ -1048,7 +1054,7 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
       const int32_t gq = grp0 + sub;  // this pass's word group
       if (DYN && gq >= ngrp) break;
       const int t = gq * W + w;       // this lane's word of the tile
-      emit_scan_chunk<T, K, W>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
+      emit_scan_chunk<T, K, W, SYM>(s, t, vb, w, p, vi, in_off, in_col, in_col_orig, n, heavy_cut, L.tile, L.pl);
       __syncthreads();
 #if XB_EMIT_VEC_STORE
       if constexpr (K <= 4) {
@@ -1156,15 +1162,19 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
 #endif
 
 // Standalone light emit of word group `grp` (words grp*W .. grp*W + W - 1,
-// W = kEmitWords<T>); its LDS is its own, not the shared union.
-template <int T, int K>
+// W = kEmitWords<T>); its LDS is its own, not the shared union.  SYM: the
+// graph is symmetric (two-plane neighbour compare).  The bound is in waves
+// per SIMD: 8 for one word per pass (64 VGPRs); 4 for two, a ceiling the
+// kernels for codes below 16 stay well under (at most 64 VGPRs, so 8 waves
+// run), while the deep-level ones take 74-90 VGPRs instead of spilling.
+template <int T, int K, bool SYM>
 __global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / kEmitWords<T>) void k_bp_emit_light(
     BPState s, int32_t grp, int32_t rows, const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
     const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
     int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
   __shared__ EmitLdsT<kEmitWords<T>> lds;
-  emit_light_phase<T, K, kEmitWords<T>, false>(self_grid(), lds, s, grp, rows, src, in_off, in_col, in_col_orig, n,
-                                               heavy_cut, lvl_out, par_out, map, nullptr, 0);
+  emit_light_phase<T, K, kEmitWords<T>, false, SYM>(self_grid(), lds, s, grp, rows, src, in_off, in_col,
+                                                    in_col_orig, n, heavy_cut, lvl_out, par_out, map, nullptr, 0);
 }
 
 // ============================================================================
@@ -1373,7 +1383,8 @@ __device__ __forceinline__ void nl_emit_light(Grid g, BPState s, int32_t word, i
                                            const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
                                            int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out,
                                            unsigned* dyn, int32_t ngrp) {
-  emit_light_phase<T, K, 1, true>(uni(g), g_lds.e, uni(s), uni(word), uni(rows), uni(src), uni(in_off), uni(in_col),
+  emit_light_phase<T, K, 1, true, false>(uni(g), g_lds.e, uni(s), uni(word), uni(rows), uni(src), uni(in_off),
+                                         uni(in_col),
                                   uni(in_col_orig), uni(n), uni(heavy_cut), uni(lvl_out), uni(par_out), RowMap{},
                                   uni(dyn), uni(ngrp));
 }
@@ -2473,18 +2484,22 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t grp, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
-                              int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
-  hipLaunchKernelGGL((k_bp_emit_light<T, K>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s, grp,
-                     rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+                              int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
+  if (symmetric)
+    hipLaunchKernelGGL((k_bp_emit_light<T, K, true>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
+                       grp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+  else
+    hipLaunchKernelGGL((k_bp_emit_light<T, K, false>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
+                       grp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
 }
 
 int emit_pass_words(int T) { return T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS; }
 
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
-                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map) {
+                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
   XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, in_col_orig, n,
-                                                               heavy_cut, lvl_out, par_out, map))));
+                                                               heavy_cut, lvl_out, par_out, map, symmetric))));
 }
 
 int persistent_blocks_per_cu() {
