@@ -889,25 +889,69 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
     if (!__any(more)) break;
     int32_t us[kEmitU], uo[kEmitU];
     uint64_t eqs[kEmitU];
+    if constexpr (W == 2) {
+      // The two word lanes of a (vertex, partition) are the lane pair 2q,
+      // 2q+1 and read the same in-list position: one loads its engine id,
+      // the other its original id, and a DPP swap hands each lane the other
+      // half (one index load per position instead of two per lane).
+      // Both DPP moves run with every lane of the wave active (the loop is
+      // wave-uniform), so each lane reads a live partner.
+      const int mo = __builtin_amdgcn_update_dpp(0, (int)more, quad_sel(1, 0, 3, 2), 0xF, 0xF, false);
+      const bool pm = more || mo != 0;
+      int32_t x[kEmitU];
 #pragma unroll
-    for (int i = 0; i < kEmitU; ++i) {
-      const int32_t j = b + k0 + i * PART + p;
-      const bool in = more && j < e;
-      us[i] = in ? in_col[j] : -1;
-      uo[i] = in ? in_col_orig[j] : -1;
-    }
-#pragma unroll
-    for (int i = 0; i < kEmitU; ++i) {
-      uint64_t eq = 0;
-      if (us[i] >= 0) {
-        constexpr int KC = (SYM && K > 2) ? 2 : K;  // planes compared
-        uint64_t pu[KC];
-        load_planes<T, KC>(s, us[i], t, pu);
-        eq = ~0ull;
-#pragma unroll
-        for (int k = 0; k < KC; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
+      for (int i = 0; i < kEmitU; ++i) {
+        const int32_t j = b + k0 + i * PART + p;
+        x[i] = (pm && j < e) ? (w ? in_col_orig : in_col)[j] : -1;
       }
-      eqs[i] = eq;
+#pragma unroll
+      for (int i = 0; i < kEmitU; ++i) {
+        const int32_t y = __builtin_amdgcn_update_dpp(0, x[i], quad_sel(1, 0, 3, 2), 0xF, 0xF, false);
+        const bool in = more && b + k0 + i * PART + p < e;
+        us[i] = in ? (w ? y : x[i]) : -1;
+        uo[i] = in ? (w ? x[i] : y) : -1;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kEmitU; ++i) {
+        const int32_t j = b + k0 + i * PART + p;
+        const bool in = more && j < e;
+        us[i] = in ? in_col[j] : -1;
+        uo[i] = in ? in_col_orig[j] : -1;
+      }
+    }
+    constexpr int KC = (SYM && K > 2) ? 2 : K;  // planes compared
+    if constexpr (KC <= 2) {
+      // The kEmitU neighbours' planes are gathered first and compared after,
+      // so the gathers are in flight together instead of each waiting on its
+      // own (8 VGPRs of planes here; more planes would cost occupancy).
+      uint64_t pu[kEmitU][KC];
+#pragma unroll
+      for (int i = 0; i < kEmitU; ++i) {
+#pragma unroll
+        for (int k = 0; k < KC; ++k) pu[i][k] = 0;
+        if (us[i] >= 0) load_planes<T, KC>(s, us[i], t, pu[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < kEmitU; ++i) {
+        uint64_t eq = us[i] >= 0 ? ~0ull : 0ull;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) eq &= ~(pu[i][k] ^ cm.tk[k]);
+        eqs[i] = eq;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kEmitU; ++i) {
+        uint64_t eq = 0;
+        if (us[i] >= 0) {
+          uint64_t pu[KC];
+          load_planes<T, KC>(s, us[i], t, pu);
+          eq = ~0ull;
+#pragma unroll
+          for (int k = 0; k < KC; ++k) eq &= ~(pu[k] ^ cm.tk[k]);
+        }
+        eqs[i] = eq;
+      }
     }
 #pragma unroll
     for (int i = 0; i < kEmitU; ++i) {
@@ -972,13 +1016,14 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
     const int rr = row0 + RSTEP * it;
     const int r = 64 * grp + rr;
     if (r >= rows) break;
+    const int32_t sr = src[r];  // the parent of level-1 cells: one load per row, not one per cell
     int32_t lv[4], pv[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const uint32_t c = (cw[it] >> (8 * a)) & 0xFFu;
       const int32_t va = (int32_t)(vb + c4 + a);
       lv[a] = (int32_t)c - 1;
-      pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? src[r] : tile[rr * LD + c4 + a]));
+      pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? sr : tile[rr * LD + c4 + a]));
     }
     const int64_t v0 = vb + c4;
     const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v0;
@@ -1164,11 +1209,13 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
 // Standalone light emit of word group `grp` (words grp*W .. grp*W + W - 1,
 // W = kEmitWords<T>); its LDS is its own, not the shared union.  SYM: the
 // graph is symmetric (two-plane neighbour compare).  The bound is in waves
-// per SIMD: 8 for one word per pass (64 VGPRs); 4 for two, a ceiling the
-// kernels for codes below 16 stay well under (at most 64 VGPRs, so 8 waves
-// run), while the deep-level ones take 74-90 VGPRs instead of spilling.
+// per SIMD: 8 (64 VGPRs) for codes below 16, which the 2-word kernels meet
+// with at most 20 bytes of spill (a 65th VGPR would leave 7 waves per SIMD:
+// LiveJournal-shaped graph 18.6 -> 20.3 ms of emit per 1,024-source step);
+// the deep-level 2-word kernels get 4, taking 72-98 VGPRs instead of
+// spilling.
 template <int T, int K, bool SYM>
-__global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / kEmitWords<T>) void k_bp_emit_light(
+__global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / (K <= 4 ? 1 : kEmitWords<T>)) void k_bp_emit_light(
     BPState s, int32_t grp, int32_t rows, const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
     const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
     int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
