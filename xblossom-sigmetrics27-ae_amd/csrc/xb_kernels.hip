@@ -467,15 +467,35 @@ __device__ __forceinline__ void or_word(uint64_t* p, uint64_t bits) {
 }
 template <int T>
 __device__ __forceinline__ void planes_or(const BPState& s, int64_t v, int t, uint32_t code, uint64_t bits) {
-  uint64_t* lo = plane_lo<T>(s, v, t);
+  if (XB_PLANES_ATOMIC) {
+    uint64_t* lo = plane_lo<T>(s, v, t);
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if ((code >> k) & 1u) or_word(&lo[k], bits);
+    for (int k = 0; k < 4; ++k)
+      if ((code >> k) & 1u) or_word(&lo[k], bits);
+    if (code >> 4) {
+      uint64_t* hi = plane_hi<T>(s, v, t);
+#pragma unroll
+      for (int k = 4; k < 8; ++k)
+        if ((code >> k) & 1u) or_word(&hi[k - 4], bits);
+    }
+    return;
+  }
+  // Planes 0-3 (and 4-7) of (v, t) are 32 contiguous, 32-byte aligned
+  // bytes: one vector read-modify-write each, so the lane waits on one load
+  // instead of one per set bit of the code.
+  typedef uint64_t u64x4 __attribute__((ext_vector_type(4)));
+  auto masks = [bits](uint32_t c) {
+    const u64x4 m = {(c & 1u) ? bits : 0ull, (c & 2u) ? bits : 0ull, (c & 4u) ? bits : 0ull,
+                     (c & 8u) ? bits : 0ull};
+    return m;
+  };
+  if (code & 0xFu) {
+    u64x4* lo = reinterpret_cast<u64x4*>(plane_lo<T>(s, v, t));
+    *lo = *lo | masks(code);
+  }
   if (code >> 4) {
-    uint64_t* hi = plane_hi<T>(s, v, t);
-#pragma unroll
-    for (int k = 4; k < 8; ++k)
-      if ((code >> k) & 1u) or_word(&hi[k - 4], bits);
+    u64x4* hi = reinterpret_cast<u64x4*>(plane_hi<T>(s, v, t));
+    *hi = *hi | masks(code >> 4);
   }
 }
 
@@ -722,7 +742,9 @@ __device__ void pull_bucket_phase(Grid g, BPState s, int cur, uint32_t code, int
     const uint64_t bal = __ballot(act && p == 0 && got != 0);
     const uint64_t grp = (bal >> (slot * G)) & ((T == 64) ? ~0ull : ((1ull << T) - 1ull));
     const bool lead = act && (l % G) == 0 && grp != 0;
-    const int32_t dv = lead ? out_off[v + 1] - out_off[v] : 0;
+    // A symmetric graph's out-lists are its in-lists: the out-degree is e - b,
+    // already in registers.
+    const int32_t dv = lead ? (out_off == in_off ? e - b : out_off[v + 1] - out_off[v]) : 0;
     stage_put(sg, lead, v, dv);
     stage_sync_flush(sg, false, nxt);
   }
