@@ -116,11 +116,21 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 // DPP quad permutation of a 64-bit value: lane 4q+i receives lane 4q+sel(i)
 // of its quad (a VALU move, no LDS crossbar round trip).
 constexpr int quad_sel(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+// DPP row shifts inside each 16-lane row: row_shr(n): lane i receives lane
+// i - n, row_shl(n): lane i + n; a lane whose source falls outside its row
+// receives 0.
+constexpr int dpp_row_shr(int n) { return 0x110 + n; }
+constexpr int dpp_row_shl(int n) { return 0x100 + n; }
+// A 64-bit DPP move (every lane of the wave must be active).
 template <int CTRL>
-__device__ __forceinline__ uint64_t quad_perm64(uint64_t x) {
+__device__ __forceinline__ uint64_t dpp64(uint64_t x) {
   const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, false);
   return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t quad_perm64(uint64_t x) {
+  return dpp64<CTRL>(x);
 }
 
 // Wave-aggregated append of v to a plain list with a 32-bit counter.
@@ -986,6 +996,24 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
         excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
         if (p == 0) excl = 0;
         all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
+      } else if constexpr (W == 2) {
+        // The 8 lanes (p, w) of a vertex are an aligned 8-lane group, w the
+        // lane parity: DPP row shifts by 2 and 4 lanes move between
+        // partitions of the same word, with no LDS crossbar round trip.
+        // Every DPP move runs on all lanes; the selects apply the partition
+        // bounds after.
+        const uint64_t y1 = dpp64<dpp_row_shr(2)>(incl);  // partition p-1
+        incl |= (p >= 1) ? y1 : 0ull;
+        const uint64_t y2 = dpp64<dpp_row_shr(4)>(incl);  // partition p-2
+        incl |= (p >= 2) ? y2 : 0ull;
+        excl = dpp64<dpp_row_shr(2)>(incl);
+        if (p == 0) excl = 0;
+        // Partition 3's total: within each quad take quad lanes 2-3 (the
+        // upper quad of the group holds partitions 2-3), then the lower quad
+        // reads the upper one 4 lanes on.
+        const uint64_t q3 = quad_perm64<quad_sel(2, 3, 2, 3)>(incl);
+        const uint64_t q3n = dpp64<dpp_row_shl(4)>(q3);
+        all = (p >= 2) ? q3 : q3n;
       } else {
         uint64_t y = __shfl_up(incl, W, kWave);
         if (p >= 1) incl |= y;
