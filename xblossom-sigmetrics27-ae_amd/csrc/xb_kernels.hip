@@ -439,28 +439,40 @@ template <int T>
 constexpr int kEmitWords = T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS;
 
 // Level-code planes of (v, word t): groups of W = min(T, XB_PLANE_GROUP_MAX)
-// words, vertex-major within a group, so the planes of a vertex's W words of
-// one group are 32W contiguous bytes and one gathered line serves an emit
-// pass of W words.  The emit's words per pass must divide the group.
+// words, vertex-major within a group.  With XB_PLANE_SPLIT (the default) the
+// array holds two halves: planes 0-1 of every (group, vertex) first, then
+// planes 2-3, so planes 0-1 of a vertex's W words of one group are 16W
+// contiguous bytes.  The light emit on a symmetric graph compares planes 0-1
+// of every in-neighbour only: its gathers then move 16 bytes per word, not
+// the 32 of planes 0-3.  Without the split, planes 0-3 of (v, t) are 32
+// contiguous bytes.  Planes 4-7 (planes_hi) use the same layout.  The emit's
+// words per pass must divide the group.
 #ifndef XB_PLANE_GROUP_MAX
 #define XB_PLANE_GROUP_MAX XB_EMIT_WORDS
+#endif
+#ifndef XB_PLANE_SPLIT
+#define XB_PLANE_SPLIT 1
 #endif
 template <int T>
 constexpr int kPlaneGroup = T < XB_PLANE_GROUP_MAX ? T : XB_PLANE_GROUP_MAX;
 
+// Index of plane pair `half` (planes 2*half and 2*half + 1; without the split,
+// of plane 2*half) of (v, t) in a plane array.
 template <int T>
-__device__ __forceinline__ int64_t plane_index(const BPState& s, int64_t v, int t) {
+__device__ __forceinline__ int64_t plane_pair_index(const BPState& s, int64_t v, int t, int half) {
   constexpr int W = kPlaneGroup<T>;
   static_assert(kPlaneGroup<T> % kEmitWords<T> == 0, "emit words per pass must divide the plane group");
-  return (((int64_t)(t / W) * s.n + v) * W + (t % W)) * 4;
+  if (XB_PLANE_SPLIT) {
+    constexpr int G = T / W;
+    return ((((int64_t)half * G + t / W) * s.n + v) * W + (t % W)) * 2;
+  }
+  return (((int64_t)(t / W) * s.n + v) * W + (t % W)) * 4 + 2 * half;
 }
+// Plane k (0-7) of (v, t).
 template <int T>
-__device__ __forceinline__ uint64_t* plane_lo(const BPState& s, int64_t v, int t) {
-  return s.planes + plane_index<T>(s, v, t);
-}
-template <int T>
-__device__ __forceinline__ uint64_t* plane_hi(const BPState& s, int64_t v, int t) {
-  return s.planes_hi + plane_index<T>(s, v, t);
+__device__ __forceinline__ uint64_t* plane_ptr(const BPState& s, int64_t v, int t, int k) {
+  uint64_t* a = k < 4 ? s.planes : s.planes_hi;
+  return a + plane_pair_index<T>(s, v, t, (k >> 1) & 1) + (k & 1);
 }
 
 // OR `bits` into the planes of (v, t) whose bit is set in `code`.  One lane
@@ -478,47 +490,47 @@ __device__ __forceinline__ void or_word(uint64_t* p, uint64_t bits) {
 template <int T>
 __device__ __forceinline__ void planes_or(const BPState& s, int64_t v, int t, uint32_t code, uint64_t bits) {
   if (XB_PLANES_ATOMIC) {
-    uint64_t* lo = plane_lo<T>(s, v, t);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if ((code >> k) & 1u) or_word(&lo[k], bits);
-    if (code >> 4) {
-      uint64_t* hi = plane_hi<T>(s, v, t);
-#pragma unroll
-      for (int k = 4; k < 8; ++k)
-        if ((code >> k) & 1u) or_word(&hi[k - 4], bits);
-    }
+    for (int k = 0; k < 8; ++k)
+      if ((code >> k) & 1u) or_word(plane_ptr<T>(s, v, t, k), bits);
     return;
   }
-  // Planes 0-3 (and 4-7) of (v, t) are 32 contiguous, 32-byte aligned
-  // bytes: one vector read-modify-write each, so the lane waits on one load
-  // instead of one per set bit of the code.
+  // One vector read-modify-write per 16-byte plane pair with a set code bit
+  // (per 32-byte group of planes 0-3 or 4-7 without the split), so the lane
+  // waits on one load per pair instead of one per set bit.
+  typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
   typedef uint64_t u64x4 __attribute__((ext_vector_type(4)));
-  auto masks = [bits](uint32_t c) {
-    const u64x4 m = {(c & 1u) ? bits : 0ull, (c & 2u) ? bits : 0ull, (c & 4u) ? bits : 0ull,
-                     (c & 8u) ? bits : 0ull};
-    return m;
+  auto pair_or = [&](uint64_t* a, uint32_t c) {
+    if (XB_PLANE_SPLIT) {
+      if (c & 3u) {
+        u64x2* p0 = reinterpret_cast<u64x2*>(a + plane_pair_index<T>(s, v, t, 0));
+        const u64x2 m = {(c & 1u) ? bits : 0ull, (c & 2u) ? bits : 0ull};
+        *p0 = *p0 | m;
+      }
+      if (c & 0xCu) {
+        u64x2* p1 = reinterpret_cast<u64x2*>(a + plane_pair_index<T>(s, v, t, 1));
+        const u64x2 m = {(c & 4u) ? bits : 0ull, (c & 8u) ? bits : 0ull};
+        *p1 = *p1 | m;
+      }
+    } else if (c & 0xFu) {
+      u64x4* p4 = reinterpret_cast<u64x4*>(a + plane_pair_index<T>(s, v, t, 0));
+      const u64x4 m = {(c & 1u) ? bits : 0ull, (c & 2u) ? bits : 0ull, (c & 4u) ? bits : 0ull,
+                       (c & 8u) ? bits : 0ull};
+      *p4 = *p4 | m;
+    }
   };
-  if (code & 0xFu) {
-    u64x4* lo = reinterpret_cast<u64x4*>(plane_lo<T>(s, v, t));
-    *lo = *lo | masks(code);
-  }
-  if (code >> 4) {
-    u64x4* hi = reinterpret_cast<u64x4*>(plane_hi<T>(s, v, t));
-    *hi = *hi | masks(code >> 4);
-  }
+  pair_or(s.planes, code & 0xFu);
+  if (code >> 4) pair_or(s.planes_hi, code >> 4);
 }
 
 // The first K planes of (v, t).
 template <int T, int K>
 __device__ __forceinline__ void load_planes(const BPState& s, int64_t v, int t, uint64_t (&pk)[K]) {
-  const uint64_t* lo = plane_lo<T>(s, v, t);
 #pragma unroll
-  for (int k = 0; k < (K < 4 ? K : 4); ++k) pk[k] = lo[k];
-  if (K > 4) {
-    const uint64_t* hi = plane_hi<T>(s, v, t);
-#pragma unroll
-    for (int k = 4; k < K; ++k) pk[k] = hi[k - 4];
+  for (int h = 0; h < (K + 1) / 2; ++h) {
+    const uint64_t* a = (h < 2 ? s.planes : s.planes_hi) + plane_pair_index<T>(s, v, t, h & 1);
+    pk[2 * h] = a[0];
+    if (2 * h + 1 < K) pk[2 * h + 1] = a[1];
   }
 }
 
@@ -563,7 +575,7 @@ __device__ void seed_phase(Grid g, BPState s, const int32_t* __restrict__ src, i
     const int64_t w = (int64_t)v * T + t;
     atomicOr((unsigned long long*)&s.seen[w], (unsigned long long)bit);
     atomicOr((unsigned long long*)&s.front[0][w], (unsigned long long)bit);
-    atomicOr((unsigned long long*)plane_lo<T>(s, v, t), (unsigned long long)bit);  // code 1 = level 0
+    atomicOr((unsigned long long*)plane_ptr<T>(s, v, t, 0), (unsigned long long)bit);  // code 1 = level 0
   }
 }
 
