@@ -77,6 +77,9 @@ constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 #ifndef XB_EMIT_WORDS
 #define XB_EMIT_WORDS 2
 #endif
+#ifndef XB_EMIT_NT
+#define XB_EMIT_NT 1
+#endif
 constexpr int kPullU = XB_PULL_U;
 constexpr int kEmitU = XB_EMIT_U;
 constexpr int kStageCap = 1536;
@@ -1052,6 +1055,15 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
 // is aligned, else one by one; a wave writes 4 rows x 256 contiguous bytes
 // per store instruction.  The level codes of the lane's 4 vertices in its 4
 // rows are packed from the LDS planes once.
+// Output-table store of the light emit (streaming, non-temporal by default).
+template <class V>
+__device__ __forceinline__ void out_store(V v, V* p) {
+  if (XB_EMIT_NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 template <int K>
 __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t grp, int32_t rows,
                                                  const int32_t* __restrict__ src, int32_t n, const int32_t* tile,
@@ -1094,16 +1106,16 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
       typedef int32_t v4i __attribute__((ext_vector_type(4)));
       if (lvl_out) {
         const v4i lq = {lv[0], lv[1], lv[2], lv[3]};
-        __builtin_nontemporal_store(lq, reinterpret_cast<v4i*>(lvl_out + lo));
+        out_store(lq, reinterpret_cast<v4i*>(lvl_out + lo));
       }
       const v4i pq = {pv[0], pv[1], pv[2], pv[3]};
-      __builtin_nontemporal_store(pq, reinterpret_cast<v4i*>(par_out + po));
+      out_store(pq, reinterpret_cast<v4i*>(par_out + po));
     } else {
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         if (v0 + a < n) {
-          if (lvl_out) __builtin_nontemporal_store(lv[a], &lvl_out[lo + a]);
-          __builtin_nontemporal_store(pv[a], &par_out[po + a]);
+          if (lvl_out) out_store(lv[a], &lvl_out[lo + a]);
+          out_store(pv[a], &par_out[po + a]);
         }
       }
     }
