@@ -121,14 +121,14 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 constexpr int quad_sel(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
 // DPP row shifts inside each 16-lane row: row_shr(n): lane i receives lane
 // i - n, row_shl(n): lane i + n; a lane whose source falls outside its row
-// receives 0.
+// receives 0 (bound_ctrl, so no register has to hold the 0 beforehand).
 constexpr int dpp_row_shr(int n) { return 0x110 + n; }
 constexpr int dpp_row_shl(int n) { return 0x100 + n; }
 // A 64-bit DPP move (every lane of the wave must be active).
 template <int CTRL>
 __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, true);
   return ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo;
 }
 template <int CTRL>
@@ -943,7 +943,7 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
       // half (one index load per position instead of two per lane).
       // Both DPP moves run with every lane of the wave active (the loop is
       // wave-uniform), so each lane reads a live partner.
-      const int mo = __builtin_amdgcn_update_dpp(0, (int)more, quad_sel(1, 0, 3, 2), 0xF, 0xF, false);
+      const int mo = __builtin_amdgcn_update_dpp(0, (int)more, quad_sel(1, 0, 3, 2), 0xF, 0xF, true);
       const bool pm = more || mo != 0;
       int32_t x[kEmitU];
 #pragma unroll
@@ -953,7 +953,7 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
       }
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
-        const int32_t y = __builtin_amdgcn_update_dpp(0, x[i], quad_sel(1, 0, 3, 2), 0xF, 0xF, false);
+        const int32_t y = __builtin_amdgcn_update_dpp(0, x[i], quad_sel(1, 0, 3, 2), 0xF, 0xF, true);
         const bool in = more && b + k0 + i * PART + p < e;
         us[i] = in ? (w ? y : x[i]) : -1;
         uo[i] = in ? (w ? x[i] : y) : -1;
@@ -1038,12 +1038,13 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
         if (p == 0) excl = 0;
         all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
       }
-      uint64_t first = c & ~excl;
-      while (first) {
-        const int bit = __builtin_ctzll(first);
-        first &= first - 1;
-        tile[(rb + bit) * LD + vi] = uo[i];
-      }
+      // Rows this position resolves first: one LDS write each, the two
+      // 32-bit halves in loops of their own (a 32-bit find-first and clear
+      // per row instead of their 64-bit forms).
+      const uint64_t first = c & ~excl;
+      int32_t* col = tile + rb * LD + vi;
+      for (uint32_t f = (uint32_t)first; f; f &= f - 1) col[__builtin_ctz(f) * LD] = uo[i];
+      for (uint32_t f = (uint32_t)(first >> 32); f; f &= f - 1) col[(32 + __builtin_ctz(f)) * LD] = uo[i];
       need &= ~all;
     }
   }
@@ -1074,17 +1075,18 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
   constexpr int RSTEP = 16;  // rows per pass: 256 lanes / 16 quads per row
   const int c4 = 4 * (tid % 16);
   const int row0 = tid / 16;
-  uint32_t cw[64 / RSTEP];
+  // Level codes of vertex c4 + a in the lane's rows row0 + 16 * it: the
+  // 4-bit code of row it is bits 16 * it .. 16 * it + 3 of cm[a] (plane k's
+  // bits of those rows, masked out of the plane word in one step each).
+  constexpr uint64_t kRowBits = 0x0001000100010001ull;
+  uint64_t cm[4];
 #pragma unroll
-  for (int it = 0; it < 64 / RSTEP; ++it) cw[it] = 0;
-#pragma unroll 1
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a) {
+    uint64_t m = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint64_t q = pl[k][c4 + a] >> row0;
-#pragma unroll
-      for (int it = 0; it < 64 / RSTEP; ++it) cw[it] |= (uint32_t)((q >> (RSTEP * it)) & 1ull) << (8 * a + k);
-    }
+    for (int k = 0; k < K; ++k) m |= ((pl[k][c4 + a] >> row0) & kRowBits) << k;
+    cm[a] = m;
+  }
 #pragma unroll 1
   for (int it = 0; it < 64 / RSTEP; ++it) {
     const int rr = row0 + RSTEP * it;
@@ -1094,10 +1096,12 @@ __device__ __forceinline__ void emit_write_chunk(int tid, int64_t vb, int32_t gr
     int32_t lv[4], pv[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const uint32_t c = (cw[it] >> (8 * a)) & 0xFFu;
+      const uint32_t c = (uint32_t)cm[a] & 0xFu;
+      cm[a] >>= RSTEP;
       const int32_t va = (int32_t)(vb + c4 + a);
+      const int32_t tv = tile[rr * LD + c4 + a];  // read for every cell: selects, no branches
       lv[a] = (int32_t)c - 1;
-      pv[a] = (c == 0) ? -1 : (c == 1 ? va : (c == 2 ? sr : tile[rr * LD + c4 + a]));
+      pv[a] = c >= 3 ? tv : (c == 2 ? sr : (c == 1 ? va : -1));
     }
     const int64_t v0 = vb + c4;
     const int64_t lo = (map.lvl_row ? map.lvl_row[r] : (int64_t)r * n) + v0;
