@@ -883,11 +883,12 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
 
 // Light-vertex emit, compute phase of one chunk of 64 consecutive vertices
 // (output order) and one word group (W words, 64*W rows).  Lane (vertex vi,
-// partition p, word w) takes every 4th in-neighbour of the vertex (kEmitU per
-// step) and tests the 64 sources of its word at once.  The W word-lanes of a
-// (vertex, partition) read the same neighbour's plane line.  A segmented
-// prefix-OR over the 4 partitions, in ascending neighbour position, gives the
-// first (lowest-id) provider of every bit, staged in the LDS tile [row][vertex]
+// partition p, word w) takes kEmitU consecutive in-neighbours of the vertex
+// per step (partition p the p-th run of the step's 4 * kEmitU) and tests the
+// 64 sources of its word at once.  The W word-lanes of a (vertex, partition)
+// read the same neighbour's plane line.  Inside a lane the positions are
+// taken in order; a prefix-OR over the 4 partitions gives the first
+// (lowest-id) provider of every bit, staged in the LDS tile [row][vertex]
 // (pitch 65); heavy vertices copy their recorded parents into it.  Planes 0-3
 // of each (word, vertex) go to pl[w][k][vi] for the write phase.  No block
 // barrier inside: lanes write disjoint tile entries.
@@ -931,40 +932,39 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
 #pragma unroll
     for (int i = 0; i < 16; ++i) tile[(rb + 16 * p + i) * LD + vi] = (int32_t)hp[16 * p + i];
   }
+  // Step k0 takes list positions b + k0 .. b + k0 + 4 * kEmitU - 1; lane p
+  // the kEmitU consecutive ones from b + k0 + p * kEmitU.
   for (int32_t k0 = 0;; k0 += PART * kEmitU) {
-    const bool more = need != 0 && (b + k0 + p < e);
+    const int32_t j0 = b + k0 + p * kEmitU;  // this lane's first position of the step
+    const bool more = need != 0 && j0 < e;
     if (!__any(more)) break;
     int32_t us[kEmitU], uo[kEmitU];
     uint64_t eqs[kEmitU];
     if constexpr (W == 2) {
       // The two word lanes of a (vertex, partition) are the lane pair 2q,
-      // 2q+1 and read the same in-list position: one loads its engine id,
-      // the other its original id, and a DPP swap hands each lane the other
-      // half (one index load per position instead of two per lane).
-      // Both DPP moves run with every lane of the wave active (the loop is
-      // wave-uniform), so each lane reads a live partner.
+      // 2q+1 and read the same in-list positions: one loads their engine
+      // ids, the other their original ids, and a DPP swap hands each lane
+      // the other half (one index load per position instead of two per
+      // lane).  Both DPP moves run with every lane of the wave active (the
+      // loop is wave-uniform), so each lane reads a live partner.
       const int mo = __builtin_amdgcn_update_dpp(0, (int)more, quad_sel(1, 0, 3, 2), 0xF, 0xF, true);
       const bool pm = more || mo != 0;
       int32_t x[kEmitU];
 #pragma unroll
-      for (int i = 0; i < kEmitU; ++i) {
-        const int32_t j = b + k0 + i * PART + p;
-        x[i] = (pm && j < e) ? (w ? in_col_orig : in_col)[j] : -1;
-      }
+      for (int i = 0; i < kEmitU; ++i) x[i] = (pm && j0 + i < e) ? (w ? in_col_orig : in_col)[j0 + i] : -1;
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
         const int32_t y = __builtin_amdgcn_update_dpp(0, x[i], quad_sel(1, 0, 3, 2), 0xF, 0xF, true);
-        const bool in = more && b + k0 + i * PART + p < e;
+        const bool in = more && j0 + i < e;
         us[i] = in ? (w ? y : x[i]) : -1;
         uo[i] = in ? (w ? x[i] : y) : -1;
       }
     } else {
 #pragma unroll
       for (int i = 0; i < kEmitU; ++i) {
-        const int32_t j = b + k0 + i * PART + p;
-        const bool in = more && j < e;
-        us[i] = in ? in_col[j] : -1;
-        uo[i] = in ? in_col_orig[j] : -1;
+        const bool in = more && j0 + i < e;
+        us[i] = in ? in_col[j0 + i] : -1;
+        uo[i] = in ? in_col_orig[j0 + i] : -1;
       }
     }
     constexpr int KC = (SYM && K > 2) ? 2 : K;  // planes compared
@@ -1000,53 +1000,63 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
         eqs[i] = eq;
       }
     }
+    // Rows each of the lane's positions provides before its earlier ones,
+    // in list order inside the lane; then one prefix-OR over the partitions
+    // (lower partitions hold lower positions) removes the rows an earlier
+    // partition provides.
+    uint64_t fl[kEmitU], run = 0;
 #pragma unroll
     for (int i = 0; i < kEmitU; ++i) {
       const uint64_t c = eqs[i] & need;
-      uint64_t incl = c, excl, all;
-      if constexpr (W == 1) {
-        // The 4 partitions of a vertex are one lane quad.
-        incl |= quad_perm64<quad_sel(0, 0, 1, 2)>(incl);  // lane p <- p-1 (lane 0 <- itself)
-        incl |= quad_perm64<quad_sel(0, 1, 0, 1)>(incl);  // lane p <- p-2 (lanes 0, 1 <- themselves)
-        excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
-        if (p == 0) excl = 0;
-        all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
-      } else if constexpr (W == 2) {
-        // The 8 lanes (p, w) of a vertex are an aligned 8-lane group, w the
-        // lane parity: DPP row shifts by 2 and 4 lanes move between
-        // partitions of the same word, with no LDS crossbar round trip.
-        // Every DPP move runs on all lanes; the selects apply the partition
-        // bounds after.
-        const uint64_t y1 = dpp64<dpp_row_shr(2)>(incl);  // partition p-1
-        incl |= (p >= 1) ? y1 : 0ull;
-        const uint64_t y2 = dpp64<dpp_row_shr(4)>(incl);  // partition p-2
-        incl |= (p >= 2) ? y2 : 0ull;
-        excl = dpp64<dpp_row_shr(2)>(incl);
-        if (p == 0) excl = 0;
-        // Partition 3's total: within each quad take quad lanes 2-3 (the
-        // upper quad of the group holds partitions 2-3), then the lower quad
-        // reads the upper one 4 lanes on.
-        const uint64_t q3 = quad_perm64<quad_sel(2, 3, 2, 3)>(incl);
-        const uint64_t q3n = dpp64<dpp_row_shl(4)>(q3);
-        all = (p >= 2) ? q3 : q3n;
-      } else {
-        uint64_t y = __shfl_up(incl, W, kWave);
-        if (p >= 1) incl |= y;
-        y = __shfl_up(incl, 2 * W, kWave);
-        if (p >= 2) incl |= y;
-        excl = __shfl_up(incl, W, kWave);
-        if (p == 0) excl = 0;
-        all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
-      }
-      // Rows this position resolves first: one LDS write each, the two
-      // 32-bit halves in loops of their own (a 32-bit find-first and clear
-      // per row instead of their 64-bit forms).
-      const uint64_t first = c & ~excl;
-      int32_t* col = tile + rb * LD + vi;
+      fl[i] = c & ~run;
+      run |= c;
+    }
+    uint64_t incl = run, excl, all;
+    if constexpr (W == 1) {
+      // The 4 partitions of a vertex are one lane quad.
+      incl |= quad_perm64<quad_sel(0, 0, 1, 2)>(incl);  // lane p <- p-1 (lane 0 <- itself)
+      incl |= quad_perm64<quad_sel(0, 1, 0, 1)>(incl);  // lane p <- p-2 (lanes 0, 1 <- themselves)
+      excl = quad_perm64<quad_sel(0, 0, 1, 2)>(incl);
+      if (p == 0) excl = 0;
+      all = quad_perm64<quad_sel(3, 3, 3, 3)>(incl);
+    } else if constexpr (W == 2) {
+      // The 8 lanes (p, w) of a vertex are an aligned 8-lane group, w the
+      // lane parity: DPP row shifts by 2 and 4 lanes move between
+      // partitions of the same word, with no LDS crossbar round trip.
+      // Every DPP move runs on all lanes; the selects apply the partition
+      // bounds after.
+      const uint64_t y1 = dpp64<dpp_row_shr(2)>(incl);  // partition p-1
+      incl |= (p >= 1) ? y1 : 0ull;
+      const uint64_t y2 = dpp64<dpp_row_shr(4)>(incl);  // partition p-2
+      incl |= (p >= 2) ? y2 : 0ull;
+      excl = dpp64<dpp_row_shr(2)>(incl);
+      if (p == 0) excl = 0;
+      // Partition 3's total: within each quad take quad lanes 2-3 (the
+      // upper quad of the group holds partitions 2-3), then the lower quad
+      // reads the upper one 4 lanes on.
+      const uint64_t q3 = quad_perm64<quad_sel(2, 3, 2, 3)>(incl);
+      const uint64_t q3n = dpp64<dpp_row_shl(4)>(q3);
+      all = (p >= 2) ? q3 : q3n;
+    } else {
+      uint64_t y = __shfl_up(incl, W, kWave);
+      if (p >= 1) incl |= y;
+      y = __shfl_up(incl, 2 * W, kWave);
+      if (p >= 2) incl |= y;
+      excl = __shfl_up(incl, W, kWave);
+      if (p == 0) excl = 0;
+      all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
+    }
+    // Rows a position resolves first: one LDS write each, the two 32-bit
+    // halves in loops of their own (a 32-bit find-first and clear per row
+    // instead of their 64-bit forms).
+    int32_t* col = tile + rb * LD + vi;
+#pragma unroll
+    for (int i = 0; i < kEmitU; ++i) {
+      const uint64_t first = fl[i] & ~excl;
       for (uint32_t f = (uint32_t)first; f; f &= f - 1) col[__builtin_ctz(f) * LD] = uo[i];
       for (uint32_t f = (uint32_t)(first >> 32); f; f &= f - 1) col[(32 + __builtin_ctz(f)) * LD] = uo[i];
-      need &= ~all;
     }
+    need &= ~all;
   }
 }
 
