@@ -881,6 +881,22 @@ __device__ __forceinline__ CodeMasks<K> code_masks(const uint64_t (&pk)[K]) {
   return m;
 }
 
+// The value of partition 0's lane of the same (vertex, word) in the emit's
+// lane layout: lanes 4q + p for one word per pass, 8q + 2p + w for two.  DPP
+// moves on every lane of the wave (the caller's loop is wave-uniform).
+template <int W>
+__device__ __forceinline__ uint64_t from_partition0(uint64_t x, int p) {
+  if constexpr (W == 1) {
+    return quad_perm64<quad_sel(0, 0, 0, 0)>(x);
+  } else if constexpr (W == 2) {
+    const uint64_t lo = quad_perm64<quad_sel(0, 1, 0, 1)>(x);  // partitions 0-1: lanes 8q + w
+    const uint64_t hi = dpp64<dpp_row_shr(4)>(lo);                // partitions 2-3: the same, 4 lanes back
+    return p < 2 ? lo : hi;
+  } else {
+    return __shfl(x, (lane_id() & ~(4 * W - 1)) + lane_id() % W, kWave);
+  }
+}
+
 // Light-vertex emit, compute phase of one chunk of 64 consecutive vertices
 // (output order) and one word group (W words, 64*W rows).  Lane (vertex vi,
 // partition p, word w) takes kEmitU consecutive in-neighbours of the vertex
@@ -1046,15 +1062,26 @@ __device__ __forceinline__ void emit_scan_chunk(const BPState& s, int t, int64_t
       if (p == 0) excl = 0;
       all = __shfl(incl, (l & ~(PART * W - 1)) | ((PART - 1) * W + w), kWave);
     }
-    // Rows a position resolves first: one LDS write each, the two 32-bit
-    // halves in loops of their own (a 32-bit find-first and clear per row
-    // instead of their 64-bit forms).
+    // Rows a position resolves first: one LDS write each.  Partition 0's
+    // positions (the step's lowest, which resolve most rows: its first one
+    // is the vertex's lowest-id in-neighbour at step 0) are shared out by
+    // row range: their masks and ids go to all 4 partition lanes of the
+    // vertex and word, and lane p writes their rows 16p .. 16p + 15.  The
+    // other partitions write their own rows, the two 32-bit halves in loops
+    // of their own.  A wave's loops run as long as its busiest lane, so this
+    // halves the iterations on the LiveJournal-shaped graph.
     int32_t* col = tile + rb * LD + vi;
 #pragma unroll
     for (int i = 0; i < kEmitU; ++i) {
       const uint64_t first = fl[i] & ~excl;
-      for (uint32_t f = (uint32_t)first; f; f &= f - 1) col[__builtin_ctz(f) * LD] = uo[i];
-      for (uint32_t f = (uint32_t)(first >> 32); f; f &= f - 1) col[(32 + __builtin_ctz(f)) * LD] = uo[i];
+      const uint64_t f0 = from_partition0<W>(first, p);
+      const int32_t id0 = (int32_t)from_partition0<W>((uint64_t)(uint32_t)uo[i], p);
+      for (uint32_t f = (uint32_t)(f0 >> (16 * p)) & 0xFFFFu; f; f &= f - 1)
+        col[(16 * p + __builtin_ctz(f)) * LD] = id0;
+      if (p != 0) {
+        for (uint32_t f = (uint32_t)first; f; f &= f - 1) col[__builtin_ctz(f) * LD] = uo[i];
+        for (uint32_t f = (uint32_t)(first >> 32); f; f &= f - 1) col[(32 + __builtin_ctz(f)) * LD] = uo[i];
+      }
     }
     need &= ~all;
   }
