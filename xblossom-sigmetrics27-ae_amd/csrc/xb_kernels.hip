@@ -442,10 +442,10 @@ template <int T>
 constexpr int kEmitWords = T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS;
 
 // Level-code planes of (v, word t): groups of W = min(T, XB_PLANE_GROUP_MAX)
-// words, vertex-major within a group.  With XB_PLANE_SPLIT (the default) the
-// array holds two halves: planes 0-1 of every (group, vertex) first, then
-// planes 2-3, so planes 0-1 of a vertex's W words of one group are 16W
-// contiguous bytes.  The light emit on a symmetric graph compares planes 0-1
+// words, vertex-major within a group.  With XB_PLANE_SPLIT (the default, for
+// tiles of XB_PLANE_SPLIT_MIN_T words and more) the array holds two halves:
+// planes 0-1 of every (group, vertex) first, then planes 2-3, so planes 0-1
+// of a vertex's W words of one group are 16W contiguous bytes.  The light emit on a symmetric graph compares planes 0-1
 // of every in-neighbour only: its gathers then move 16 bytes per word, not
 // the 32 of planes 0-3.  Without the split, planes 0-3 of (v, t) are 32
 // contiguous bytes.  Planes 4-7 (planes_hi) use the same layout.  The emit's
@@ -456,8 +456,19 @@ constexpr int kEmitWords = T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS;
 #ifndef XB_PLANE_SPLIT
 #define XB_PLANE_SPLIT 1
 #endif
+// Narrowest tile (words) that takes the split layout.  The split puts 4
+// vertices' plane pairs of a group in one line where the pull writes
+// consecutive vertices, but an emit reading a vertex's own planes 0-3 then
+// touches two lines: on the LiveJournal-shaped graph's 16-word tile the pull
+// gains more (9.9 against 10.8 ms per step), on the uneven batch's 1-8-word
+// tiles the emit loses more (34.5 against 33.6 ms per replay).
+#ifndef XB_PLANE_SPLIT_MIN_T
+#define XB_PLANE_SPLIT_MIN_T 16
+#endif
 template <int T>
 constexpr int kPlaneGroup = T < XB_PLANE_GROUP_MAX ? T : XB_PLANE_GROUP_MAX;
+template <int T>
+constexpr bool kPlaneSplit = XB_PLANE_SPLIT && T >= XB_PLANE_SPLIT_MIN_T;
 
 // Index of plane pair `half` (planes 2*half and 2*half + 1; without the split,
 // of plane 2*half) of (v, t) in a plane array.
@@ -465,7 +476,7 @@ template <int T>
 __device__ __forceinline__ int64_t plane_pair_index(const BPState& s, int64_t v, int t, int half) {
   constexpr int W = kPlaneGroup<T>;
   static_assert(kPlaneGroup<T> % kEmitWords<T> == 0, "emit words per pass must divide the plane group");
-  if (XB_PLANE_SPLIT) {
+  if constexpr (kPlaneSplit<T>) {
     constexpr int G = T / W;
     return ((((int64_t)half * G + t / W) * s.n + v) * W + (t % W)) * 2;
   }
@@ -504,7 +515,7 @@ __device__ __forceinline__ void planes_or(const BPState& s, int64_t v, int t, ui
   typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
   typedef uint64_t u64x4 __attribute__((ext_vector_type(4)));
   auto pair_or = [&](uint64_t* a, uint32_t c) {
-    if (XB_PLANE_SPLIT) {
+    if constexpr (kPlaneSplit<T>) {
       if (c & 3u) {
         u64x2* p0 = reinterpret_cast<u64x2*>(a + plane_pair_index<T>(s, v, t, 0));
         const u64x2 m = {(c & 1u) ? bits : 0ull, (c & 2u) ? bits : 0ull};
