@@ -184,7 +184,8 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 // emit_pass_words(T)); symmetric: the graph's in- and out-lists are the same
 // (a two-plane level test per in-neighbour).
 int emit_pass_words(int T);
-void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t K, int32_t rows, const int32_t* src,
+void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t ngrp, int32_t K, int32_t rows,
+                    const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                     int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map,
                     bool symmetric);

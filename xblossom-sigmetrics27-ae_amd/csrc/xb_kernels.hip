@@ -1198,6 +1198,9 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
   // passes its SUB word groups back to back, so the later passes find the
   // in-neighbours' plane lines (shared by the group's words) in the L2.
   constexpr int SUB = DYN ? kPlaneGroup<T> / W : 1;
+  // Static schedule: ngrp word groups from grp, back to back per chunk (the
+  // chunk's in-lists and offsets stay in the caches for the later groups).
+  const int32_t nsub = DYN ? SUB : (ngrp > 1 ? ngrp : 1);
   const int tid = threadIdx.x;
   const int w = tid % W;
   const int p = (tid / W) % PART;
@@ -1221,7 +1224,7 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
       vb = it * 64;
       if (vb >= n) break;
     }
-    for (int sub = 0; sub < SUB; ++sub) {
+    for (int sub = 0; sub < nsub; ++sub) {
       const int32_t gq = grp0 + sub;  // this pass's word group
       if (DYN && gq >= ngrp) break;
       const int t = gq * W + w;       // this lane's word of the tile
@@ -1342,12 +1345,12 @@ __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const i
 // spilling.
 template <int T, int K, bool SYM>
 __global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / (K <= 4 ? 1 : kEmitWords<T>)) void k_bp_emit_light(
-    BPState s, int32_t grp, int32_t rows, const int32_t* __restrict__ src, const int32_t* __restrict__ in_off,
-    const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig, int32_t n, int32_t heavy_cut,
-    int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
+    BPState s, int32_t grp, int32_t ngrp, int32_t rows, const int32_t* __restrict__ src,
+    const int32_t* __restrict__ in_off, const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
+    int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
   __shared__ EmitLdsT<kEmitWords<T>> lds;
   emit_light_phase<T, K, kEmitWords<T>, false, SYM>(self_grid(), lds, s, grp, rows, src, in_off, in_col,
-                                                    in_col_orig, n, heavy_cut, lvl_out, par_out, map, nullptr, 0);
+                                                    in_col_orig, n, heavy_cut, lvl_out, par_out, map, nullptr, ngrp);
 }
 
 // ============================================================================
@@ -2655,24 +2658,25 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 }
 
 template <int T, int K>
-static void emit_light_launch(hipStream_t st, BPState s, int32_t grp, int32_t rows, const int32_t* src,
+static void emit_light_launch(hipStream_t st, BPState s, int32_t grp, int32_t ngrp, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                               int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
   if (symmetric)
     hipLaunchKernelGGL((k_bp_emit_light<T, K, true>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
-                       grp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
   else
     hipLaunchKernelGGL((k_bp_emit_light<T, K, false>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
-                       grp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
 }
 
 int emit_pass_words(int T) { return T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS; }
 
-void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t K, int32_t rows, const int32_t* src,
-                    const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
-                    int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
-  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, rows, src, in_off, in_col, in_col_orig, n,
-                                                               heavy_cut, lvl_out, par_out, map, symmetric))));
+void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t ngrp, int32_t K, int32_t rows,
+                    const int32_t* src, const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
+                    int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
+  XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, ngrp, rows, src, in_off, in_col,
+                                                               in_col_orig, n, heavy_cut, lvl_out, par_out, map,
+                                                               symmetric))));
 }
 
 int persistent_blocks_per_cu() {
