@@ -80,6 +80,12 @@ constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 #ifndef XB_EMIT_NT
 #define XB_EMIT_NT 1
 #endif
+// A one-launch emit of several word groups takes them in turn across the
+// whole grid (all workgroups on group g, then g + 1), not chunk by chunk, so
+// the neighbours' plane lines in the caches are those of one group at a time.
+#ifndef XB_EMIT_GROUP_MAJOR
+#define XB_EMIT_GROUP_MAJOR 1
+#endif
 constexpr int kPullU = XB_PULL_U;
 constexpr int kEmitU = XB_EMIT_U;
 constexpr int kStageCap = 1536;
@@ -1198,8 +1204,8 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
   // passes its SUB word groups back to back, so the later passes find the
   // in-neighbours' plane lines (shared by the group's words) in the L2.
   constexpr int SUB = DYN ? kPlaneGroup<T> / W : 1;
-  // Static schedule: ngrp word groups from grp, back to back per chunk (the
-  // chunk's in-lists and offsets stay in the caches for the later groups).
+  // Static schedule: ngrp word groups from grp in one launch, group by group
+  // across the grid (XB_EMIT_GROUP_MAJOR) or back to back per chunk.
   const int32_t nsub = DYN ? SUB : (ngrp > 1 ? ngrp : 1);
   const int tid = threadIdx.x;
   const int w = tid % W;
@@ -1220,11 +1226,18 @@ __device__ void emit_light_phase(Grid g, EmitLdsT<W>& L, BPState s, int32_t grp,
       if (c >= (int64_t)nsup * nchunks) break;
       grp0 = (int32_t)(c / nchunks) * SUB;
       vb = (c % nchunks) * 64;
+    } else if (XB_EMIT_GROUP_MAJOR && nsub > 1) {
+      // Word groups in turn across the whole grid: item it is chunk
+      // it % nchunks of group grp + it / nchunks.
+      if (it >= (int64_t)nsub * nchunks) break;
+      grp0 = grp + (int32_t)(it / nchunks);
+      vb = (it % nchunks) * 64;
     } else {
       vb = it * 64;
       if (vb >= n) break;
     }
-    for (int sub = 0; sub < nsub; ++sub) {
+    const int32_t npass = (!DYN && XB_EMIT_GROUP_MAJOR) ? 1 : nsub;
+    for (int sub = 0; sub < npass; ++sub) {
       const int32_t gq = grp0 + sub;  // this pass's word group
       if (DYN && gq >= ngrp) break;
       const int t = gq * W + w;       // this lane's word of the tile
