@@ -36,7 +36,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 # rocprofv3 kernel names of the engine's kernel classes (for the PMC traffic lookup).
 KERNEL_SYMBOL = {
     "emit": "k_bp_emit_light",
-    "pull": "k_bp_pull_bucket",
+    "pull": "k_bp_pull_fused",
     "pull_hub": "k_bp_pull_hub",
     "push": "k_bp_push",
     "plain_push": "k_plain_push",

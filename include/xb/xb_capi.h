@@ -98,8 +98,10 @@ typedef struct xb_msbfs_opts {
 #define XB_K_SEED 1       /* source seeding                                     */
 #define XB_K_PUSH 2       /* bit-parallel top-down push                         */
 #define XB_K_COMMIT 3     /* commit of pushed / hub-pulled next words            */
-#define XB_K_PULL 4       /* bit-parallel bottom-up pull, degree buckets          */
-#define XB_K_PULL_HUB 5   /* bottom-up pull, hub adjacency chunks               */
+#define XB_K_PULL 4       /* bit-parallel bottom-up pull: degree buckets and, in  */
+                          /* the one-launch level (default), hub chunks too       */
+#define XB_K_PULL_HUB 5   /* bottom-up pull, hub adjacency chunks (when a level  */
+                          /* launches buckets and hub chunks separately)         */
 #define XB_K_CLEAR 6      /* frontier clear                                     */
 #define XB_K_EMIT_HEAVY 7 /* reserved: heavy parents are recorded by pull/push */
 #define XB_K_EMIT 8       /* level/parent table write-out                       */

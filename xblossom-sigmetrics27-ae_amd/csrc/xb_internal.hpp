@@ -176,6 +176,16 @@ void launch_bp_commit(hipStream_t st, int T, BPState s, int cur, int32_t code, c
 void launch_bp_pull_bucket(hipStream_t st, int T, int cls, BPState s, int cur, int32_t code, int32_t cnt,
                            const int32_t* in_off, const int32_t* in_col, const int32_t* out_off,
                            const int32_t* perm, int32_t shift, int32_t begin, int32_t end, Queue nxt);
+// One pull level in one launch: the three in-degree buckets (workgroup ranges
+// of their own) and the hub pieces.  Bucket c covers [begin[c], end[c]) of
+// perm[c] (null: the id range itself, shifted by shift[c]).
+struct PullPlan {
+  const int32_t* perm[3];
+  int32_t shift[3], begin[3], end[3];
+};
+void launch_bp_pull_fused(hipStream_t st, int T, BPState s, int cur, int32_t code, int32_t cnt,
+                          const int32_t* in_off, const int32_t* in_col, const int32_t* out_off, const PullPlan& plan,
+                          Queue nxt, const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, const int32_t* in_col,
                         const HubChunk* chunks, int32_t n_chunks);
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn);

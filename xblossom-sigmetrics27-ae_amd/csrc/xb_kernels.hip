@@ -1334,6 +1334,43 @@ __global__ __launch_bounds__(kBlock) void k_bp_pull_hub(BPState s, int cur, int3
   pull_hub_phase<T>(self_grid(), s, cur, cnt, in_col, chunks, n_chunks);
 }
 
+// One pull level in one launch, heaviest work first: workgroups [0, nb_hub)
+// take the hub pieces, the next nb[2] bucket 2 (one wave per vertex), then
+// nb[1] bucket 1 and the rest bucket 0.  The four touch disjoint vertices, so
+// they run side by side, and the level ends in one grid tail instead of four.
+struct PullSplit {
+  int32_t nb[3];
+  int32_t nb_hub;
+};
+template <int T>
+__global__ __launch_bounds__(kBlock, XB_PULL_MIN_BLOCKS) void k_bp_pull_fused(
+    BPState s, int cur, uint32_t code, int32_t cnt, const int32_t* __restrict__ in_off,
+    const int32_t* __restrict__ in_col, const int32_t* __restrict__ out_off, PullPlan plan, PullSplit split,
+    Queue nxt, const HubChunk* __restrict__ chunks, int32_t n_chunks) {
+  constexpr int P1 = T * 4 <= kWave ? 4 : kWave / T;
+  constexpr int P2 = kWave / T;
+  int bid = (int)blockIdx.x;
+  if (bid < split.nb_hub) {
+    pull_hub_phase<T>(Grid{bid, split.nb_hub}, s, cur, cnt, in_col, chunks, n_chunks);
+    return;
+  }
+  bid -= split.nb_hub;
+  if (bid < split.nb[2]) {
+    pull_bucket_phase<T, P2, true>(Grid{bid, split.nb[2]}, s, cur, code, cnt, in_off, in_col, out_off, plan.perm[2],
+                                   plan.shift[2], plan.begin[2], plan.end[2], nxt);
+    return;
+  }
+  bid -= split.nb[2];
+  if (bid < split.nb[1]) {
+    pull_bucket_phase<T, P1, false>(Grid{bid, split.nb[1]}, s, cur, code, cnt, in_off, in_col, out_off,
+                                    plan.perm[1], plan.shift[1], plan.begin[1], plan.end[1], nxt);
+    return;
+  }
+  bid -= split.nb[1];
+  pull_bucket_phase<T, 1, false>(Grid{bid, split.nb[0]}, s, cur, code, cnt, in_off, in_col, out_off, plan.perm[0],
+                                 plan.shift[0], plan.begin[0], plan.end[0], nxt);
+}
+
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_bp_clear(BPState s, int cur, const int32_t* __restrict__ qv,
                                                      int32_t qn) {
@@ -2662,6 +2699,34 @@ void launch_bp_pull_hub(hipStream_t st, int T, BPState s, int cur, int32_t cnt, 
   if (n_chunks <= 0) return;
   XB_DISPATCH_T(T, hipLaunchKernelGGL(k_bp_pull_hub<TT>, dim3(grid_for((int64_t)n_chunks * kWave, kBlock)),
                                       dim3(kBlock), 0, st, s, cur, cnt, in_col, chunks, n_chunks));
+}
+
+template <int T>
+static void pull_fused_launch(hipStream_t st, BPState s, int cur, int32_t code, int32_t cnt, const int32_t* in_off,
+                              const int32_t* in_col, const int32_t* out_off, const PullPlan& plan, Queue nxt,
+                              const HubChunk* chunks, int32_t n_chunks) {
+  constexpr int P1 = T * 4 <= kWave ? 4 : kWave / T;
+  constexpr int P2 = kWave / T;
+  const int vpb[3] = {(kBlock / kWave) * (kWave / T), (kBlock / kWave) * (kWave / (T * P1)),
+                      (kBlock / kWave) * (kWave / (T * P2))};
+  PullSplit split;
+  int64_t total = 0;
+  for (int c = 0; c < 3; ++c) {
+    split.nb[c] = plan.end[c] > plan.begin[c] ? grid_for((int64_t)(plan.end[c] - plan.begin[c]), vpb[c]) : 0;
+    total += split.nb[c];
+  }
+  split.nb_hub = n_chunks > 0 ? grid_for((int64_t)n_chunks * kWave, kBlock) : 0;
+  total += split.nb_hub;
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_bp_pull_fused<T>, dim3((unsigned)total), dim3(kBlock), 0, st, s, cur, (uint32_t)code, cnt,
+                     in_off, in_col, out_off, plan, split, nxt, chunks, n_chunks);
+}
+
+void launch_bp_pull_fused(hipStream_t st, int T, BPState s, int cur, int32_t code, int32_t cnt,
+                          const int32_t* in_off, const int32_t* in_col, const int32_t* out_off, const PullPlan& plan,
+                          Queue nxt, const HubChunk* chunks, int32_t n_chunks) {
+  XB_DISPATCH_T(T, (pull_fused_launch<TT>(st, s, cur, code, cnt, in_off, in_col, out_off, plan, nxt, chunks,
+                                          n_chunks)));
 }
 
 void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t qn) {
