@@ -71,6 +71,11 @@ extern "C" {
                               between two workspaces and streams, so a
                               tile's traversal overlaps the previous tile's
                               emit; costs a second tile workspace)          */
+#define XB_BATCH_EMIT_INSIDE 64u /* xb_batch_*: every team emits its tiles
+                                    inside the persistent kernel (default: a
+                                    team of one tile leaves its emit to
+                                    standalone launches captured after the
+                                    kernel)                                   */
 
 typedef struct xb_graph xb_graph;
 

@@ -166,3 +166,38 @@ def test_batch_refuses_run_after_layout_rebuild(mixed):
     b2.run()
     assert np.array_equal(lv.numpy(), refs[i][0]) and np.array_equal(pa.numpy(), refs[i][1])
     b2.close()
+
+
+@pytest.mark.parametrize("emit_inside", [False, True])
+def test_batch_emit_after_kernel_and_inside(emit_inside):
+    """A graph whose sources fit one tile is emitted by standalone launches
+    captured after the persistent kernel, gated on the depth the kernel
+    leaves in the job table (4 planes up to level 14, 8 beyond); with
+    XB_BATCH_EMIT_INSIDE every team emits inside the kernel.  Both give the
+    oracle's tables on every replay: a 99-level path (the 8-plane launch), a
+    symmetric and a directed R-MAT, and a graph of two tiles (emitted inside
+    the kernel either way)."""
+    ro, ci = tg.path(100)
+    gs = [(xb.Csr(ro, ci), False),
+          (xb.gen_rmat(11, 16, 0.57, 0.19, 0.19, seed=12), False),
+          (xb.gen_rmat(12, 8, 0.6, 0.15, 0.15, seed=13, symmetric=False), True),
+          (xb.gen_chung_lu(20_000, 2.1, 12.0, 2_000, seed=14), False)]
+    counts = [3, 100, 40, 300]  # 300 sources at 4 words per tile: two tiles
+    hgs = [xb.HGraph(g.row_offsets, g.col_indices, directed=d) for g, d in gs]
+    srcs = [np.array([0, 50, 99], dtype=np.int32)]
+    srcs += [xb.sample_sources(g, c, seed=20 + i) for i, ((g, _), c) in enumerate(zip(gs[1:], counts[1:]))]
+    refs = [oracle.msbfs(g.row_offsets, g.col_indices, s, threads=8)[:2] for (g, _), s in zip(gs, srcs)]
+    lv = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    pa = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa], words_per_tile=4, emit_inside=emit_inside)
+    for rep in range(3):
+        for t in lv + pa:
+            t.upload(np.full(t.count, 7, dtype=np.int32))
+        st = b.run()
+        assert st.kernel_launches[11] == 1 and st.max_level == 99
+        for i, (lr, pr) in enumerate(refs):
+            assert np.array_equal(lv[i].numpy(), lr), f"replay {rep} graph {i}: levels"
+            assert np.array_equal(pa[i].numpy(), pr), f"replay {rep} graph {i}: parents"
+    b.close()
+    for t in lv + pa:
+        t.free()

@@ -1675,6 +1675,12 @@ extern "C" int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int
 #define XB_REBALANCE_FLOOR 4
 #endif
 
+// A team of one tile leaves its emit to standalone launches captured after
+// the persistent kernel (two words per pass in 512-thread workgroups).
+#ifndef XB_BATCH_EMIT_AFTER
+#define XB_BATCH_EMIT_AFTER 1
+#endif
+
 // The persistent kernel instantiates its tile loop for 1, 2, 4, 8 and 16
 // words, as the host-driven engine does.
 constexpr int kBatchMaxWords = 16;
@@ -1932,6 +1938,9 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
       b->h_jobs.push_back(j);
     }
     tm.job_end = (int32_t)b->h_jobs.size();
+    tm.emit_after =
+        (XB_BATCH_EMIT_AFTER && tm.job_end - tm.job_begin == 1 && (o.flags & XB_BATCH_EMIT_INSIDE) == 0) ? 1 : 0;
+    tm.pad_ = 0;
     const size_t nw = (size_t)g->n * tw + 1;
     tm.s.seen = b->alloc<uint64_t>(nw);
     tm.s.front[0] = b->alloc<uint64_t>(nw);
@@ -1969,10 +1978,34 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   h2d(b->d_jobs, b->h_jobs.data(), sizeof(xb::TileJob) * b->h_jobs.size(), b->stream);
   batch_clear_state(b.get());
 
-  // Capture the launch (control reset + persistent kernel) once.
+  // Capture the launch (control reset + persistent kernel) once, followed by
+  // the emits of the one-tile teams: per team one launch for codes below 16
+  // and one for deeper tiles, each gated on the tile's status and depth as
+  // the kernel leaves them in the job table.
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   xb::launch_bp_persistent(b->stream, b->grid, b->d_teams, b->n_teams, b->d_views, b->d_jobs,
                            (int32_t)b->h_jobs.size(), b->d_ctl, b->alpha, b->d_err);
+  for (int t = 0; t < nt; ++t) {
+    const xb::Team& tm = b->h_teams[t];
+    if (!tm.emit_after) continue;
+    const xb::TileJob& j = b->h_jobs[tm.job_begin];
+    const xb_graph* g = graphs[j.graph];
+    xb::BPState s = tm.s;
+    s.hidx = g->d_hidx;
+    s.new_of = g->d_new_of;
+    s.orig_of = g->d_orig_of;
+    s.in_col_o = g->d_ein_col_o;
+    s.planes_hi = s.planes + (size_t)g->n * j.words * 4;  // as the kernel's tile set-up places them
+    const int W = xb::emit_pass_words(j.words);
+    const int ngrp = (j.words + W - 1) / W;
+    const int32_t* gate = &b->d_jobs[tm.job_begin].status;  // {status, levels}
+    const bool sym = (g->flags & XB_GRAPH_DIRECTED) == 0;
+    // Codes up to 15 (deepest level 14) in 4 planes, deeper tiles in 8.
+    xb::launch_bp_emit(b->stream, j.words, s, 0, ngrp, 4, j.cnt, j.src, g->d_oin_off, g->d_oin_col, g->d_oin_colo,
+                       g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 0, 14);
+    xb::launch_bp_emit(b->stream, j.words, s, 0, ngrp, 8, j.cnt, j.src, g->d_oin_off, g->d_oin_col, g->d_oin_colo,
+                       g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 15, 1 << 30);
+  }
   HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
   HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   return b.release();

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace xb {
@@ -115,6 +116,9 @@ struct TileJob {
   int32_t pad_;
 };
 
+static_assert(offsetof(TileJob, levels) == offsetof(TileJob, status) + sizeof(int32_t),
+              "a captured emit reads {status, levels} as one pair");
+
 // Team barrier counters live in 256-byte slots of their own: arrivals from
 // thousands of workgroups serialize per address, so every XCD gets its own
 // arrival counter and only the last arriver of each XCD touches the team's.
@@ -148,6 +152,11 @@ static_assert(sizeof(TeamCtl) % 8 == 0, "TeamCtl is cleared as 64-bit words");
 struct Team {
   int32_t first_block, nblocks;
   int32_t job_begin, job_end;
+  // 1: the team runs one tile and leaves its emit to the standalone emit
+  // launches that follow the persistent kernel in the captured graph (the
+  // tile's planes stay in the team's workspace until then).
+  int32_t emit_after;
+  int32_t pad_;
   BPState s;
   int32_t* qv[2];
   int32_t* qeo[2];
@@ -194,11 +203,14 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 // emit_pass_words(T)); symmetric: the graph's in- and out-lists are the same
 // (a two-plane level test per in-neighbour).
 int emit_pass_words(int T);
+// gate (optional): a tile's {status, levels} pair in device memory; the
+// launch then writes nothing unless status is 0 and gate_lo <= levels <=
+// gate_hi (a captured emit whose tile depth is known only on the device).
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t grp, int32_t ngrp, int32_t K, int32_t rows,
                     const int32_t* src,
                     const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
                     int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map,
-                    bool symmetric);
+                    bool symmetric, const int32_t* gate = nullptr, int32_t gate_lo = 0, int32_t gate_hi = 0);
 
 // ---- forest mode (alternating-forest search of maximum matching) ----
 // Pool of pending second calls of the path flip's recursion (x, y, next

@@ -1397,7 +1397,12 @@ template <int T, int K, bool SYM>
 __global__ __launch_bounds__(kBlock * kEmitWords<T>, XB_EMIT_WAVES_PER_SIMD / (K <= 4 ? 1 : kEmitWords<T>)) void k_bp_emit_light(
     BPState s, int32_t grp, int32_t ngrp, int32_t rows, const int32_t* __restrict__ src,
     const int32_t* __restrict__ in_off, const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_col_orig,
-    int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map) {
+    int32_t n, int32_t heavy_cut, int32_t* __restrict__ lvl_out, int32_t* __restrict__ par_out, RowMap map,
+    const int32_t* __restrict__ gate, int32_t gate_lo, int32_t gate_hi) {
+  if (gate != nullptr) {
+    const int32_t status = gate[0], levels = gate[1];
+    if (status != 0 || levels < gate_lo || levels > gate_hi) return;
+  }
   __shared__ EmitLdsT<kEmitWords<T>> lds;
   emit_light_phase<T, K, kEmitWords<T>, false, SYM>(self_grid(), lds, s, grp, rows, src, in_off, in_col,
                                                     in_col_orig, n, heavy_cut, lvl_out, par_out, map, nullptr, ngrp);
@@ -1716,7 +1721,10 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   int K = 0;
   while ((1 << K) <= d + 1) ++K;
   bool ok = true;
-  switch (K) {
+  // A team of one tile leaves the emit to the standalone launches after the
+  // kernel (two source words per pass in 512-thread workgroups, which this
+  // kernel's 256-thread workgroups and 20 KB LDS union cannot hold).
+  if (!tm.emit_after) switch (K) {
     case 1: ok = emit_all_words<T, 1>(g, s, gv, job, ts, gerr); break;
     case 2: ok = emit_all_words<T, 2>(g, s, gv, job, ts, gerr); break;
     case 3: ok = emit_all_words<T, 3>(g, s, gv, job, ts, gerr); break;
@@ -2738,23 +2746,27 @@ void launch_bp_clear(hipStream_t st, int T, BPState s, int cur, Queue q, int32_t
 template <int T, int K>
 static void emit_light_launch(hipStream_t st, BPState s, int32_t grp, int32_t ngrp, int32_t rows, const int32_t* src,
                               const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig, int32_t n,
-                              int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
+                              int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric,
+                              const int32_t* gate, int32_t gate_lo, int32_t gate_hi) {
   if (symmetric)
     hipLaunchKernelGGL((k_bp_emit_light<T, K, true>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
-                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map, gate,
+                       gate_lo, gate_hi);
   else
     hipLaunchKernelGGL((k_bp_emit_light<T, K, false>), dim3(grid_for(n, 64)), dim3(kBlock * kEmitWords<T>), 0, st, s,
-                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map);
+                       grp, ngrp, rows, src, in_off, in_col, in_col_orig, n, heavy_cut, lvl_out, par_out, map, gate,
+                       gate_lo, gate_hi);
 }
 
 int emit_pass_words(int T) { return T < XB_EMIT_WORDS ? T : XB_EMIT_WORDS; }
 
 void launch_bp_emit(hipStream_t st, int T, BPState s, int32_t word, int32_t ngrp, int32_t K, int32_t rows,
                     const int32_t* src, const int32_t* in_off, const int32_t* in_col, const int32_t* in_col_orig,
-                    int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric) {
+                    int32_t n, int32_t heavy_cut, int32_t* lvl_out, int32_t* par_out, RowMap map, bool symmetric,
+                    const int32_t* gate, int32_t gate_lo, int32_t gate_hi) {
   XB_DISPATCH_T(T, XB_DISPATCH_K(K, (emit_light_launch<TT, KK>(st, s, word, ngrp, rows, src, in_off, in_col,
                                                                in_col_orig, n, heavy_cut, lvl_out, par_out, map,
-                                                               symmetric))));
+                                                               symmetric, gate, gate_lo, gate_hi))));
 }
 
 int persistent_blocks_per_cu() {

@@ -52,6 +52,7 @@ XB_NO_LEVELS = 4
 XB_FIXED_TEAMS = 8
 XB_MATCH_REGROW = 16
 XB_NO_PIPELINE = 32
+XB_BATCH_EMIT_INSIDE = 64
 
 # Every symbol include/xb/xb_capi.h declares.
 EXPORTED_SYMBOLS = (
@@ -540,10 +541,12 @@ class Batch:
     graphs: list of HGraph; sources: list of int32 arrays; levels/parents: lists
     of device addresses (ints) of S_i x n_i int32 tables (levels may be None with
     want_levels=False).  Each run re-splits the workgroups between the graphs'
-    teams from the run's team clocks unless fixed_teams is set."""
+    teams from the run's team clocks unless fixed_teams is set.  A graph whose
+    sources fit one tile is emitted by standalone launches captured after the
+    persistent kernel, unless emit_inside is set (XB_BATCH_EMIT_INSIDE)."""
 
     def __init__(self, graphs, sources, levels, parents, want_levels=True, profile=False, fixed_teams=False,
-                 **tunables):
+                 emit_inside=False, **tunables):
         lib = load_engine()
         self._keep = [np.ascontiguousarray(s, dtype=np.int32) for s in sources]
         G = len(graphs)
@@ -552,7 +555,7 @@ class Batch:
         S = np.array([s.size for s in self._keep], dtype=np.int32)
         self._S = S
         flags = (XB_OUT_DEVICE | (0 if want_levels else XB_NO_LEVELS) | (XB_PROFILE if profile else 0)
-                 | (XB_FIXED_TEAMS if fixed_teams else 0))
+                 | (XB_FIXED_TEAMS if fixed_teams else 0) | (XB_BATCH_EMIT_INSIDE if emit_inside else 0))
         o = default_opts(flags=flags, **tunables)
         self._h = ctypes.c_void_p()
         lv = _ptr_array(levels) if want_levels else None
