@@ -205,7 +205,8 @@ def cpu_baseline(g, src, rounds: int):
 def bench_uneven(args) -> int:  # noqa: C901
     """The uneven batch: 32 R-MAT graphs (scales 12-22) with 1-512 sources each.
 
-    Batched = one persistent launch for the whole batch (hipGraph replay);
+    Batched = one replay of the captured batch: the persistent kernel (every
+    graph's traversal) and the emit launches of the one-tile teams after it;
     reported beside the host-driven single-graph engine run graph by graph and
     the plain per-source engine on every source."""
     import numpy as np
@@ -284,7 +285,8 @@ def bench_uneven(args) -> int:  # noqa: C901
         "plain_per_source": plain,
         "speedup_vs_plain": round(gteps / plain["gteps"], 2) if plain else None,
         "bit_exact_sums_vs_host_driven_and_plain": exact,
-        "roofline": {"bound": "hbm", "kernel": "k_bp_persistent", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "batch replay: k_bp_persistent + the emit launches after it",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_GBps": round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None,
