@@ -45,6 +45,18 @@
 // A pull level's three in-degree buckets and its hub pieces in one launch,
 // hub pieces and heavy buckets first (LiveJournal-shaped pull 11.3 -> 10.7 ms
 // per step, 3 alternating rounds on one box).
+// A level leaves the words of its frontier set instead of clearing them (a
+// launch per level over the queue's words, 0.73 ms per LiveJournal-shaped
+// step); the next tile clears both frontier arrays with two memsets.  Left
+// words are harmless: a word left in front[cur] at level d + 2 holds the
+// level-d frontier bits of a vertex u, and every out-neighbour of u has those
+// sources in its seen words since level d, so the pull's need masks and the
+// push's seen filter drop them.  The words of every vertex that gains bits
+// are overwritten, so the push, which reads the words of the queue's
+// vertices only, sees exact frontier words for the new bits.
+#ifndef XB_LAZY_FRONT_CLEAR
+#define XB_LAZY_FRONT_CLEAR 1
+#endif
 #ifndef XB_PULL_FUSED
 #define XB_PULL_FUSED 1
 #endif
@@ -150,6 +162,9 @@ struct Workspace {
   int32_t n = 0;
   xb::BPState bp{};
   bool bp_dirty = false;
+  // Frontier words left set by the last tile (levels do not clear the words of
+  // their frontier): cleared at the start of the next tile.
+  size_t front_dirty_words = 0;
   xb::Queue q[2]{};
   unsigned long long* ctrs = nullptr;  // q[0].ctr, q[1].ctr
   uint32_t* tcount = nullptr;
@@ -944,6 +959,13 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
     HIPCHK(hipMemsetAsync(s.tflag, 0, sizeof(uint32_t) * ((size_t)n + 1), w.st));
     prof.add(XB_K_MEMSET, 24.0 * cap + 4.0 * n);
     w.bp_dirty = false;
+    w.front_dirty_words = 0;
+  }
+  if (w.front_dirty_words > 0) {
+    HIPCHK(hipMemsetAsync(s.front[0], 0, sizeof(uint64_t) * w.front_dirty_words, w.st));
+    HIPCHK(hipMemsetAsync(s.front[1], 0, sizeof(uint64_t) * w.front_dirty_words, w.st));
+    prof.add(XB_K_MEMSET, 16.0 * w.front_dirty_words);
+    w.front_dirty_words = 0;
   }
   // Planes 0-3 (codes below 16) are cleared here; planes 4-7 only before the
   // first level whose code needs them (level 15), which most tiles never reach.
@@ -1060,14 +1082,17 @@ bool run_bp_tile(const xb_graph* g, Workspace& w, const BPOpts& o, const int32_t
       prof.end();
       ++pushes;
     }
-    prof.begin(XB_K_CLEAR);
-    xb::launch_bp_clear(w.st, T, s, cur, qc, qn);
-    prof.end();
-    prof.add(XB_K_CLEAR, (4.0 + 8.0 * T) * qn);
+    if (!XB_LAZY_FRONT_CLEAR) {
+      prof.begin(XB_K_CLEAR);
+      xb::launch_bp_clear(w.st, T, s, cur, qc, qn);
+      prof.end();
+      prof.add(XB_K_CLEAR, (4.0 + 8.0 * T) * qn);
+    }
     read_ctr(w, qx.ctr, &qn, &E);
     cur ^= 1;
     if (qn > 0) ++d;
   }
+  if (XB_LAZY_FRONT_CLEAR) w.front_dirty_words = std::max(w.front_dirty_words, nw);
   if (overflow) {
     w.bp_dirty = true;
     return false;
@@ -1680,6 +1705,9 @@ extern "C" int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int
 #ifndef XB_BATCH_EMIT_AFTER
 #define XB_BATCH_EMIT_AFTER 1
 #endif
+#ifndef XB_BATCH_EMIT_BRANCHES
+#define XB_BATCH_EMIT_BRANCHES 4
+#endif
 
 // The persistent kernel instantiates its tile loop for 1, 2, 4, 8 and 16
 // words, as the host-driven engine does.
@@ -1708,6 +1736,10 @@ struct xb_batch {
   bool profile = false;  // XB_PROFILE: phase clocks of the persistent kernel go into the stats
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Branches of the captured graph that run the emits after the kernel.
+  std::vector<hipStream_t> branch_streams;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
 
   template <class U>
   U* alloc(size_t count) {
@@ -1719,6 +1751,9 @@ struct xb_batch {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipStream_t x : branch_streams) (void)hipStreamDestroy(x);
     for (void* p : allocs) dfree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -1981,13 +2016,39 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
   // Capture the launch (control reset + persistent kernel) once, followed by
   // the emits of the one-tile teams: per team one launch for codes below 16
   // and one for deeper tiles, each gated on the tile's status and depth as
-  // the kernel leaves them in the job table.
+  // the kernel leaves them in the job table.  The emits go to up to
+  // XB_BATCH_EMIT_BRANCHES branches of the graph (forked after the kernel,
+  // joined at its end), largest tile first to the least loaded branch, so
+  // the emits of small graphs run beside each other.
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   xb::launch_bp_persistent(b->stream, b->grid, b->d_teams, b->n_teams, b->d_views, b->d_jobs,
                            (int32_t)b->h_jobs.size(), b->d_ctl, b->alpha, b->d_err);
-  for (int t = 0; t < nt; ++t) {
+  std::vector<int> emit_teams;
+  for (int t = 0; t < nt; ++t)
+    if (b->h_teams[t].emit_after) emit_teams.push_back(t);
+  auto emit_cells = [&](int t) {
+    const xb::TileJob& j = b->h_jobs[b->h_teams[t].job_begin];
+    return (double)j.cnt * (double)graphs[j.graph]->n;
+  };
+  std::stable_sort(emit_teams.begin(), emit_teams.end(), [&](int a, int c) { return emit_cells(a) > emit_cells(c); });
+  const int nbr = std::max(1, std::min<int>(XB_BATCH_EMIT_BRANCHES, (int)emit_teams.size()));
+  std::vector<hipStream_t> br((size_t)nbr, b->stream);
+  if (nbr > 1) {
+    HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(b->fork, b->stream));
+    for (int i = 1; i < nbr; ++i) {
+      hipStream_t sx = nullptr;
+      HIPCHK(hipStreamCreateWithFlags(&sx, hipStreamNonBlocking));
+      b->branch_streams.push_back(sx);
+      br[(size_t)i] = sx;
+      HIPCHK(hipStreamWaitEvent(sx, b->fork, 0));
+    }
+  }
+  std::vector<double> load((size_t)nbr, 0.0);
+  for (int t : emit_teams) {
+    const int bi = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[(size_t)bi] += emit_cells(t);
     const xb::Team& tm = b->h_teams[t];
-    if (!tm.emit_after) continue;
     const xb::TileJob& j = b->h_jobs[tm.job_begin];
     const xb_graph* g = graphs[j.graph];
     xb::BPState s = tm.s;
@@ -2001,10 +2062,18 @@ xb_batch* build_batch(xb_graph* const* graphs, int32_t G, const int32_t* const* 
     const int32_t* gate = &b->d_jobs[tm.job_begin].status;  // {status, levels}
     const bool sym = (g->flags & XB_GRAPH_DIRECTED) == 0;
     // Codes up to 15 (deepest level 14) in 4 planes, deeper tiles in 8.
-    xb::launch_bp_emit(b->stream, j.words, s, 0, ngrp, 4, j.cnt, j.src, g->d_oin_off, g->d_oin_col, g->d_oin_colo,
-                       g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 0, 14);
-    xb::launch_bp_emit(b->stream, j.words, s, 0, ngrp, 8, j.cnt, j.src, g->d_oin_off, g->d_oin_col, g->d_oin_colo,
-                       g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 15, 1 << 30);
+    xb::launch_bp_emit(br[(size_t)bi], j.words, s, 0, ngrp, 4, j.cnt, j.src, g->d_oin_off, g->d_oin_col,
+                       g->d_oin_colo, g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 0, 14);
+    xb::launch_bp_emit(br[(size_t)bi], j.words, s, 0, ngrp, 8, j.cnt, j.src, g->d_oin_off, g->d_oin_col,
+                       g->d_oin_colo, g->n, bo.cut_medium, j.lvl_out, j.par_out, xb::RowMap{}, sym, gate, 15,
+                       1 << 30);
+  }
+  for (int i = 1; i < nbr; ++i) {
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->join.push_back(ev);
+    HIPCHK(hipEventRecord(ev, br[(size_t)i]));
+    HIPCHK(hipStreamWaitEvent(b->stream, ev, 0));
   }
   HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
   HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
