@@ -86,6 +86,11 @@ constexpr int kChunk = kBlock * kPushR;  // edges per block chunk
 #ifndef XB_EMIT_GROUP_MAJOR
 #define XB_EMIT_GROUP_MAJOR 1
 #endif
+// The persistent kernel's levels leave their frontier words set; each tile
+// clears both frontier arrays at its start.
+#ifndef XB_PERSIST_LAZY_FRONT_CLEAR
+#define XB_PERSIST_LAZY_FRONT_CLEAR 1
+#endif
 constexpr int kPullU = XB_PULL_U;
 constexpr int kEmitU = XB_EMIT_U;
 constexpr int kStageCap = 1536;
@@ -1647,7 +1652,7 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   s.in_col_o = gv.in_col_o;
   const int32_t n = gv.n;
   const int64_t nw = (int64_t)n * T;
-  // Tile set-up: seen and planes cleared, heavy parents unresolved; fronts,
+  // Tile set-up: seen, planes and fronts cleared, heavy parents unresolved;
   // next words and touched flags are clean between tiles by construction.
   nl_fill_u64(g, s.seen, nw, 0ull);
   // The team's plane block holds both arrays, planes 4-7 after this tile's
@@ -1655,6 +1660,12 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
   // code needs them (level 15), which most tiles never reach.
   s.planes_hi = s.planes + nw * 4;
   nl_fill_u64(g, s.planes, nw * 4, 0ull);
+  if (XB_PERSIST_LAZY_FRONT_CLEAR) {
+    // Levels leave their frontier words set (masked by seen in pull and
+    // push); both frontier arrays are cleared here instead, once per tile.
+    nl_fill_u64(g, s.front[0], nw, 0ull);
+    nl_fill_u64(g, s.front[1], nw, 0ull);
+  }
   if (gv.n_heavy > 0) nl_fill_u32(g, s.hpar, (int64_t)gv.n_heavy * T * 64, 0xFFFFFFFFu);
   if (g.bid == 0 && threadIdx.x == 0) {
     ctl->ctr[1] = 0ull;
@@ -1710,7 +1721,7 @@ __device__ bool run_tile(Grid g, const Team& tm, const TeamSync& ts, const Graph
     }
     if (!team_barrier(ts, g, gerr, pull ? XB_K_PULL : XB_K_PUSH)) return false;
     nl_commit<T>(g, s, cur, code, gv.off, qx);
-    nl_clear<T>(g, s, cur, qc.v, qn);
+    if (!XB_PERSIST_LAZY_FRONT_CLEAR) nl_clear<T>(g, s, cur, qc.v, qn);
     if (!team_barrier(ts, g, gerr, XB_K_COMMIT)) return false;
     const unsigned long long c =
         __hip_atomic_load(&ctl->ctr[(d + 1) % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
