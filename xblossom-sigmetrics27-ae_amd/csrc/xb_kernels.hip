@@ -559,6 +559,12 @@ __device__ __forceinline__ void load_planes(const BPState& s, int64_t v, int t, 
   }
 }
 
+// Tile-state fills of the persistent kernel (gigabytes per tile, read back
+// from HBM by later phases) as streaming, non-temporal stores.
+#ifndef XB_FILL_NT
+#define XB_FILL_NT 1
+#endif
+
 // 16-byte stores over the 16-byte aligned body, 8-byte stores for the
 // unaligned first word and an odd last one.
 __device__ void fill_u64(Grid g, uint64_t* __restrict__ p, int64_t count, uint64_t val) {
@@ -567,7 +573,12 @@ __device__ void fill_u64(Grid g, uint64_t* __restrict__ p, int64_t count, uint64
   typedef uint64_t v2u __attribute__((ext_vector_type(2)));
   v2u* __restrict__ q = reinterpret_cast<v2u*>(p + head);
   const v2u vv = {val, val};
-  for (int64_t i = gthread(g); i < pairs; i += gnthreads(g)) q[i] = vv;
+  for (int64_t i = gthread(g); i < pairs; i += gnthreads(g)) {
+    if (XB_FILL_NT)
+      __builtin_nontemporal_store(vv, &q[i]);
+    else
+      q[i] = vv;
+  }
   if (gthread(g) == 0) {
     if (head) p[0] = val;
     if (head + 2 * pairs < count) p[count - 1] = val;
@@ -583,7 +594,12 @@ __device__ void fill_u32(Grid g, uint32_t* __restrict__ p, int64_t count, uint32
   typedef uint32_t v4u __attribute__((ext_vector_type(4)));
   v4u* __restrict__ q = reinterpret_cast<v4u*>(p + head);
   const v4u vv = {val, val, val, val};
-  for (int64_t i = gthread(g); i < quads; i += gnthreads(g)) q[i] = vv;
+  for (int64_t i = gthread(g); i < quads; i += gnthreads(g)) {
+    if (XB_FILL_NT)
+      __builtin_nontemporal_store(vv, &q[i]);
+    else
+      q[i] = vv;
+  }
   const int64_t t = gthread(g);
   if (t < head) p[t] = val;
   const int64_t tail0 = head + 4 * quads;
