@@ -224,6 +224,24 @@ def test_runs_are_bit_identical(rmat12):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+def test_frontier_words_left_between_tiles_and_runs(rmat12):
+    """Levels leave their frontier words set; the next tile clears both
+    frontier arrays over the words the last tile used.  One engine (one
+    workspace) runs tiles of 16, 2 and 1 words and back again, with partial
+    last tiles, pull-only and push-only direction choices, and each run's
+    tables equal the oracle's: a left word may not leak into the next tile or
+    run, whatever its width."""
+    g, hg = rmat12
+    eng = xb.XBlossomGpuEngine10(hg)
+    for k, (S, T, alpha) in enumerate([(1100, 16, 20), (130, 2, 1), (40, 1, 1_000_000_000),
+                                       (1024, 16, 1), (70, 16, 20), (200, 4, 20)]):
+        src = xb.sample_sources(g, S, seed=40 + k)
+        lv_ref, pa_ref, _ = oracle.msbfs(g.row_offsets, g.col_indices, src, threads=8)
+        lv, pa = eng.MultiSourceBFS(src, mode=xb.XB_MODE_BATCHED, words_per_tile=T, alpha=alpha)
+        assert np.array_equal(lv, lv_ref), f"run {k}: levels differ in {(lv != lv_ref).sum()} entries"
+        assert np.array_equal(pa, pa_ref), f"run {k}: parents differ in {(pa != pa_ref).sum()} entries"
+
+
 def test_source_out_of_range_rejected(rmat12):
     g, hg = rmat12
     eng = xb.XBlossomGpuEngine10(hg)
