@@ -45,6 +45,9 @@
 // A pull level's three in-degree buckets and its hub pieces in one launch,
 // hub pieces and heavy buckets first (LiveJournal-shaped pull 11.3 -> 10.7 ms
 // per step, 3 alternating rounds on one box).
+#ifndef XB_PULL_FUSED
+#define XB_PULL_FUSED 1
+#endif
 // A level leaves the words of its frontier set instead of clearing them (a
 // launch per level over the queue's words, 0.73 ms per LiveJournal-shaped
 // step); the next tile clears both frontier arrays with two memsets.  Left
@@ -56,9 +59,6 @@
 // vertices only, sees exact frontier words for the new bits.
 #ifndef XB_LAZY_FRONT_CLEAR
 #define XB_LAZY_FRONT_CLEAR 1
-#endif
-#ifndef XB_PULL_FUSED
-#define XB_PULL_FUSED 1
 #endif
 
 namespace {
