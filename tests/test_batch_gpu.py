@@ -201,3 +201,29 @@ def test_batch_emit_after_kernel_and_inside(emit_inside):
     b.close()
     for t in lv + pa:
         t.free()
+
+
+def test_batch_emit_gate_at_the_plane_boundary():
+    """The captured emits after the persistent kernel split at the depth the
+    kernel records: 4 planes up to level 14, 8 from level 15.  Paths whose
+    tiles end exactly at levels 13, 14, 15 and 16 (one graph each, one tile
+    each) in one batch come out equal to the oracle, on two replays."""
+    gs = [xb.Csr(*tg.path(k)) for k in (14, 15, 16, 17)]
+    hgs = [xb.HGraph(g.row_offsets, g.col_indices) for g in gs]
+    srcs = [np.array([0, g.n // 2], dtype=np.int32) for g in gs]
+    refs = [oracle.msbfs(g.row_offsets, g.col_indices, s, threads=2)[:2] for g, s in zip(gs, srcs)]
+    assert [int(r[0].max()) for r in refs] == [13, 14, 15, 16]
+    lv = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    pa = [xb.DeviceBuffer((s.size, g.nnodes)) for s, g in zip(srcs, hgs)]
+    b = xb.Batch(hgs, srcs, [t.ptr for t in lv], [t.ptr for t in pa], words_per_tile=1)
+    for rep in range(2):
+        for t in lv + pa:
+            t.upload(np.full(t.count, 7, dtype=np.int32))
+        st = b.run()
+        assert st.max_level == 16
+        for i, (lr, pr) in enumerate(refs):
+            assert np.array_equal(lv[i].numpy(), lr), f"replay {rep} path {i}: levels"
+            assert np.array_equal(pa[i].numpy(), pr), f"replay {rep} path {i}: parents"
+    b.close()
+    for t in lv + pa:
+        t.free()
