@@ -218,13 +218,16 @@ int32_t xb_msbfs_run_noalloc(xb_graph* g, const int32_t* sources, int32_t S,
                              int32_t* levels_dev, int32_t* parents_dev,
                              const xb_msbfs_opts* opts, xb_run_stats* stats);
 
-/* ---- batched multi-graph runs: one persistent, hipGraph-captured launch ----
+/* ---- batched multi-graph runs: one hipGraph-captured replay -------------
  *
  * A batch is G graphs, each with its own source set and device output
  * tables.  xb_batch_create plans it once: every graph gets a team of
  * workgroups sized by its estimated cost, its source tiles (64*T sources
  * each) run on that team with the level loop on the device, and the whole
- * batch is one persistent kernel launch captured in a hipGraph.
+ * batch is one persistent kernel launch captured in a hipGraph.  A graph
+ * whose sources fit one tile has its tables written by emit launches
+ * captured after the kernel in the same graph (XB_BATCH_EMIT_INSIDE keeps
+ * every emit inside the kernel).
  * xb_batch_run replays the graph; tiles deeper than the 8-bit level code are
  * re-run on the plain path afterwards.  After each run the workgroups are
  * re-split between the teams from their measured spans (unless the options
